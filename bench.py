@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark of the KPM moment engine on one MI355X.
+
+Workload (one "step" = one full moment computation over one batch of synthetic input):
+    Periodic honeycomb (graphene) supercell 2048 x 1024 cells, N = 4,194,304 sites, t = -2.8 eV,
+    Anderson onsite disorder uniform in [-0.5, 0.5] eV (numpy PCG64 seed 2), real fp64,
+    explicit energy range (-8.9, 8.9) eV, BFS-reordered H2 (the reference's default optimized
+    Hamiltonian), 4096 moments, 64 Rademacher vectors from the reference's mt19937 starter stream
+    computed as one 64-wide block SpMM recurrence (2048 fused Chebyshev steps).
+
+Inputs (H2 and the starter block) are resident in HBM before the timed region; the timed region
+runs K complete moment computations through the C-ABI (kpmx_moments_diagonal_run) and is
+bracketed by stream synchronisation.  value = GNNZ*moments/s = nnz * M * R / t / 1e9.
+
+roofline: the fused step kernel's algorithmic bytes per launch
+    B_step = nnz*(s + 4) + 4*(N + 1) + 3*N*B*s
+divided by its average launch duration (HIP events on the engine's stream around the timed
+region / number of step launches), against 8 TB/s.
+
+cpu_baseline: the oracle's restatement of the reference's default CPU compute (ELL, 4-wide fp64
+vector batches, threads over batches, interleaved BFS slices) on a bounded sample of the same
+workload: the same matrix and the same 64 vectors with fewer moments; reported in the same metric.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(n1, n2, disorder, dtype):
+    from pybinding_amd import generators
+    from pybinding_amd._native import load_module
+    native = load_module()
+    t0 = time.time()
+    h = generators.honeycomb_supercell(n1, n2, disorder=disorder, seed=2, dtype=dtype)
+    log("built H: n=%d nnz=%d (%.1fs)" % (h.shape[0], h.nnz, time.time() - t0))
+    t0 = time.time()
+    ip, ix, dv, rmap, borders, _, _, (a, b) = native.optimize(h.shape, h.indptr, h.indices, h.data, (-8.9, 8.9),
+                                                               True, [0], [0])
+    import scipy.sparse as sp
+    h2 = sp.csr_matrix((dv, ix, ip), shape=h.shape)
+    log("optimized H2 (scale a=%.6f b=%.3g, BFS reorder, %d slices) (%.1fs)" % (a, b, len(borders), time.time() - t0))
+    return h2, rmap, borders, native
+
+
+def starters(native, n, nvec, dtype, rmap):
+    t0 = time.time()
+    r0 = native.random_starters(n, nvec, np.dtype(dtype).name, rmap)
+    log("drew %d reference starters (%.1fs)" % (nvec, time.time() - t0))
+    return r0
+
+
+def step_bytes(n, nnz, vw, s):
+    return nnz * (s + 4) + 4 * (n + 1) + 3 * n * vw * s
+
+
+def cpu_baseline(h2, rmap, borders, nvec, target_seconds=15.0):
+    """Oracle restatement of the reference CPU compute on a bounded sample (fewer moments)."""
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ell = oracle.to_ell(h2)
+    n = h2.shape[0]
+    # calibrate: 10 moments (interleaved needs (M - 2) % 4 == 0)
+    m_cal = 10
+    _, t_cal = oracle.cpu_baseline_dos(ell, n, m_cal, nvec, threads, perm=rmap, borders=borders, interleaved=True)
+    per_moment = t_cal / m_cal
+    m = int(target_seconds / max(per_moment, 1e-9))
+    m = max(10, min(4098, m - (m - 2) % 4))
+    _, t = oracle.cpu_baseline_dos(ell, n, m, nvec, threads, perm=rmap, borders=borders, interleaved=True)
+    value = h2.nnz * m * nvec / t / 1e9
+    return {"value": value, "unit": "GNNZ*moments/s", "cores": threads, "kind": "port",
+            "sample": "same H2 (2048x1024 periodic honeycomb, BFS order) and the same %d vectors, %d moments "
+                      "instead of 4096 (%.1f s); restated reference CPU KPM (not the original binary)"
+                      % (nvec, m, t)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--moments", type=int, default=4096)
+    ap.add_argument("--vectors", type=int, default=64)
+    ap.add_argument("--cells", type=int, nargs=2, default=(2048, 1024))
+    ap.add_argument("--dtype", choices=["float64", "float32"], default="float64")
+    ap.add_argument("--format", choices=["CSR", "SELL"], default="CSR")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the single-vector (plain) comparison")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; the multi-GPU path (vectors split across GPUs "
+              "with one moment all-reduce) is described in DESIGN.md but not built.", file=sys.stderr)
+        sys.exit(2)
+
+    from pybinding_amd import abi
+    dtype = np.dtype(args.dtype)
+    s = dtype.itemsize
+    h2, rmap, borders, native = build_workload(args.cells[0], args.cells[1], 0.5, dtype)
+    n, nnz = h2.shape[0], h2.nnz
+    M, R = args.moments, args.vectors
+    r0 = starters(native, n, R, dtype, rmap)
+
+    ctx = abi.Ctx(0)
+    mat = ctx.mat(h2, fmt=1 if args.format == "SELL" else 0)
+    blk = ctx.block(r0)
+    ctx.synchronize()
+
+    for _ in range(args.warmup):
+        abi.moments_diagonal_run(mat, blk, M)
+    ctx.synchronize()
+    log("warmup done")
+
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        abi.moments_diagonal_run(mat, blk, M)
+    gpu_ms = ctx.timer_stop()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    mu = abi.moments_diagonal_fetch(mat, blk, M)
+    assert np.all(np.isfinite(mu)), "non-finite moments"
+    ms_per_step = wall * 1000 / args.steps
+    value = nnz * M * R / (wall / args.steps) / 1e9
+
+    launches = M // 2  # fused step launches per moment computation (one per two moments)
+    tiles = (R + 63) // 64
+    vw = min(64, 1 << (max(R, 1) - 1).bit_length())
+    step_ms = gpu_ms / (args.steps * launches * tiles)
+    bytes_per_launch = step_bytes(n, nnz, vw, s)
+    achieved = bytes_per_launch / (step_ms / 1000)
+    log("GPU: %.1f ms per moment computation, %.4f ms per step launch, %.1f GB/s algorithmic"
+        % (gpu_ms / args.steps, step_ms, achieved / 1e9))
+
+    extra = {}
+    if not args.no_plain:
+        # the plain single-vector path (one vector, SELL-64 kernel) on the same matrix, timed over
+        # 512 moments and reported in the same metric
+        mat1 = ctx.mat(h2, fmt=1)
+        blk1 = ctx.block(r0[:1])
+        abi.moments_diagonal_run(mat1, blk1, 512)
+        ctx.synchronize()
+        ctx.timer_start()
+        abi.moments_diagonal_run(mat1, blk1, 512)
+        ms1 = ctx.timer_stop()
+        v1 = nnz * 512 * 1 / (ms1 / 1000) / 1e9
+        b1 = step_bytes(n, nnz, 1, s)
+        extra["plain_single_vector"] = {"value": v1, "unit": "GNNZ*moments/s",
+                                        "step_ms": ms1 / 256, "achieved_GBps": b1 / (ms1 / 256 / 1000) / 1e9,
+                                        "block_over_plain": value / v1}
+        log("plain: %.1f GNNZ*moments/s; block/plain = %.2f" % (v1, value / v1))
+        blk1.close()
+        mat1.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(h2, rmap, borders, R)
+            log("cpu baseline: %.2f GNNZ*moments/s on %d threads" % (cpu["value"], cpu["cores"]))
+        except Exception as e:  # the baseline is reported, never required
+            log("cpu baseline failed: %s" % e)
+
+    blk.close()
+    mat.close()
+    ctx.close()
+
+    line = {
+        "metric": "KPM GNNZ*moments/s (block SpMM KPM DOS moments)",
+        "value": value,
+        "unit": "GNNZ*moments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if dtype == np.float64 else "f32",
+        "data": "synthetic (seeded disorder, reference mt19937 starters)",
+        "config": {"workload": "periodic honeycomb %dx%d cells (N=%d, nnz=%d), Anderson W=1 eV, %d moments, "
+                               "%d random vectors as one block SpMM" % (args.cells[0], args.cells[1], n, nnz, M, R),
+                   "matrix_format": args.format, "moments": M, "vectors": R},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/*
+ * kpmx -- C-ABI of the MI355X-native KPM Chebyshev-moment engine (libkpmx.so).
+ *
+ * This is the device boundary of the engine.  It replaces the reference's compute plugin
+ *   cpb::kpm::Compute::Interface::moments(MomentsRef, Starter const&, AlgorithmConfig const&,
+ *                                         OptimizedHamiltonian const&) const
+ *   (cppcore/include/kpm/Core.hpp:21-38, implemented on the CPU by DefaultCompute,
+ *    cppcore/include/kpm/default/Compute.hpp:9-27, cppcore/src/kpm/default/Compute.cpp:128-131)
+ * with one entry point per `MomentsRef` alternative (cppcore/include/kpm/Moments.hpp:125-126).
+ * The host C++ compute class (HipCompute) and the Python facade sit above it.
+ *
+ * Conventions
+ *   - Every function returns KPMX_OK (0) or a nonzero KPMX_E* code; kpmx_last_error() gives the
+ *     message of the last failure on the calling thread.  The host layer turns it into the
+ *     reference's exception (std::runtime_error / RuntimeError).
+ *   - Plain pointers and sizes only.  "host" buffers are caller-owned and read or written only
+ *     during the call.  Device buffers are owned by the library (kpmx_mat, kpmx_block, kpmx_ctx).
+ *   - Scalars: KPMX_F32 float, KPMX_F64 double, KPMX_C64 / KPMX_C128 complex stored as {re, im}
+ *     pairs (the layout of std::complex<float/double>).
+ *   - Matrices are given as CSR (int32 row pointers and column indices, row-major).  The matrix is
+ *     the already scaled H2 = (H - b) * 2/a of the reference's OptimizedHamiltonian
+ *     (cppcore/src/kpm/OptimizedHamiltonian.cpp:54-135), uploaded once.
+ *   - Starter blocks are given vector-major: num_vectors x n, each vector contiguous, already in
+ *     the matrix's (optimized) order.
+ *   - Moments follow the reference collectors: mu_0 is stored halved
+ *     (cppcore/src/kpm/default/collectors.cpp:5-106).
+ *   - One HIP stream per context; a context is used from one thread at a time (the reference's
+ *     KPM object has the same rule).
+ */
+#ifndef KPMX_H
+#define KPMX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KPMX_OK 0
+#define KPMX_EINVAL 1   /* invalid argument (size mismatch, bad scalar code, ...) */
+#define KPMX_EHIP 2     /* HIP runtime error */
+#define KPMX_ENOMEM 3   /* device allocation failed */
+
+#define KPMX_F32 0
+#define KPMX_F64 1
+#define KPMX_C64 2
+#define KPMX_C128 3
+
+#define KPMX_FMT_CSR 0  /* CSR, one wavefront per row group                                  */
+#define KPMX_FMT_SELL 1 /* sliced ELL with 64-row slices (single-vector recurrences)          */
+                        /* replaces kpm::MatrixFormat {CSR, ELL} (cppcore/include/kpm/Config.hpp:7) */
+
+#define KPMX_KUBO_FP64 0        /* mu_nm contraction in FP64 (reference arithmetic)             */
+#define KPMX_KUBO_FP32_SPLIT 1  /* FP32 MFMA with hi/lo split-precision compensation            */
+
+typedef struct kpmx_ctx kpmx_ctx;
+typedef struct kpmx_mat kpmx_mat;
+typedef struct kpmx_block kpmx_block;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* kpmx_last_error(void);
+/* ABI version (major * 100 + minor). */
+int kpmx_version(void);
+/* Number of visible HIP devices. */
+int kpmx_device_count(int* count);
+
+/* A device context: one GPU, one stream, reusable workspaces.
+ * Replaces the reference's DefaultCompute object (thread pool + SIMD batching),
+ * cppcore/src/kpm/default/Compute.cpp:121-126. */
+int kpmx_ctx_create(int device, kpmx_ctx** out);
+int kpmx_ctx_destroy(kpmx_ctx* ctx);
+int kpmx_ctx_synchronize(kpmx_ctx* ctx);
+/* HIP-event timer on the context's stream (no counterpart in the reference, which times on the
+ * host with Chrono, cppcore/include/utils/Chrono.hpp:32-71): start records an event, stop records
+ * a second one, waits for it and returns the elapsed milliseconds. */
+int kpmx_ctx_timer_start(kpmx_ctx* ctx);
+int kpmx_ctx_timer_stop(kpmx_ctx* ctx, double* ms);
+
+/* Upload a CSR matrix once (the scaled H2, or an operator such as the velocity).
+ * format: KPMX_FMT_CSR or KPMX_FMT_SELL (SELL is built on the device side from the CSR).
+ * Replaces OptimizedHamiltonian::matrix() as seen by the compute backend
+ * (cppcore/include/kpm/OptimizedHamiltonian.hpp:117,157; ELL conversion
+ *  cppcore/include/numeric/ellmatrix.hpp:64-82). */
+int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t nnz, const int32_t* indptr,
+                    const int32_t* indices, const void* values, kpmx_mat** out);
+int kpmx_mat_destroy(kpmx_mat* m);
+
+/* Upload starter vectors (host, vector-major num_vectors x n) and keep them resident.
+ * Replaces make_r0 for vector and block starters (cppcore/include/kpm/Starter.hpp:36-50). */
+int kpmx_block_upload(kpmx_ctx* ctx, int scalar, int64_t n, int64_t num_vectors, const void* r0, kpmx_block** out);
+int kpmx_block_destroy(kpmx_block* b);
+
+/* DiagonalMoments / BatchDiagonalMoments: mu_n = <r|T_n(H~)|r> for every vector of the block,
+ * two moments per Chebyshev step (cppcore/include/kpm/calc_moments.hpp:35-89,
+ * cppcore/src/kpm/default/collectors.cpp:5-34).  num_moments must be even.
+ * mu: host, num_vectors x num_moments, scalar type of h2. */
+int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu);
+
+/* Same recurrence, results kept as raw per-step sums on the device; for timing loops.
+ * Leaves the sums readable through kpmx_moments_diagonal_fetch. */
+int kpmx_moments_diagonal_run(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments);
+int kpmx_moments_diagonal_fetch(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu);
+
+/* GenericMoments: mu_n = <beta| op T_n(H~) |r0> for the first vector of r0
+ * (cppcore/src/kpm/default/collectors.cpp:36-58).  op may be null (op_nnz = 0): identity.
+ * beta: host, n scalars.  op: host CSR (n x n).  mu: host, num_moments scalars. */
+int kpmx_moments_generic(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, const void* beta,
+                         int64_t op_nnz, const int32_t* op_indptr, const int32_t* op_indices, const void* op_values,
+                         void* mu);
+
+/* MultiUnitMoments: mu[i][n] = (T_n(H~) r0)[dest[i]] (cppcore/src/kpm/default/collectors.cpp:60-75).
+ * mu: host, ndest x num_moments. */
+int kpmx_moments_multi_unit(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, int64_t ndest,
+                            const int32_t* dest, void* mu);
+
+/* DenseMatrixMoments: out[c][n][:] = op T_n(H~) r0_c (op may be null) for every vector c of r0
+ * (cppcore/src/kpm/default/collectors.cpp:77-106).  out: host, num_vectors x num_moments x n. */
+int kpmx_moments_dense(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, const kpmx_mat* op, void* out);
+
+/* Kubo-Bastin 2-D moments (cppcore/src/kpm/Core.cpp:119-150 with MomentMultiplication,
+ * cppcore/src/kpm/Moments.cpp:92-101):
+ *   mu_nm = (1/R) sum_r sum_i L_r[n,i] conj(R_r[m,i]),
+ *   L_r rows = T_n(H~) (vl r),  R_r rows = vr T_m(H~) r,  r = the R vectors of `r0`.
+ * mode: KPMX_KUBO_FP64 or KPMX_KUBO_FP32_SPLIT.  mu_nm: host, M x M, scalar type of h2. */
+int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr, const kpmx_block* r0,
+                      int64_t num_moments, int mode, void* mu_nm);
+
+/* Extreme eigenvalues of the (unscaled) Hamiltonian `h` by the Lanczos procedure, converged to
+ * `precision_percent` (cppcore/include/compute/lanczos.hpp:93-154, used by Bounds::compute_bounds,
+ * cppcore/src/kpm/Bounds.cpp:21-31).  The matrix-vector products and reductions run on the GPU,
+ * the small tridiagonal eigenproblem on the host. */
+int kpmx_lanczos_bounds(const kpmx_mat* h, double precision_percent, double* min_energy, double* max_energy,
+                        int* loops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KPMX_H */

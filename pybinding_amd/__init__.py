@@ -1,0 +1,13 @@
+"""pybinding_amd -- an MI355X-native kernel polynomial method engine.
+
+A drop-in compute backend for the reference's `pb.kpm(model, ...)` (pybinding/chebyshev.py):
+the host C++ core (`_kpmx`) calls hand-written HIP kernels (`libkpmx.so`, C-ABI in
+include/kpmx.h) for every Chebyshev moment.
+"""
+from .chebyshev import *  # noqa: F401,F403
+from .chebyshev import KPM, kpm, kpm_cuda, kpm_hip, jackson_kernel, lorentz_kernel, dirichlet_kernel  # noqa: F401
+from .results import Series, Deferred  # noqa: F401
+from . import generators  # noqa: F401
+from .model import Model  # noqa: F401
+
+__version__ = "0.1.0"

@@ -1,0 +1,171 @@
+"""ctypes binding of the engine's C-ABI (include/kpmx.h).
+
+This is the binding a caller uses to drive the engine directly through its drop-in boundary
+(plain pointers and sizes), without the C++ host module: the GPU parity tests and the benchmark
+use it, so that what they measure and check is the boundary itself.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(_PKG)
+HEADER = os.path.join(ROOT, "include", "kpmx.h")
+LIB = os.path.join(_PKG, "libkpmx.so")
+
+SCALAR = {np.dtype(np.float32): 0, np.dtype(np.float64): 1,
+          np.dtype(np.complex64): 2, np.dtype(np.complex128): 3}
+
+
+def declared_functions():
+    """Names of every function declared in include/kpmx.h."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kpmx_[a-z0-9_]+)\s*\(", text)))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = ctypes.CDLL(LIB)
+        _lib.kpmx_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("kpmx error %d: %s" % (rc, lib().kpmx_last_error().decode()))
+
+
+class Ctx:
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        _check(lib().kpmx_ctx_create(ctypes.c_int(device), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().kpmx_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def mat(self, csr, fmt=0):
+        return Mat(self, csr, fmt)
+
+    def block(self, r0):
+        return Block(self, r0)
+
+    def synchronize(self):
+        _check(lib().kpmx_ctx_synchronize(self.h))
+
+    def timer_start(self):
+        """Record a HIP event on the context's stream (start of a timed region)."""
+        _check(lib().kpmx_ctx_timer_start(self.h))
+
+    def timer_stop(self):
+        """Record the closing event, wait for it, and return the elapsed milliseconds."""
+        ms = ctypes.c_double()
+        _check(lib().kpmx_ctx_timer_stop(self.h, ctypes.byref(ms)))
+        return ms.value
+
+
+class Mat:
+    def __init__(self, ctx, csr, fmt=0):
+        csr = csr.tocsr()
+        self.dtype = np.dtype(csr.dtype)
+        self.n = csr.shape[0]
+        self.indptr = np.ascontiguousarray(csr.indptr, dtype=np.int32)
+        self.indices = np.ascontiguousarray(csr.indices, dtype=np.int32)
+        self.data = np.ascontiguousarray(csr.data)
+        self.h = ctypes.c_void_p()
+        _check(lib().kpmx_mat_upload(ctx.h, ctypes.c_int(SCALAR[self.dtype]), ctypes.c_int(fmt),
+                                     ctypes.c_int64(self.n), ctypes.c_int64(len(self.data)), _p(self.indptr),
+                                     _p(self.indices), _p(self.data), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().kpmx_mat_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+
+class Block:
+    def __init__(self, ctx, r0):
+        r0 = np.ascontiguousarray(np.atleast_2d(r0))
+        self.dtype = r0.dtype
+        self.nvec, self.n = r0.shape
+        self.h = ctypes.c_void_p()
+        _check(lib().kpmx_block_upload(ctx.h, ctypes.c_int(SCALAR[self.dtype]), ctypes.c_int64(self.n),
+                                       ctypes.c_int64(self.nvec), _p(r0), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().kpmx_block_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+
+def moments_diagonal(mat, block, num_moments):
+    out = np.empty((block.nvec, num_moments), dtype=mat.dtype)
+    _check(lib().kpmx_moments_diagonal(mat.h, block.h, ctypes.c_int64(num_moments), _p(out)))
+    return out
+
+
+def moments_diagonal_run(mat, block, num_moments):
+    """Run the diagonal recurrence and keep the raw sums on the device (no host transfer)."""
+    _check(lib().kpmx_moments_diagonal_run(mat.h, block.h, ctypes.c_int64(num_moments)))
+
+
+def moments_diagonal_fetch(mat, block, num_moments):
+    out = np.empty((block.nvec, num_moments), dtype=mat.dtype)
+    _check(lib().kpmx_moments_diagonal_fetch(mat.h, block.h, ctypes.c_int64(num_moments), _p(out)))
+    return out
+
+
+def moments_generic(mat, block, num_moments, beta, op=None):
+    out = np.empty(num_moments, dtype=mat.dtype)
+    beta = np.ascontiguousarray(beta, dtype=mat.dtype)
+    if op is not None:
+        op = op.tocsr()
+        ip = np.ascontiguousarray(op.indptr, dtype=np.int32)
+        ix = np.ascontiguousarray(op.indices, dtype=np.int32)
+        dv = np.ascontiguousarray(op.data, dtype=mat.dtype)
+        _check(lib().kpmx_moments_generic(mat.h, block.h, ctypes.c_int64(num_moments), _p(beta),
+                                          ctypes.c_int64(len(dv)), _p(ip), _p(ix), _p(dv), _p(out)))
+    else:
+        _check(lib().kpmx_moments_generic(mat.h, block.h, ctypes.c_int64(num_moments), _p(beta),
+                                          ctypes.c_int64(0), None, None, None, _p(out)))
+    return out
+
+
+def moments_multi_unit(mat, block, num_moments, dest):
+    dest = np.ascontiguousarray(dest, dtype=np.int32)
+    out = np.empty((len(dest), num_moments), dtype=mat.dtype)
+    _check(lib().kpmx_moments_multi_unit(mat.h, block.h, ctypes.c_int64(num_moments), ctypes.c_int64(len(dest)),
+                                         _p(dest), _p(out)))
+    return out
+
+
+def moments_dense(mat, block, num_moments, op=None):
+    out = np.empty((block.nvec, num_moments, mat.n), dtype=mat.dtype)
+    _check(lib().kpmx_moments_dense(mat.h, block.h, ctypes.c_int64(num_moments), op.h if op is not None else None,
+                                    _p(out)))
+    return out
+
+
+def kubo_moments(mat, vl, vr, block, num_moments, mode=0):
+    out = np.empty((num_moments, num_moments), dtype=mat.dtype)
+    _check(lib().kpmx_kubo_moments(mat.h, vl.h, vr.h, block.h, ctypes.c_int64(num_moments), ctypes.c_int(mode), _p(out)))
+    return out
+
+
+def lanczos_bounds(mat, precision_percent=0.002):
+    lo, hi, loops = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    _check(lib().kpmx_lanczos_bounds(mat.h, ctypes.c_double(precision_percent), ctypes.byref(lo), ctypes.byref(hi),
+                                     ctypes.byref(loops)))
+    return lo.value, hi.value, loops.value

@@ -1,0 +1,440 @@
+// Starters, moment containers, the compute interface with its HIP backend, the KPM Core and the
+// reconstruction functions (host side of the engine).
+#pragma once
+#include <mutex>
+#include <variant>
+
+#include "kpm_core.hpp"
+
+namespace kpmx_host {
+
+// ------------------------------------------------------------------------------------- Starter
+// constant / unit / random starters (cppcore/include/kpm/Starter.hpp:12-34,
+// cppcore/src/kpm/Starter.cpp:9-97).  Each make() returns one vector in optimized order.
+template <class T>
+struct Starter {
+    std::function<std::vector<T>()> make;
+    idx_t vector_size = 0;
+    mutable idx_t count = 0;
+};
+
+template <class T>
+Starter<T> constant_starter(OptimizedHamiltonian<T> const& oh, std::vector<cd> const& alpha) {
+    return {[&oh, alpha]() {
+                std::vector<T> r0(alpha.size());
+                for (size_t i = 0; i < alpha.size(); ++i) r0[i] = from_cd<T>(alpha[i]);
+                oh.reorder(r0);
+                return r0;
+            },
+            oh.size()};
+}
+
+template <class T>
+Starter<T> unit_starter(OptimizedHamiltonian<T> const& oh) {
+    auto sources = std::make_shared<std::vector<int32_t>>(oh.optimized_idx.src);
+    auto i = std::make_shared<size_t>(0);
+    const idx_t size = oh.size();
+    return {[sources, i, size]() {
+                std::vector<T> r0((size_t)size, T(0));
+                if (*i < sources->size()) {
+                    r0[(size_t)(*sources)[*i]] = T(1);
+                    ++*i;
+                }
+                return r0;
+            },
+            size};
+}
+
+// Random starter: std::mt19937 default seed, std::uniform_real_distribution<real_t>; real:
+// x < 0.5f ? -1 : 1, complex: exp(i 2 pi_f x); optional op applied before the reorder.
+template <class T>
+Starter<T> random_starter(OptimizedHamiltonian<T> const& oh, Csr<T> const* op = nullptr) {
+    using R = real_t<T>;
+    auto gen = std::make_shared<std::mt19937>();
+    std::shared_ptr<Csr<T>> opc = op ? std::make_shared<Csr<T>>(*op) : nullptr;
+    const idx_t size = oh.size();
+    return {[&oh, gen, opc, size]() {
+                std::vector<T> r0((size_t)size);
+                std::uniform_real_distribution<R> dist;
+                for (idx_t i = 0; i < size; ++i) {
+                    const R x = dist(*gen);
+                    if constexpr (is_cplx<T>) {
+                        const auto k = T{2 * PI_F * std::complex<float>(0, 1)};
+                        r0[(size_t)i] = std::exp(k * x);
+                    } else {
+                        r0[(size_t)i] = (x < 0.5f) ? R(-1.f) : R(1.f);
+                    }
+                }
+                if (opc) {
+                    std::vector<T> tmp((size_t)size, T(0));
+                    for (idx_t r = 0; r < size; ++r) {
+                        T acc = T(0);
+                        for (int32_t k = opc->indptr[r]; k < opc->indptr[r + 1]; ++k) acc += opc->data[k] * r0[opc->indices[k]];
+                        tmp[(size_t)r] = acc;
+                    }
+                    r0.swap(tmp);
+                }
+                oh.reorder(r0);
+                return r0;
+            },
+            size};
+}
+
+// ------------------------------------------------------------------------------------- moments
+template <class T>
+struct DiagonalMoments {
+    idx_t num_moments;
+    std::vector<T> data;
+};
+
+// collect(result, new_moments (M x cols, column-major), first_index, num_vectors)
+template <class T>
+struct BatchDiagonalMoments {
+    idx_t num_moments, num_vectors;
+    bool accumulate;  // true: BatchAccumulator (mean), false: BatchConcatenator (columns)
+    std::vector<T> data;  // accumulate: M; concatenate: M x num_vectors column-major
+};
+
+template <class T>
+struct GenericMoments {
+    idx_t num_moments;
+    std::vector<cd> const* alpha;
+    std::vector<cd> const* beta;
+    Csr<T> const* op;  // null: identity
+    std::vector<T> data;
+};
+
+template <class T>
+struct MultiUnitMoments {
+    idx_t num_moments;
+    Indices const* idx;
+    std::vector<std::vector<T>> data;
+};
+
+template <class T>
+using MomentsRef = std::variant<DiagonalMoments<T>*, BatchDiagonalMoments<T>*, GenericMoments<T>*, MultiUnitMoments<T>*>;
+
+template <class T>
+void apply_damping(std::vector<T>& m, Kernel const& k, idx_t rows, idx_t cols = 1) {
+    const auto g = k.damping_coefficients(rows);
+    for (idx_t c = 0; c < cols; ++c)
+        for (idx_t r = 0; r < rows; ++r) m[(size_t)(c * rows + r)] *= (real_t<T>)g[(size_t)r];
+}
+
+// ------------------------------------------------------------------------------------- compute
+using ProgressCallback = std::function<void(idx_t, idx_t)>;
+
+// One GPU context per device, shared by every Core on that device.
+class Device {
+public:
+    explicit Device(int device) {
+        check(kpmx_ctx_create(device, &ctx_));
+    }
+    ~Device() { kpmx_ctx_destroy(ctx_); }
+    kpmx_ctx* get() const { return ctx_; }
+    static std::shared_ptr<Device> instance(int device) {
+        static std::mutex mu;
+        static std::vector<std::weak_ptr<Device>> devices(64);
+        std::lock_guard<std::mutex> lk(mu);
+        if (device < 0 || device >= 64) throw std::invalid_argument("invalid device index");
+        auto d = devices[(size_t)device].lock();
+        if (!d) {
+            d = std::make_shared<Device>(device);
+            devices[(size_t)device] = d;
+        }
+        return d;
+    }
+
+private:
+    kpmx_ctx* ctx_ = nullptr;
+};
+
+struct MatHandle {
+    kpmx_mat* m = nullptr;
+    MatHandle() = default;
+    MatHandle(MatHandle const&) = delete;
+    MatHandle& operator=(MatHandle const&) = delete;
+    ~MatHandle() { if (m) kpmx_mat_destroy(m); }
+};
+struct BlockHandle {
+    kpmx_block* b = nullptr;
+    BlockHandle() = default;
+    BlockHandle(BlockHandle const&) = delete;
+    BlockHandle& operator=(BlockHandle const&) = delete;
+    ~BlockHandle() { if (b) kpmx_block_destroy(b); }
+};
+
+template <class T>
+std::unique_ptr<MatHandle> upload(kpmx_ctx* ctx, Csr<T> const& m, int format) {
+    auto h = std::make_unique<MatHandle>();
+    check(kpmx_mat_upload(ctx, scalar_code<T>(), format, m.rows, m.nnz(), m.indptr.data(), m.indices.data(),
+                          m.data.data(), &h->m));
+    return h;
+}
+
+template <class T>
+std::unique_ptr<BlockHandle> upload_block(kpmx_ctx* ctx, idx_t n, idx_t nvec, std::vector<T> const& r0) {
+    auto h = std::make_unique<BlockHandle>();
+    check(kpmx_block_upload(ctx, scalar_code<T>(), n, nvec, r0.data(), &h->b));
+    return h;
+}
+
+// HipCompute: the compute plugin (Compute::Interface::moments, cppcore/include/kpm/Core.hpp:21-38)
+// implemented with the engine's C-ABI.  The scaled H2 is uploaded once per optimization and
+// reused by every later call.
+template <class T>
+class HipCompute {
+public:
+    HipCompute(int device, ProgressCallback progress) : dev_(Device::instance(device)), progress_(std::move(progress)) {}
+
+    kpmx_ctx* ctx() const { return dev_->get(); }
+
+    kpmx_mat* matrix(OptimizedHamiltonian<T> const& oh, int format) {
+        if (!h2_ || version_ != oh.version || format_ != format || oh_ != &oh) {
+            h2_.reset();
+            h2_ = upload(ctx(), oh.h2, format);
+            version_ = oh.version;
+            format_ = format;
+            oh_ = &oh;
+        }
+        return h2_->m;
+    }
+
+    void moments(MomentsRef<T> m, Starter<T> const& s, AlgorithmConfig const&, OptimizedHamiltonian<T> const& oh,
+                 int format) {
+        kpmx_mat* h2 = matrix(oh, format);
+        const idx_t n = oh.size();
+        std::visit(
+            [&](auto* mm) {
+                using M = std::decay_t<decltype(*mm)>;
+                if constexpr (std::is_same<M, DiagonalMoments<T>>::value) {
+                    auto r0 = s.make();
+                    ++s.count;
+                    auto blk = upload_block(ctx(), n, 1, r0);
+                    mm->data.assign((size_t)mm->num_moments, T(0));
+                    check(kpmx_moments_diagonal(h2, blk->b, mm->num_moments, mm->data.data()));
+                } else if constexpr (std::is_same<M, BatchDiagonalMoments<T>>::value) {
+                    const idx_t M_ = mm->num_moments, R = mm->num_vectors;
+                    report(-1, R);
+                    if (mm->accumulate) mm->data.assign((size_t)M_, T(0));
+                    else mm->data.assign((size_t)(M_ * R), T(0));
+                    // vectors in groups of up to 64 (one device block per group)
+                    std::vector<T> acc((size_t)M_, T(0));
+                    for (idx_t v0 = 0; v0 < R; v0 += 64) {
+                        const idx_t nv = std::min<idx_t>(64, R - v0);
+                        std::vector<T> r0((size_t)(n * nv));
+                        for (idx_t v = 0; v < nv; ++v) {
+                            auto col = s.make();
+                            ++s.count;
+                            std::copy(col.begin(), col.end(), r0.begin() + v * n);
+                        }
+                        auto blk = upload_block(ctx(), n, nv, r0);
+                        std::vector<T> mu((size_t)(nv * M_));
+                        check(kpmx_moments_diagonal(h2, blk->b, M_, mu.data()));
+                        for (idx_t v = 0; v < nv; ++v) {
+                            if (mm->accumulate) {
+                                for (idx_t k = 0; k < M_; ++k) acc[(size_t)k] += mu[(size_t)(v * M_ + k)];
+                            } else {
+                                std::copy(mu.begin() + v * M_, mu.begin() + (v + 1) * M_, mm->data.begin() + (v0 + v) * M_);
+                            }
+                        }
+                        report(nv, R);
+                    }
+                    if (mm->accumulate) {
+                        const real_t<T> div = (R != 1) ? (real_t<T>)R : real_t<T>(1);
+                        for (idx_t k = 0; k < M_; ++k) mm->data[(size_t)k] = acc[(size_t)k] / div;
+                    }
+                    report(R, R);
+                } else if constexpr (std::is_same<M, GenericMoments<T>>::value) {
+                    auto r0 = s.make();
+                    ++s.count;
+                    auto blk = upload_block(ctx(), n, 1, r0);
+                    // beta defaults to alpha; both reordered like the starter (collectors.cpp:36-47)
+                    std::vector<cd> const& b = (mm->beta && !mm->beta->empty()) ? *mm->beta : *mm->alpha;
+                    std::vector<T> beta(b.size());
+                    for (size_t i = 0; i < b.size(); ++i) beta[i] = from_cd<T>(b[i]);
+                    oh.reorder(beta);
+                    mm->data.assign((size_t)mm->num_moments, T(0));
+                    if (mm->op && mm->op->nnz() > 0) {
+                        Csr<T> op = oh.reorder(*mm->op);
+                        check(kpmx_moments_generic(h2, blk->b, mm->num_moments, beta.data(), op.nnz(), op.indptr.data(),
+                                                   op.indices.data(), op.data.data(), mm->data.data()));
+                    } else {
+                        check(kpmx_moments_generic(h2, blk->b, mm->num_moments, beta.data(), 0, nullptr, nullptr, nullptr,
+                                                   mm->data.data()));
+                    }
+                } else {
+                    auto r0 = s.make();
+                    ++s.count;
+                    auto blk = upload_block(ctx(), n, 1, r0);
+                    const auto& dest = mm->idx->dest;
+                    std::vector<T> out(dest.size() * (size_t)mm->num_moments);
+                    check(kpmx_moments_multi_unit(h2, blk->b, mm->num_moments, (idx_t)dest.size(), dest.data(), out.data()));
+                    mm->data.assign(dest.size(), std::vector<T>());
+                    for (size_t i = 0; i < dest.size(); ++i)
+                        mm->data[i].assign(out.begin() + i * mm->num_moments, out.begin() + (i + 1) * mm->num_moments);
+                }
+            },
+            m);
+    }
+
+    // Kubo-Bastin mu_nm on the device (L/R panels + contraction never leave the GPU).
+    std::vector<T> kubo(OptimizedHamiltonian<T> const& oh, int format, Csr<T> const& vl, Csr<T> const& vr,
+                        idx_t num_moments, idx_t num_random, int mode) {
+        kpmx_mat* h2 = matrix(oh, format);
+        const idx_t n = oh.size();
+        auto vlr = upload(ctx(), oh.reorder(vl), KPMX_FMT_CSR);
+        auto vrr = upload(ctx(), oh.reorder(vr), KPMX_FMT_CSR);
+        // Both starters are default-seeded random starters, so they draw the same vectors
+        // (cppcore/src/kpm/Core.cpp:132-137); vl is applied on the device.
+        auto st = random_starter(oh);
+        std::vector<T> r0((size_t)(n * num_random));
+        for (idx_t v = 0; v < num_random; ++v) {
+            auto col = st.make();
+            std::copy(col.begin(), col.end(), r0.begin() + v * n);
+        }
+        auto blk = upload_block(ctx(), n, num_random, r0);
+        std::vector<T> mu((size_t)(num_moments * num_moments));
+        check(kpmx_kubo_moments(h2, vlr->m, vrr->m, blk->b, num_moments, mode, mu.data()));
+        return mu;
+    }
+
+    void report(idx_t delta, idx_t total) {
+        if (progress_) progress_(delta, total);
+    }
+
+private:
+    std::shared_ptr<Device> dev_;
+    ProgressCallback progress_;
+    std::unique_ptr<MatHandle> h2_;
+    uint64_t version_ = 0;
+    int format_ = -1;
+    OptimizedHamiltonian<T> const* oh_ = nullptr;
+};
+
+// ------------------------------------------------------------------------------------- reconstruct
+// SpectralDensity (cppcore/include/kpm/reconstruct.hpp:16-54): out[e * cols + c]
+template <class T>
+std::vector<double> spectral_density(std::vector<T> const& moments, idx_t M, idx_t cols, std::vector<double> const& energy,
+                                     Scale<> s) {
+    using R = real_t<T>;
+    const Scale<R> scale(s);
+    const R k = R(2 / PI_F) / scale.a;
+    std::vector<double> out(energy.size() * (size_t)cols);
+    std::vector<R> cosn((size_t)M);
+    for (size_t i = 0; i < energy.size(); ++i) {
+        const R E = scale((R)energy[i]);
+        const R ac = std::acos(E);
+        for (idx_t n = 0; n < M; ++n) cosn[(size_t)n] = std::cos((R)n * ac);
+        const R pref = k / std::sqrt(1 - E * E);
+        for (idx_t c = 0; c < cols; ++c) {
+            R sum = 0;
+            for (idx_t n = 0; n < M; ++n) sum += (R)re(moments[(size_t)(c * M + n)]) * cosn[(size_t)n];
+            out[i * (size_t)cols + (size_t)c] = (double)(pref * sum);
+        }
+    }
+    return out;
+}
+
+// GreensFunction (reconstruct.hpp:59-85)
+template <class T>
+std::vector<cd> greens_function(std::vector<T> const& moments, std::vector<double> const& energy, Scale<> s) {
+    using R = real_t<T>;
+    using C = std::complex<R>;
+    const Scale<R> scale(s);
+    const C i1(0, 1);
+    const C k = -R(2) * i1 / scale.a;
+    const idx_t M = (idx_t)moments.size();
+    std::vector<cd> out(energy.size());
+    for (size_t i = 0; i < energy.size(); ++i) {
+        const R E = scale((R)energy[i]);
+        const R ac = std::acos(E);
+        C sum(0, 0);
+        for (idx_t n = 0; n < M; ++n) sum += C(moments[(size_t)n]) * std::exp(-i1 * (R)n * ac);
+        const C g = k / std::sqrt(1 - E * E) * sum;
+        out[i] = cd(g.real(), g.imag());
+    }
+    return out;
+}
+
+// KuboBastin (reconstruct.hpp:90-143); mu is M x M row-major.
+template <class T>
+std::vector<cd> kubo_bastin(std::vector<T> const& mu, idx_t M, std::vector<double> const& chemical_potential,
+                            std::vector<double> const& energy_samples, double temperature, Scale<> s) {
+    using R = real_t<T>;
+    using C = std::complex<R>;
+    const Scale<R> scale(s);
+    const R inv_kbt = static_cast<R>(scale.a / (KB_F * temperature));
+    const C i1(0, 1);
+    std::vector<R> es(energy_samples.size());
+    for (size_t i = 0; i < es.size(); ++i) es[i] = scale((R)energy_samples[i]);
+    std::vector<C> sum_nm(es.size());
+    std::vector<C> gp((size_t)(M * M));
+    for (size_t q = 0; q < es.size(); ++q) {
+        const R en = es[q];
+        const R ac = std::acos(en);
+        const R sq = std::sqrt(R(1) - en * en);
+        for (idx_t n = 0; n < M; ++n) {
+            const R tm = std::cos(ac * (R)n);
+            for (idx_t m = 0; m < M; ++m) {
+                const C sqrt_n = en - i1 * (R)m * sq;
+                const C exp_n = std::exp(i1 * ac * (R)m);
+                gp[(size_t)(n * M + m)] = sqrt_n * exp_n * tm;
+            }
+        }
+        C acc(0, 0);
+        for (idx_t n = 0; n < M; ++n)
+            for (idx_t m = 0; m < M; ++m) {
+                const C gamma = gp[(size_t)(n * M + m)] + std::conj(gp[(size_t)(m * M + n)]);
+                acc += C(mu[(size_t)(n * M + m)]) * gamma;
+            }
+        const R k = R(1) / ((R(1) - en * en) * (R(1) - en * en));
+        sum_nm[q] = k * acc;
+    }
+    R emin = *std::min_element(es.begin(), es.end()), emax = *std::max_element(es.begin(), es.end());
+    const R coeff = (emax - emin) / static_cast<R>(2 * es.size());
+    const C prefix = C(4) / (scale.a * scale.a);
+    std::vector<cd> out(chemical_potential.size());
+    for (size_t j = 0; j < chemical_potential.size(); ++j) {
+        const R mu_ = scale((R)chemical_potential[j]);
+        C total(0, 0);
+        for (size_t q = 0; q < es.size(); ++q) {
+            const R fd = R(1) / (R(1) + std::exp((es[q] - mu_) * inv_kbt));
+            total += fd * sum_nm[q];
+        }
+        const R fd0 = R(1) / (R(1) + std::exp((es.front() - mu_) * inv_kbt));
+        const R fd1 = R(1) / (R(1) + std::exp((es.back() - mu_) * inv_kbt));
+        const C integ = coeff * (R(2) * total - fd0 * sum_nm.front() - fd1 * sum_nm.back());
+        const C v = prefix * integ;
+        out[j] = cd(v.real(), v.imag());
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------------------------- Core
+// kpm::Core (cppcore/include/kpm/Core.hpp:40-89, cppcore/src/kpm/Core.cpp:17-156)
+class CoreBase {
+public:
+    virtual ~CoreBase() = default;
+    virtual Scale<> scaling_factors() = 0;
+    virtual std::vector<cd> moments(idx_t num_moments, std::vector<cd> const& alpha, std::vector<cd> const& beta,
+                                    Csr<cd> const& op) = 0;
+    virtual std::vector<double> ldos(std::vector<idx_t> const& idx, std::vector<double> const& energy, double broadening,
+                                     idx_t* cols_out) = 0;
+    virtual std::vector<double> dos(std::vector<double> const& energy, double broadening, idx_t num_random) = 0;
+    virtual std::vector<std::vector<cd>> greens_vector(idx_t row, std::vector<idx_t> const& cols,
+                                                       std::vector<double> const& energy, double broadening) = 0;
+    virtual std::vector<cd> conductivity(std::vector<float> const& left, std::vector<float> const& right,
+                                         std::vector<double> const& chemical_potential, double broadening,
+                                         double temperature, idx_t num_random, idx_t num_points) = 0;
+    virtual std::vector<cd> dos_moments(idx_t num_moments, idx_t num_random) = 0;
+    virtual Stats const& stats() const = 0;
+    virtual Config const& config() const = 0;
+    virtual idx_t size() const = 0;
+    virtual bool is_complex() const = 0;
+    virtual double min_energy() = 0;
+    virtual double max_energy() = 0;
+    virtual int lanczos_loops() const = 0;
+};
+
+}  // namespace kpmx_host

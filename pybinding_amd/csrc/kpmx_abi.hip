@@ -1,0 +1,1002 @@
+// C-ABI of the KPM engine (declared in include/kpmx.h): contexts, resident matrices and starter
+// blocks, and the recurrence drivers for each kind of moments.  All heavy work runs in the
+// kernels of kpmx_kernels.hip / kpmx_gemm.hip on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <random>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/kpmx.h"
+#include "kpmx_internal.hpp"
+
+using namespace kpmx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define KPMX_HIP(expr)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) throw Error{KPMX_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define KPMX_REQUIRE(cond, msg)                          \
+    do {                                                 \
+        if (!(cond)) throw Error{KPMX_EINVAL, (msg)};    \
+    } while (0)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return KPMX_OK;
+    } catch (Error const& e) {
+        g_last_error = e.msg;
+        return e.code;
+    } catch (std::bad_alloc const&) {
+        g_last_error = "host allocation failed";
+        return KPMX_ENOMEM;
+    } catch (std::exception const& e) {
+        g_last_error = e.what();
+        return KPMX_EINVAL;
+    }
+}
+
+void* dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error{KPMX_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e)};
+    }
+    return p;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// Grow-only device workspace.
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    void* get(size_t need) {
+        if (need > bytes) {
+            // the old buffer may still be read by queued kernels: drain the device first
+            if (ptr) (void)hipDeviceSynchronize();
+            dev_free(ptr);
+            ptr = nullptr;
+            bytes = 0;
+            ptr = dev_alloc(need);
+            bytes = need;
+        }
+        return ptr;
+    }
+    ~Workspace() { dev_free(ptr); }
+};
+
+bool valid_scalar(int s) { return s >= 0 && s <= 3; }
+bool is_complex(int s) { return s >= 2; }
+
+}  // namespace
+
+struct kpmx_ctx {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t stream = nullptr;
+    Workspace partials, sums, work0, work1, aux0, aux1, aux2, panel_l, panel_r;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+};
+
+struct kpmx_mat {
+    kpmx_ctx* ctx = nullptr;
+    int scalar = 1;
+    int format = KPMX_FMT_CSR;
+    int64_t rows = 0, nnz = 0;
+    int32_t* indptr = nullptr;
+    int32_t* indices = nullptr;
+    void* vals = nullptr;
+    // SELL-64 copy (format == SELL)
+    int64_t nslices = 0;
+    int64_t* slice_ptr = nullptr;
+    int32_t* slice_width = nullptr;
+    int32_t* sell_indices = nullptr;
+    void* sell_vals = nullptr;
+    ~kpmx_mat() {
+        dev_free(indptr);
+        dev_free(indices);
+        dev_free(vals);
+        dev_free(slice_ptr);
+        dev_free(slice_width);
+        dev_free(sell_indices);
+        dev_free(sell_vals);
+    }
+};
+
+struct Tile {
+    int64_t c0 = 0, cols = 0;
+    int vw = 1;
+    void* data = nullptr;   // n x vw row-major pristine starters
+};
+
+struct kpmx_block {
+    kpmx_ctx* ctx = nullptr;
+    int scalar = 1;
+    int64_t n = 0, nvec = 0;
+    std::vector<Tile> tiles;
+    // raw sums of the last kpmx_moments_diagonal_run, per tile: (M/2) x (4 vw) doubles
+    std::vector<void*> sums;
+    int64_t sums_moments = 0;
+    ~kpmx_block() {
+        for (auto& t : tiles) dev_free(t.data);
+        for (auto* s : sums) dev_free(s);
+    }
+};
+
+namespace {
+
+int pick_vw(int64_t remaining) {
+    int vw = 1;
+    while (vw < remaining && vw < 64) vw <<= 1;
+    return vw;
+}
+
+int grid_size(kpmx_ctx* ctx, int64_t num_tiles) {
+    int64_t max_wg = (int64_t)ctx->num_cu * 8;
+    int64_t g = std::min<int64_t>(num_tiles, max_wg);
+    if (g >= 8) g -= g % 8;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+StepLaunch base_launch(const kpmx_mat* h2, int vw, bool use_sell) {
+    StepLaunch L;
+    L.scalar = h2->scalar;
+    L.vw = vw;
+    L.n = h2->rows;
+    if (use_sell && vw == 1 && h2->format == KPMX_FMT_SELL) {
+        L.format = FORMAT_SELL;
+        L.indices = h2->sell_indices;
+        L.vals = h2->sell_vals;
+        L.slice_ptr = h2->slice_ptr;
+        L.slice_width = h2->slice_width;
+    } else {
+        L.format = FORMAT_CSR;
+        L.indptr = h2->indptr;
+        L.indices = h2->indices;
+        L.vals = h2->vals;
+    }
+    L.num_wg = grid_size(h2->ctx, (h2->rows + step_rows_per_tile(L.format, vw) - 1) / step_rows_per_tile(L.format, vw));
+    return L;
+}
+
+void launch(const StepLaunch& L, hipStream_t s) {
+    if (launch_step(L, s) != 0) throw Error{KPMX_EINVAL, "unsupported step configuration"};
+    KPMX_HIP(hipGetLastError());
+}
+
+// Run the Chebyshev recurrence r_{k+1} = H2 r_k - r_{k-1} over one tile for `steps` launches
+// (k = 0 is the first step r_1 = 0.5 H2 r_0).  `per_step` customises each launch (collection
+// pointers) and the partials of reduction modes are summed into `sums` (steps x 4vw doubles).
+template <class PerStep, class PostStep>
+void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, const void* w, double* sums,
+                    PerStep per_step, PostStep post_step) {
+    kpmx_ctx* ctx = h2->ctx;
+    const int64_t n = h2->rows;
+    const size_t sb = scalar_bytes(h2->scalar);
+    void* W0 = ctx->work0.get((size_t)n * t.vw * sb);
+    void* W1 = ctx->work1.get((size_t)n * t.vw * sb);
+
+    StepLaunch L = base_launch(h2, t.vw, true);
+    L.mode = mode;
+    L.w = w;
+    const int width = 4 * t.vw;
+    int64_t chunk = 1;
+    double* partials = nullptr;
+    if (mode != MODE_NONE) {
+        const size_t per_step_bytes = (size_t)L.num_wg * width * sizeof(double);
+        const size_t budget = (size_t)256 << 20;
+        chunk = std::max<int64_t>(1, std::min<int64_t>(steps, (int64_t)(budget / per_step_bytes)));
+        partials = static_cast<double*>(ctx->partials.get(per_step_bytes * chunk));
+    }
+    int64_t chunk_start = 0;
+    for (int64_t k = 0; k < steps; ++k) {
+        if (k == 0) {
+            L.x = t.data;
+            L.yin = nullptr;
+            L.yout = W1;
+            L.hscale = 0.5;
+            L.sub_y = 0;
+        } else {
+            const bool odd = (k % 2) == 1;
+            L.x = odd ? W1 : W0;
+            L.yout = odd ? W0 : W1;
+            L.yin = (k == 1) ? t.data : L.yout;
+            L.hscale = 1.0;
+            L.sub_y = 1;
+        }
+        if (partials) L.partials = partials + (size_t)(k - chunk_start) * L.num_wg * width;
+        per_step(L, k);
+        launch(L, ctx->stream);
+        post_step(L, k);
+        if (partials && (k - chunk_start + 1 == chunk || k == steps - 1)) {
+            launch_reduce_partials(partials, L.num_wg, width, k - chunk_start + 1, sums + (size_t)chunk_start * width,
+                                   ctx->stream);
+            KPMX_HIP(hipGetLastError());
+            chunk_start = k + 1;
+        }
+    }
+}
+
+// Host-side collector for the diagonal recurrence (collectors.cpp:5-34), from raw sums.
+// Step k = 0 gives A = |r0|^2 and B = sum conj(r1) r0 = r1.dot(r0); step k >= 1 gives
+// A = |r_k|^2 and B = sum conj(r_{k+1}) r_k, so mu_2k = 2 (A - m0) and mu_2k+1 = 2 B - m1
+// with m0 = A_0 / 2 and m1 = B_0 (the doubling identities T_2k = 2 T_k^2 - T_0 and
+// T_2k+1 = 2 T_k+1 T_k - T_1).
+template <class R>
+void diag_collect(const std::vector<double>& s, int vw, int64_t cols, int64_t M, bool cplx, R* mu_base,
+                  int64_t c0) {
+    const int width = 4 * vw;
+    for (int64_t c = 0; c < cols; ++c) {
+        R* mu = mu_base + (c0 + c) * M * (cplx ? 2 : 1);
+        auto put = [&](int64_t m, double re, double im) {
+            if (cplx) {
+                mu[2 * m] = (R)re;
+                mu[2 * m + 1] = (R)im;
+            } else {
+                mu[m] = (R)re;
+            }
+        };
+        const double m0 = 0.5 * s[0 * width + 0 * vw + c];
+        const double m1r = s[0 * width + 2 * vw + c], m1i = s[0 * width + 3 * vw + c];
+        put(0, m0, 0.0);
+        put(1, m1r, m1i);
+        for (int64_t k = 1; k < M / 2; ++k) {
+            const double a = s[k * width + 0 * vw + c];
+            const double br = s[k * width + 2 * vw + c], bi = s[k * width + 3 * vw + c];
+            put(2 * k, 2.0 * (a - m0), 0.0);
+            put(2 * k + 1, 2.0 * br - m1r, 2.0 * bi - m1i);
+        }
+    }
+}
+
+void check_pair(const kpmx_mat* h2, const kpmx_block* r0) {
+    KPMX_REQUIRE(h2 && r0, "null matrix or starter block");
+    KPMX_REQUIRE(h2->ctx == r0->ctx, "matrix and starter block belong to different contexts");
+    KPMX_REQUIRE(h2->scalar == r0->scalar, "matrix and starter block have different scalar types");
+    KPMX_REQUIRE(h2->rows == r0->n, "Size mismatch between the matrix and the starter vectors");
+}
+
+void diag_run(const kpmx_mat* h2, const kpmx_block* r0c, int64_t M) {
+    auto* r0 = const_cast<kpmx_block*>(r0c);
+    check_pair(h2, r0);
+    KPMX_REQUIRE(M >= 2 && M % 2 == 0, "num_moments must be even and >= 2");
+    const int64_t steps = M / 2;
+    if (r0->sums_moments != M) {
+        if (!r0->sums.empty()) KPMX_HIP(hipStreamSynchronize(h2->ctx->stream));
+        for (auto* s : r0->sums) dev_free(s);
+        r0->sums.clear();
+        for (auto& t : r0->tiles) r0->sums.push_back(dev_alloc((size_t)steps * 4 * t.vw * sizeof(double)));
+        r0->sums_moments = M;
+    }
+    for (size_t i = 0; i < r0->tiles.size(); ++i) {
+        run_recurrence(h2, r0->tiles[i], MODE_DIAG, steps, nullptr, static_cast<double*>(r0->sums[i]),
+                       [](StepLaunch&, int64_t) {}, [](StepLaunch const&, int64_t) {});
+    }
+}
+
+void diag_fetch(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, void* mu) {
+    check_pair(h2, r0);
+    KPMX_REQUIRE(r0->sums_moments == M, "no finished run for this number of moments");
+    kpmx_ctx* ctx = h2->ctx;
+    const bool cplx = is_complex(h2->scalar);
+    const int64_t steps = M / 2;
+    for (size_t i = 0; i < r0->tiles.size(); ++i) {
+        const Tile& t = r0->tiles[i];
+        std::vector<double> s((size_t)steps * 4 * t.vw);
+        KPMX_HIP(hipMemcpyAsync(s.data(), r0->sums[i], s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        if (h2->scalar == 0 || h2->scalar == 2)
+            diag_collect<float>(s, t.vw, t.cols, M, cplx, static_cast<float*>(mu), t.c0);
+        else
+            diag_collect<double>(s, t.vw, t.cols, M, cplx, static_cast<double*>(mu), t.c0);
+    }
+}
+
+// op^H beta on the host: w_j = sum_i conj(op_ij) beta_i
+template <class R>
+void op_adjoint_apply(int64_t n, bool cplx, int64_t nnz, const int32_t* indptr, const int32_t* indices,
+                      const R* vals, const R* beta, std::vector<R>& w) {
+    const int c = cplx ? 2 : 1;
+    w.assign((size_t)n * c, R(0));
+    if (nnz == 0 || !vals) {
+        std::copy(beta, beta + n * c, w.begin());
+        return;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+            const int64_t j = indices[k];
+            if (cplx) {
+                const R ar = vals[2 * k], ai = -vals[2 * k + 1];
+                const R br = beta[2 * i], bi = beta[2 * i + 1];
+                w[2 * j] += ar * br - ai * bi;
+                w[2 * j + 1] += ar * bi + ai * br;
+            } else {
+                w[j] += vals[k] * beta[i];
+            }
+        }
+    }
+}
+
+
+// Extreme eigenvalues of the symmetric tridiagonal matrix (diag a[0..k), off-diagonal b[0..k-1))
+// by Sturm-sequence bisection (Golub & Van Loan, sec. 8.4.1) started from the Gershgorin interval.
+int sturm_count(const std::vector<double>& a, const std::vector<double>& b, size_t k, double x) {
+    int cnt = 0;
+    double d = 1.0;
+    for (size_t i = 0; i < k; ++i) {
+        const double off = (i > 0) ? b[i - 1] * b[i - 1] : 0.0;
+        d = a[i] - x - ((i > 0) ? off / d : 0.0);
+        if (d == 0.0) d = -1e-300;
+        if (d < 0) ++cnt;
+    }
+    return cnt;
+}
+
+void tridiagonal_minmax(const std::vector<double>& a, const std::vector<double>& b, size_t k, double* mn, double* mx) {
+    double lo = a[0], hi = a[0];
+    for (size_t i = 0; i < k; ++i) {
+        const double r = ((i > 0) ? std::abs(b[i - 1]) : 0.0) + ((i + 1 < k) ? std::abs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    const double span = std::max(hi - lo, 1e-300);
+    // smallest eigenvalue: smallest x with count(x) >= 1
+    double l = lo - 1e-12 * span, h = hi + 1e-12 * span;
+    for (int it = 0; it < 200 && h - l > 0; ++it) {
+        const double m = 0.5 * (l + h);
+        if (m <= l || m >= h) break;
+        if (sturm_count(a, b, k, m) >= 1) h = m; else l = m;
+    }
+    *mn = 0.5 * (l + h);
+    // largest eigenvalue: largest x with count(x) <= k - 1
+    l = lo - 1e-12 * span;
+    h = hi + 1e-12 * span;
+    for (int it = 0; it < 200 && h - l > 0; ++it) {
+        const double m = 0.5 * (l + h);
+        if (m <= l || m >= h) break;
+        if (sturm_count(a, b, k, m) <= (int)k - 1) l = m; else h = m;
+    }
+    *mx = 0.5 * (l + h);
+}
+
+}  // namespace
+
+// ==================================================================================== C-ABI
+extern "C" {
+
+const char* kpmx_last_error(void) { return g_last_error.c_str(); }
+
+int kpmx_version(void) { return 100; }
+
+int kpmx_device_count(int* count) {
+    return guarded([&] {
+        KPMX_REQUIRE(count, "null count");
+        int c = 0;
+        hipError_t e = hipGetDeviceCount(&c);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            c = 0;
+        }
+        *count = c;
+    });
+}
+
+int kpmx_ctx_create(int device, kpmx_ctx** out) {
+    return guarded([&] {
+        KPMX_REQUIRE(out, "null output");
+        int count = 0;
+        KPMX_HIP(hipGetDeviceCount(&count));
+        KPMX_REQUIRE(device >= 0 && device < count, "invalid device index");
+        auto* ctx = new kpmx_ctx();
+        ctx->device = device;
+        try {
+            KPMX_HIP(hipSetDevice(device));
+            hipDeviceProp_t prop;
+            KPMX_HIP(hipGetDeviceProperties(&prop, device));
+            ctx->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+            KPMX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        } catch (...) {
+            delete ctx;
+            throw;
+        }
+        *out = ctx;
+    });
+}
+
+int kpmx_ctx_destroy(kpmx_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipStreamDestroy(ctx->stream);
+        }
+        if (ctx->t0) (void)hipEventDestroy(ctx->t0);
+        if (ctx->t1) (void)hipEventDestroy(ctx->t1);
+        delete ctx;
+    });
+}
+
+int kpmx_ctx_synchronize(kpmx_ctx* ctx) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t nnz, const int32_t* indptr,
+                    const int32_t* indices, const void* values, kpmx_mat** out) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx && out, "null context or output");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        KPMX_REQUIRE(format == KPMX_FMT_CSR || format == KPMX_FMT_SELL, "invalid matrix format");
+        KPMX_REQUIRE(rows >= 0 && nnz >= 0 && rows < (int64_t)1 << 31 && nnz < (int64_t)1 << 31,
+                     "matrix too large for 32-bit indices");
+        KPMX_REQUIRE(indptr && (nnz == 0 || (indices && values)), "null CSR arrays");
+        KPMX_REQUIRE(indptr[0] == 0 && indptr[rows] == nnz, "inconsistent CSR row pointers");
+        for (int64_t i = 0; i < rows; ++i) KPMX_REQUIRE(indptr[i] <= indptr[i + 1], "decreasing CSR row pointers");
+        for (int64_t k = 0; k < nnz; ++k) KPMX_REQUIRE(indices[k] >= 0 && indices[k] < rows, "CSR column index out of range");
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const size_t sb = scalar_bytes(scalar);
+        auto m = std::make_unique<kpmx_mat>();
+        m->ctx = ctx;
+        m->scalar = scalar;
+        m->format = format;
+        m->rows = rows;
+        m->nnz = nnz;
+        m->indptr = static_cast<int32_t*>(dev_alloc((rows + 1) * sizeof(int32_t)));
+        m->indices = static_cast<int32_t*>(dev_alloc(nnz * sizeof(int32_t)));
+        m->vals = dev_alloc(nnz * sb);
+        KPMX_HIP(hipMemcpyAsync(m->indptr, indptr, (rows + 1) * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        if (nnz) {
+            KPMX_HIP(hipMemcpyAsync(m->indices, indices, nnz * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+            KPMX_HIP(hipMemcpyAsync(m->vals, values, nnz * sb, hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (format == KPMX_FMT_SELL) {
+            // Slices of 64 rows; inside a slice, column-major with the slice's max row length.
+            // Padding: value 0, column index = the row itself (a valid gather of data the row
+            // touches anyway).
+            const int64_t ns = (rows + 63) / 64;
+            std::vector<int64_t> sptr(ns + 1, 0);
+            std::vector<int32_t> swidth(ns, 0);
+            for (int64_t s = 0; s < ns; ++s) {
+                int32_t w = 0;
+                for (int64_t r = s * 64; r < std::min<int64_t>(rows, (s + 1) * 64); ++r)
+                    w = std::max<int32_t>(w, indptr[r + 1] - indptr[r]);
+                swidth[s] = w;
+                sptr[s + 1] = sptr[s] + (int64_t)w * 64;
+            }
+            const int64_t total = sptr[ns];
+            std::vector<int32_t> sidx((size_t)total);
+            std::vector<char> svals((size_t)total * sb, 0);
+            const char* v = static_cast<const char*>(values);
+            for (int64_t s = 0; s < ns; ++s) {
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int64_t r = s * 64 + lane;
+                    for (int32_t k = 0; k < swidth[s]; ++k) {
+                        const int64_t dst = sptr[s] + (int64_t)k * 64 + lane;
+                        if (r < rows && indptr[r] + k < indptr[r + 1]) {
+                            sidx[dst] = indices[indptr[r] + k];
+                            std::memcpy(&svals[dst * sb], v + (size_t)(indptr[r] + k) * sb, sb);
+                        } else {
+                            sidx[dst] = (int32_t)std::min<int64_t>(r, rows - 1);
+                        }
+                    }
+                }
+            }
+            m->nslices = ns;
+            m->slice_ptr = static_cast<int64_t*>(dev_alloc((ns + 1) * sizeof(int64_t)));
+            m->slice_width = static_cast<int32_t*>(dev_alloc(ns * sizeof(int32_t)));
+            m->sell_indices = static_cast<int32_t*>(dev_alloc(total * sizeof(int32_t)));
+            m->sell_vals = dev_alloc(total * sb);
+            KPMX_HIP(hipMemcpyAsync(m->slice_ptr, sptr.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+            KPMX_HIP(hipMemcpyAsync(m->slice_width, swidth.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+            if (total) {
+                KPMX_HIP(hipMemcpyAsync(m->sell_indices, sidx.data(), total * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+                KPMX_HIP(hipMemcpyAsync(m->sell_vals, svals.data(), total * sb, hipMemcpyHostToDevice, ctx->stream));
+            }
+            KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        *out = m.release();
+    });
+}
+
+int kpmx_mat_destroy(kpmx_mat* m) {
+    return guarded([&] {
+        if (!m) return;
+        (void)hipSetDevice(m->ctx->device);
+        (void)hipStreamSynchronize(m->ctx->stream);
+        delete m;
+    });
+}
+
+int kpmx_block_upload(kpmx_ctx* ctx, int scalar, int64_t n, int64_t num_vectors, const void* r0, kpmx_block** out) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx && out, "null context or output");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        KPMX_REQUIRE(n > 0 && num_vectors > 0 && r0, "empty starter block");
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const size_t sb = scalar_bytes(scalar);
+        auto b = std::make_unique<kpmx_block>();
+        b->ctx = ctx;
+        b->scalar = scalar;
+        b->n = n;
+        b->nvec = num_vectors;
+        void* staging = ctx->aux0.get((size_t)n * num_vectors * sb);
+        KPMX_HIP(hipMemcpyAsync(staging, r0, (size_t)n * num_vectors * sb, hipMemcpyHostToDevice, ctx->stream));
+        for (int64_t c0 = 0; c0 < num_vectors;) {
+            Tile t;
+            t.c0 = c0;
+            t.vw = pick_vw(num_vectors - c0);
+            t.cols = std::min<int64_t>(t.vw, num_vectors - c0);
+            t.data = dev_alloc((size_t)n * t.vw * sb);
+            b->tiles.push_back(t);
+            launch_gather_block(scalar, staging, n, t.c0, t.cols, t.vw, t.data, ctx->stream);
+            KPMX_HIP(hipGetLastError());
+            c0 += t.cols;
+        }
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        *out = b.release();
+    });
+}
+
+int kpmx_block_destroy(kpmx_block* b) {
+    return guarded([&] {
+        if (!b) return;
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+        delete b;
+    });
+}
+
+int kpmx_moments_diagonal_run(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments) {
+    return guarded([&] {
+        KPMX_REQUIRE(h2 && r0, "null matrix or starter block");
+        KPMX_HIP(hipSetDevice(h2->ctx->device));
+        diag_run(h2, r0, num_moments);
+    });
+}
+
+int kpmx_moments_diagonal_fetch(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu) {
+    return guarded([&] {
+        KPMX_REQUIRE(h2 && r0 && mu, "null argument");
+        KPMX_HIP(hipSetDevice(h2->ctx->device));
+        diag_fetch(h2, r0, num_moments, mu);
+    });
+}
+
+int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu) {
+    return guarded([&] {
+        KPMX_REQUIRE(h2 && r0 && mu, "null argument");
+        KPMX_HIP(hipSetDevice(h2->ctx->device));
+        diag_run(h2, r0, num_moments);
+        diag_fetch(h2, r0, num_moments, mu);
+    });
+}
+
+int kpmx_moments_generic(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, const void* beta, int64_t op_nnz,
+                         const int32_t* op_indptr, const int32_t* op_indices, const void* op_values, void* mu) {
+    return guarded([&] {
+        check_pair(h2, r0);
+        KPMX_REQUIRE(beta && mu, "null beta or output");
+        KPMX_REQUIRE(M >= 2, "num_moments must be >= 2");
+        KPMX_REQUIRE(op_nnz == 0 || (op_indptr && op_indices && op_values), "null operator arrays");
+        kpmx_ctx* ctx = h2->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int64_t n = h2->rows;
+        const bool cplx = is_complex(h2->scalar);
+        const size_t sb = scalar_bytes(h2->scalar);
+        // w = op^H beta (host), uploaded once
+        std::vector<char> wbuf((size_t)n * sb);
+        if (h2->scalar == 0 || h2->scalar == 2) {
+            std::vector<float> w;
+            op_adjoint_apply<float>(n, cplx, op_nnz, op_indptr, op_indices, static_cast<const float*>(op_values),
+                                    static_cast<const float*>(beta), w);
+            std::memcpy(wbuf.data(), w.data(), wbuf.size());
+        } else {
+            std::vector<double> w;
+            op_adjoint_apply<double>(n, cplx, op_nnz, op_indptr, op_indices, static_cast<const double*>(op_values),
+                                     static_cast<const double*>(beta), w);
+            std::memcpy(wbuf.data(), w.data(), wbuf.size());
+        }
+        void* dw = ctx->aux1.get(wbuf.size());
+        KPMX_HIP(hipMemcpyAsync(dw, wbuf.data(), wbuf.size(), hipMemcpyHostToDevice, ctx->stream));
+        // one-vector tile view of the first starter
+        const Tile& first = r0->tiles.front();
+        KPMX_REQUIRE(first.vw == 1 || r0->nvec == 1 || true, "");
+        Tile t;
+        t.vw = 1;
+        t.cols = 1;
+        if (first.vw == 1) {
+            t.data = first.data;
+        } else {
+            void* tmp = ctx->aux2.get((size_t)n * sb);
+            launch_scatter_block(h2->scalar, first.data, n, first.vw, 0, 1, tmp, ctx->stream);
+            KPMX_HIP(hipGetLastError());
+            t.data = tmp;
+        }
+        const int64_t steps = M - 1;
+        double* sums = static_cast<double*>(ctx->sums.get((size_t)steps * 4 * sizeof(double)));
+        run_recurrence(h2, t, MODE_DOT, steps, dw, sums, [](StepLaunch&, int64_t) {}, [](StepLaunch const&, int64_t) {});
+        std::vector<double> s((size_t)steps * 4);
+        KPMX_HIP(hipMemcpyAsync(s.data(), sums, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        auto put = [&](int64_t m, double re, double im) {
+            if (h2->scalar == 0) static_cast<float*>(mu)[m] = (float)re;
+            else if (h2->scalar == 1) static_cast<double*>(mu)[m] = re;
+            else if (h2->scalar == 2) { static_cast<float*>(mu)[2 * m] = (float)re; static_cast<float*>(mu)[2 * m + 1] = (float)im; }
+            else { static_cast<double*>(mu)[2 * m] = re; static_cast<double*>(mu)[2 * m + 1] = im; }
+        };
+        put(0, 0.5 * s[0], 0.5 * s[1]);
+        for (int64_t k = 0; k < steps; ++k) put(k + 1, s[k * 4 + 2], s[k * 4 + 3]);
+    });
+}
+
+int kpmx_moments_multi_unit(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, int64_t ndest, const int32_t* dest,
+                            void* mu) {
+    return guarded([&] {
+        check_pair(h2, r0);
+        KPMX_REQUIRE(mu && (ndest == 0 || dest), "null output or dest");
+        KPMX_REQUIRE(M >= 2, "num_moments must be >= 2");
+        kpmx_ctx* ctx = h2->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int64_t n = h2->rows;
+        const size_t sb = scalar_bytes(h2->scalar);
+        // unique destinations -> slots
+        std::vector<int32_t> mark((size_t)n, -1);
+        std::vector<int32_t> slot_of((size_t)ndest);
+        std::vector<int32_t> uniq;
+        for (int64_t i = 0; i < ndest; ++i) {
+            KPMX_REQUIRE(dest[i] >= 0 && dest[i] < n, "destination index out of range");
+            if (mark[dest[i]] < 0) {
+                mark[dest[i]] = (int32_t)uniq.size();
+                uniq.push_back(dest[i]);
+            }
+            slot_of[i] = mark[dest[i]];
+        }
+        const int64_t nu = (int64_t)uniq.size();
+        int32_t* dmark = static_cast<int32_t*>(ctx->aux1.get((size_t)n * sizeof(int32_t)));
+        KPMX_HIP(hipMemcpyAsync(dmark, mark.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        char* dmu = static_cast<char*>(ctx->aux2.get((size_t)std::max<int64_t>(nu, 1) * M * sb));
+        const Tile& first = r0->tiles.front();
+        Tile t;
+        t.vw = 1;
+        t.cols = 1;
+        void* vec;
+        if (first.vw == 1) {
+            vec = first.data;
+        } else {
+            vec = ctx->aux0.get((size_t)n * sb);
+            launch_scatter_block(h2->scalar, first.data, n, first.vw, 0, 1, vec, ctx->stream);
+            KPMX_HIP(hipGetLastError());
+        }
+        t.data = vec;
+        run_recurrence(h2, t, MODE_NONE, M - 1, nullptr, nullptr, [&](StepLaunch& L, int64_t k) {
+            L.mark = dmark;
+            L.mark_out = dmu + (size_t)(k + 1) * sb;
+            L.mark_stride = M;
+        }, [](StepLaunch const&, int64_t) {});
+        std::vector<char> host((size_t)nu * M * sb);
+        if (nu) KPMX_HIP(hipMemcpyAsync(host.data(), dmu, host.size(), hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<char> r0host((size_t)nu * sb);
+        for (int64_t i = 0; i < nu; ++i)
+            KPMX_HIP(hipMemcpyAsync(&r0host[i * sb], static_cast<char*>(vec) + (size_t)uniq[i] * sb, sb,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        char* out = static_cast<char*>(mu);
+        for (int64_t i = 0; i < ndest; ++i) {
+            const int32_t s = slot_of[i];
+            std::memcpy(out + (size_t)i * M * sb, &host[(size_t)s * M * sb], (size_t)M * sb);
+            // moment 0: 0.5 * r0[dest]
+            if (h2->scalar == 0 || h2->scalar == 2) {
+                float* o = reinterpret_cast<float*>(out + (size_t)i * M * sb);
+                const float* v = reinterpret_cast<const float*>(&r0host[(size_t)s * sb]);
+                o[0] = v[0] * 0.5f;
+                if (h2->scalar == 2) o[1] = v[1] * 0.5f;
+            } else {
+                double* o = reinterpret_cast<double*>(out + (size_t)i * M * sb);
+                const double* v = reinterpret_cast<const double*>(&r0host[(size_t)s * sb]);
+                o[0] = v[0] * 0.5;
+                if (h2->scalar == 3) o[1] = v[1] * 0.5;
+            }
+        }
+    });
+}
+
+
+int kpmx_moments_dense(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, const kpmx_mat* op, void* out) {
+    return guarded([&] {
+        check_pair(h2, r0);
+        KPMX_REQUIRE(out, "null output");
+        KPMX_REQUIRE(M >= 2, "num_moments must be >= 2");
+        if (op) {
+            KPMX_REQUIRE(op->ctx == h2->ctx && op->scalar == h2->scalar && op->rows == h2->rows,
+                         "operator does not match the Hamiltonian");
+        }
+        kpmx_ctx* ctx = h2->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int64_t n = h2->rows;
+        const size_t sb = scalar_bytes(h2->scalar);
+        for (const Tile& t : r0->tiles) {
+            // device panel: vw x M x n (vector, moment, site)
+            char* D = static_cast<char*>(ctx->aux2.get((size_t)t.vw * M * n * sb));
+            StepLaunch O;
+            if (op) {
+                O = base_launch(op, t.vw, false);
+                O.mode = MODE_NONE;
+                O.yout = nullptr;
+                O.sub_y = 0;
+                O.copy_col_stride = M * n;
+                // row 0: op (0.5 r0)
+                O.x = t.data;
+                O.hscale = 0.5;
+                O.copy_out = D;
+                launch(O, ctx->stream);
+            } else {
+                launch_scatter_scaled(h2->scalar, t.data, n, t.vw, t.vw, 0.5, D, M * n, ctx->stream);
+                KPMX_HIP(hipGetLastError());
+            }
+            run_recurrence(
+                h2, t, MODE_NONE, M - 1, nullptr, nullptr,
+                [&](StepLaunch& L, int64_t k) {
+                    if (!op) {
+                        L.copy_out = D + (size_t)(k + 1) * n * sb;
+                        L.copy_col_stride = M * n;
+                    }
+                },
+                [&](StepLaunch const& L, int64_t k) {
+                    if (op) {
+                        O.x = L.yout;
+                        O.hscale = 1.0;
+                        O.copy_out = D + (size_t)(k + 1) * n * sb;
+                        launch(O, ctx->stream);
+                    }
+                });
+            for (int64_t c = 0; c < t.cols; ++c)
+                KPMX_HIP(hipMemcpyAsync(static_cast<char*>(out) + (size_t)(t.c0 + c) * M * n * sb, D + (size_t)c * M * n * sb,
+                                        (size_t)M * n * sb, hipMemcpyDeviceToHost, ctx->stream));
+            KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr, const kpmx_block* r0, int64_t M,
+                      int mode, void* mu_nm) {
+    return guarded([&] {
+        check_pair(h2, r0);
+        KPMX_REQUIRE(vl && vr && mu_nm, "null argument");
+        KPMX_REQUIRE(mode == KPMX_KUBO_FP64 || mode == KPMX_KUBO_FP32_SPLIT, "invalid contraction mode");
+        for (const kpmx_mat* v : {vl, vr})
+            KPMX_REQUIRE(v->ctx == h2->ctx && v->scalar == h2->scalar && v->rows == h2->rows,
+                         "velocity operator does not match the Hamiltonian");
+        KPMX_REQUIRE(M >= 2, "num_moments must be >= 2");
+        kpmx_ctx* ctx = h2->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int64_t n = h2->rows;
+        const int sc = h2->scalar;
+        const size_t sb = scalar_bytes(sc);
+        const bool cplx = is_complex(sc);
+        const int64_t cw = cplx ? 2 : 1;
+
+        // group width g: both panels (2 g M n scalars) within ~45% of the free device memory
+        size_t free_b = 0, total_b = 0;
+        KPMX_HIP(hipMemGetInfo(&free_b, &total_b));
+        const double budget = 0.45 * (double)free_b;
+        const size_t panel1 = (size_t)M * n * sb;
+        KPMX_REQUIRE(2.0 * panel1 <= budget, "Kubo moment panels do not fit in device memory");
+
+        double* C = static_cast<double*>(ctx->sums.get((size_t)M * M * cw * sizeof(double)));
+        launch_zero_f64(C, M * M * cw, ctx->stream);
+        KPMX_HIP(hipGetLastError());
+
+        for (const Tile& full : r0->tiles) {
+            int g = full.vw;
+            while (g > 1 && 2.0 * g * panel1 > budget) g >>= 1;
+            for (int64_t c0 = 0; c0 < full.cols; c0 += g) {
+                const int64_t cols = std::min<int64_t>(g, full.cols - c0);
+                // this group's starters r (n x g), and vl r (n x g)
+                void* Pr = ctx->aux0.get((size_t)n * g * sb);
+                if (g == full.vw) {
+                    KPMX_HIP(hipMemcpyAsync(Pr, full.data, (size_t)n * g * sb, hipMemcpyDeviceToDevice, ctx->stream));
+                } else {
+                    // pull columns [c0, c0+g) of the tile out (via a vector-major bounce)
+                    void* vm = ctx->aux1.get((size_t)n * full.vw * sb);
+                    launch_scatter_block(sc, full.data, n, full.vw, 0, full.vw, vm, ctx->stream);
+                    launch_gather_block(sc, vm, n, c0, cols, g, Pr, ctx->stream);
+                    KPMX_HIP(hipGetLastError());
+                }
+                char* Lpan = static_cast<char*>(ctx->panel_l.get((size_t)g * panel1));
+                char* Rpan = static_cast<char*>(ctx->panel_r.get((size_t)g * panel1));
+                void* L0 = ctx->aux2.get((size_t)n * g * sb);
+
+                // L0 = vl r
+                StepLaunch V = base_launch(vl, g, false);
+                V.mode = MODE_NONE;
+                V.sub_y = 0;
+                V.hscale = 1.0;
+                V.x = Pr;
+                V.yout = L0;
+                launch(V, ctx->stream);
+                // L panel rows: T_n(H~) L0 (row 0 halved)
+                Tile tl;
+                tl.vw = g;
+                tl.cols = cols;
+                tl.data = L0;
+                launch_scatter_scaled(sc, L0, n, g, g, 0.5, Lpan, M * n, ctx->stream);
+                KPMX_HIP(hipGetLastError());
+                run_recurrence(
+                    h2, tl, MODE_NONE, M - 1, nullptr, nullptr,
+                    [&](StepLaunch& L, int64_t k) {
+                        L.copy_out = Lpan + (size_t)(k + 1) * n * sb;
+                        L.copy_col_stride = M * n;
+                    },
+                    [](StepLaunch const&, int64_t) {});
+                // R panel rows: vr T_m(H~) r (row 0 halved)
+                StepLaunch O = base_launch(vr, g, false);
+                O.mode = MODE_NONE;
+                O.sub_y = 0;
+                O.yout = nullptr;
+                O.copy_col_stride = M * n;
+                O.x = Pr;
+                O.hscale = 0.5;
+                O.copy_out = Rpan;
+                launch(O, ctx->stream);
+                Tile tr;
+                tr.vw = g;
+                tr.cols = cols;
+                tr.data = Pr;
+                run_recurrence(
+                    h2, tr, MODE_NONE, M - 1, nullptr, nullptr, [](StepLaunch&, int64_t) {},
+                    [&](StepLaunch const& L, int64_t k) {
+                        O.x = L.yout;
+                        O.hscale = 1.0;
+                        O.copy_out = Rpan + (size_t)(k + 1) * n * sb;
+                        launch(O, ctx->stream);
+                    });
+                // mu_nm += sum_c L_c R_c^H
+                const int64_t wd = gemm_workspace_doubles(M, n, sc);
+                double* work = static_cast<double*>(ctx->partials.get((size_t)wd * sizeof(double)));
+                for (int64_t c = 0; c < cols; ++c) {
+                    if (launch_gemm_nc(sc, mode, Lpan + (size_t)c * M * n * sb, n, Rpan + (size_t)c * M * n * sb, n, M,
+                                       n, C, work, ctx->stream) != 0)
+                        throw Error{KPMX_EINVAL, "unsupported contraction"};
+                    KPMX_HIP(hipGetLastError());
+                }
+            }
+        }
+        std::vector<double> h((size_t)M * M * cw);
+        KPMX_HIP(hipMemcpyAsync(h.data(), C, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        const double inv = 1.0 / (double)r0->nvec;
+        for (size_t e = 0; e < h.size(); ++e) {
+            if (sc == 0 || sc == 2) static_cast<float*>(mu_nm)[e] = (float)(h[e] * inv);
+            else static_cast<double*>(mu_nm)[e] = h[e] * inv;
+        }
+    });
+}
+
+
+int kpmx_lanczos_bounds(const kpmx_mat* h, double precision_percent, double* min_energy, double* max_energy,
+                        int* loops) {
+    return guarded([&] {
+        KPMX_REQUIRE(h && min_energy && max_energy && loops, "null argument");
+        KPMX_REQUIRE(h->rows > 0, "empty matrix");
+        kpmx_ctx* ctx = h->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int64_t n = h->rows;
+        const int sc = h->scalar;
+        const size_t sb = scalar_bytes(sc);
+        // v1 = uniform [0, 1) reals from a default-seeded mt19937, normalized; v0 = 0
+        std::vector<char> hv((size_t)n * sb, 0);
+        std::mt19937 gen;
+        if (sc == 0 || sc == 2) {
+            std::uniform_real_distribution<float> dist;
+            std::vector<float> v((size_t)n);
+            for (auto& x : v) x = dist(gen);
+            float nrm = 0;
+            for (auto x : v) nrm += x * x;
+            nrm = std::sqrt(nrm);
+            float* o = reinterpret_cast<float*>(hv.data());
+            for (int64_t i = 0; i < n; ++i) o[(sc == 2 ? 2 : 1) * i] = v[(size_t)i] / nrm;
+        } else {
+            std::uniform_real_distribution<double> dist;
+            std::vector<double> v((size_t)n);
+            for (auto& x : v) x = dist(gen);
+            double nrm = 0;
+            for (auto x : v) nrm += x * x;
+            nrm = std::sqrt(nrm);
+            double* o = reinterpret_cast<double*>(hv.data());
+            for (int64_t i = 0; i < n; ++i) o[(sc == 3 ? 2 : 1) * i] = v[(size_t)i] / nrm;
+        }
+        void* v1 = ctx->work1.get((size_t)n * sb);
+        void* v0 = ctx->work0.get((size_t)n * sb);
+        KPMX_HIP(hipMemcpyAsync(v1, hv.data(), hv.size(), hipMemcpyHostToDevice, ctx->stream));
+        launch_zero(sc, v0, n, ctx->stream);
+        const int nblocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, (int64_t)ctx->num_cu * 4));
+        double* partials = static_cast<double*>(ctx->partials.get((size_t)nblocks * sizeof(double)));
+        double* red = static_cast<double*>(ctx->sums.get(4 * sizeof(double)));
+        std::vector<double> alpha, beta;
+        const double precision = precision_percent / 100;
+        double prev_min = std::numeric_limits<double>::max(), prev_max = std::numeric_limits<double>::lowest();
+        for (int i = 0; i < 1000; ++i) {
+            const double b_prev = beta.empty() ? 0.0 : beta.back();
+            launch_lanczos_spmv(sc, h->indptr, h->indices, h->vals, v1, v0, n, b_prev, partials, nblocks, ctx->stream);
+            launch_reduce_partials(partials, nblocks, 1, 1, red, ctx->stream);
+            double a = 0;
+            KPMX_HIP(hipMemcpyAsync(&a, red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KPMX_HIP(hipStreamSynchronize(ctx->stream));
+            launch_lanczos_axpy(sc, v1, v0, n, a, partials, nblocks, ctx->stream);
+            launch_reduce_partials(partials, nblocks, 1, 1, red, ctx->stream);
+            double b2 = 0;
+            KPMX_HIP(hipMemcpyAsync(&b2, red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            KPMX_HIP(hipStreamSynchronize(ctx->stream));
+            const double b = std::sqrt(b2);
+            alpha.push_back(a);
+            beta.push_back(b);
+            if (b > 0) launch_scale(sc, v0, n, 1.0 / b, ctx->stream);
+            KPMX_HIP(hipGetLastError());
+            std::swap(v0, v1);
+            double mn, mx;
+            tridiagonal_minmax(alpha, beta, alpha.size(), &mn, &mx);
+            const bool conv_min = std::abs((prev_min - mn) / mn) < precision;
+            const bool conv_max = std::abs((prev_max - mx) / mx) < precision;
+            if ((conv_min && conv_max) || b == 0) {
+                *min_energy = mn;
+                *max_energy = mx;
+                *loops = i;
+                return;
+            }
+            prev_min = mn;
+            prev_max = mx;
+        }
+        throw Error{KPMX_EINVAL, "Lanczos algorithm did not converge for the min/max eigenvalues."};
+    });
+}
+
+int kpmx_ctx_timer_start(kpmx_ctx* ctx) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_HIP(hipSetDevice(ctx->device));
+        if (!ctx->t0) KPMX_HIP(hipEventCreate(&ctx->t0));
+        if (!ctx->t1) KPMX_HIP(hipEventCreate(&ctx->t1));
+        KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
+    });
+}
+
+int kpmx_ctx_timer_stop(kpmx_ctx* ctx, double* ms) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx && ms, "null argument");
+        KPMX_REQUIRE(ctx->t0 && ctx->t1, "timer was not started");
+        KPMX_HIP(hipEventRecord(ctx->t1, ctx->stream));
+        KPMX_HIP(hipEventSynchronize(ctx->t1));
+        float f = 0;
+        KPMX_HIP(hipEventElapsedTime(&f, ctx->t0, ctx->t1));
+        *ms = (double)f;
+    });
+}
+}  // extern "C"

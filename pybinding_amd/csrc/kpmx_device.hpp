@@ -1,0 +1,61 @@
+// Device-side scalar helpers for the KPM kernels (gfx950).
+//
+// Scalars: float, double, and the complex pairs cf32 / cf64 laid out as {re, im} exactly like
+// std::complex<T>, so host arrays of std::complex can be copied byte for byte.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kpmx {
+
+template <class R>
+struct cplx {
+    R re, im;
+};
+using cf32 = cplx<float>;
+using cf64 = cplx<double>;
+
+template <class T> struct traits;
+template <> struct traits<float>  { using real = float;  static constexpr bool is_complex = false; static constexpr int code = 0; };
+template <> struct traits<double> { using real = double; static constexpr bool is_complex = false; static constexpr int code = 1; };
+template <> struct traits<cf32>   { using real = float;  static constexpr bool is_complex = true;  static constexpr int code = 2; };
+template <> struct traits<cf64>   { using real = double; static constexpr bool is_complex = true;  static constexpr int code = 3; };
+
+// ----------------------------------------------------------------------------- arithmetic
+__host__ __device__ __forceinline__ float  zero(float)  { return 0.f; }
+__host__ __device__ __forceinline__ double zero(double) { return 0.0; }
+template <class R> __host__ __device__ __forceinline__ cplx<R> zero(cplx<R>) { return {R(0), R(0)}; }
+
+// a*b + c (plain product, no complex special-value handling, like the reference's detail::mul)
+__device__ __forceinline__ float  fmac(float a, float b, float c)    { return a * b + c; }
+__device__ __forceinline__ double fmac(double a, double b, double c) { return a * b + c; }
+template <class R>
+__device__ __forceinline__ cplx<R> fmac(cplx<R> a, cplx<R> b, cplx<R> c) {
+    return {a.re * b.re - a.im * b.im + c.re, a.re * b.im + a.im * b.re + c.im};
+}
+
+__device__ __forceinline__ float  scal(float a, float s)   { return a * s; }
+__device__ __forceinline__ double scal(double a, double s) { return a * s; }
+template <class R> __device__ __forceinline__ cplx<R> scal(cplx<R> a, R s) { return {a.re * s, a.im * s}; }
+
+__device__ __forceinline__ float  sub(float a, float b)   { return a - b; }
+__device__ __forceinline__ double sub(double a, double b) { return a - b; }
+template <class R> __device__ __forceinline__ cplx<R> sub(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
+
+// |a|^2 in double
+__device__ __forceinline__ double sqabs(float a)  { return double(a) * double(a); }
+__device__ __forceinline__ double sqabs(double a) { return a * a; }
+template <class R> __device__ __forceinline__ double sqabs(cplx<R> a) {
+    return double(a.re) * double(a.re) + double(a.im) * double(a.im);
+}
+
+// conj(a)*b accumulated in double: returns {re, im}
+struct d2 { double re, im; };
+__device__ __forceinline__ d2 cdot(float a, float b)   { return {double(a) * double(b), 0.0}; }
+__device__ __forceinline__ d2 cdot(double a, double b) { return {a * b, 0.0}; }
+template <class R> __device__ __forceinline__ d2 cdot(cplx<R> a, cplx<R> b) {
+    return {double(a.re) * double(b.re) + double(a.im) * double(b.im),
+            double(a.re) * double(b.im) - double(a.im) * double(b.re)};
+}
+
+}  // namespace kpmx

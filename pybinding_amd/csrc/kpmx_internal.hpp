@@ -1,0 +1,73 @@
+// Internal launch interface between the C-ABI layer (kpmx_abi.hip) and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kpmx {
+
+enum { MODE_DIAG = 0, MODE_DOT = 1, MODE_NONE = 2 };
+enum { FORMAT_CSR = 0, FORMAT_SELL = 1 };
+
+struct StepLaunch {
+    int scalar = 1;          // 0 f32, 1 f64, 2 c64, 3 c128
+    int format = FORMAT_CSR;
+    int vw = 1;              // vectors per row (power of two <= 64)
+    int mode = MODE_DIAG;
+    int num_wg = 0;
+    const int32_t* indptr = nullptr;   // CSR row pointers (CSR format)
+    const int32_t* indices = nullptr;  // CSR or SELL column indices
+    const void* vals = nullptr;
+    const int64_t* slice_ptr = nullptr;  // SELL
+    const int32_t* slice_width = nullptr;
+    const void* x = nullptr;
+    const void* yin = nullptr;
+    void* yout = nullptr;
+    const void* w = nullptr;
+    void* copy_out = nullptr;
+    int64_t copy_col_stride = 0;
+    const int32_t* mark = nullptr;
+    void* mark_out = nullptr;
+    int64_t mark_stride = 0;
+    double* partials = nullptr;
+    int64_t n = 0;
+    double hscale = 1.0;
+    int sub_y = 1;
+};
+
+int launch_step(const StepLaunch& L, hipStream_t stream);
+int64_t step_rows_per_tile(int format, int vw);
+void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
+                            hipStream_t stream);
+void launch_gather_block(int scalar, const void* src, int64_t n, int64_t c0, int64_t cols, int vw, void* dst,
+                         hipStream_t s);
+void launch_scatter_block(int scalar, const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst,
+                          hipStream_t s);
+void launch_scatter_scaled(int scalar, const void* src, int64_t n, int vw, int64_t cols, double scale, void* dst,
+                           int64_t col_stride, hipStream_t s);
+// C (M x M, {re, im} doubles, or doubles for real scalars) += sum_k A[n][k] conj(B[m][k]) over
+// k < kdim, A and B row-major with leading dimensions lda / ldb (elements).  Deterministic
+// split-K: `work` must hold gemm_workspace_doubles(M, kdim, scalar) doubles.
+int64_t gemm_workspace_doubles(int64_t M, int64_t kdim, int scalar);
+int launch_gemm_nc(int scalar, int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M,
+                   int64_t kdim, double* C, double* work, hipStream_t s);
+void launch_zero(int scalar, void* dst, int64_t count, hipStream_t s);
+void launch_zero_f64(double* dst, int64_t count, hipStream_t s);
+void launch_csr_matvec(int scalar, const int32_t* indptr, const int32_t* indices, const void* vals, const void* x,
+                       int64_t n, void* y, hipStream_t s);
+
+void launch_lanczos_spmv(int scalar, const int32_t* indptr, const int32_t* indices, const void* vals, const void* v1,
+                         void* v0, int64_t n, double b_prev, double* partials, int nblocks, hipStream_t s);
+void launch_lanczos_axpy(int scalar, const void* v1, void* v0, int64_t n, double a, double* partials, int nblocks,
+                         hipStream_t s);
+void launch_scale(int scalar, void* v, int64_t n, double f, hipStream_t s);
+
+inline size_t scalar_bytes(int scalar) {
+    switch (scalar) {
+        case 0: return 4;
+        case 1: return 8;
+        case 2: return 8;
+        default: return 16;
+    }
+}
+
+}  // namespace kpmx

@@ -1,0 +1,487 @@
+// HIP kernels of the KPM Chebyshev-moment engine, written for gfx950 (MI355X, CDNA4).
+//
+// The hot kernel is one fused Chebyshev step over a block of VW vectors stored row-major
+// (n x VW, the layout of the reference's SIMD vector blocks):
+//
+//     y_new = s * (H2 x) - y            (s = 1; the first step uses s = 0.5 and no y)
+//
+// fused with the two per-column reductions that give two moments per step
+// (cppcore/include/compute/kernel_polynomial.hpp:60-81 semantics):
+//
+//     MODE_DIAG:  A = sum |x_i|^2            B = sum conj(y_new_i) x_i
+//     MODE_DOT :  A = sum conj(w_i) x_i       B = sum conj(w_i) y_new_i     (w = op^H beta)
+//     MODE_NONE:  no reduction (dense / multi-unit collection)
+//
+// Mapping: a 64-lane wavefront covers 64/VW rows x VW columns, lane = sub * VW + col, so every
+// gathered x row (VW contiguous scalars) is one coalesced load.  Rows are cut into tiles of
+// 4 waves x 64/VW rows; tiles are dealt round-robin inside 8 contiguous row regions, region r
+// belonging to the workgroups with blockIdx % 8 == r, which the hardware places on XCD r.  All
+// waves of one XCD therefore sweep one contiguous band of rows together and the gathered x rows
+// of that band stay in that XCD's L2.
+//
+// Reductions are deterministic: wave shuffles in a fixed tree, the 4 waves of a block summed in
+// order through LDS, one partial per workgroup written to `partials`, and a second kernel that
+// sums the partials of each step in workgroup order.
+#include "kpmx_device.hpp"
+#include "kpmx_internal.hpp"
+
+namespace kpmx {
+
+constexpr int WAVE = 64;
+constexpr int WAVES_PER_BLOCK = 4;
+constexpr int BLOCK = WAVE * WAVES_PER_BLOCK;
+
+template <class T>
+struct StepArgs {
+    using real = typename traits<T>::real;
+    const int32_t* indptr;
+    const int32_t* indices;
+    const T* vals;
+    const T* x;
+    const T* yin;        // may be null when !sub_y
+    T* yout;             // may alias yin
+    const T* w;          // MODE_DOT weights, n x VW
+    T* copy_out;         // optional: y_new also written to copy_out[col * copy_col_stride + row]
+    int64_t copy_col_stride;
+    const int32_t* mark; // optional: per-row slot (or -1); y_new written to mark_out[slot * mark_stride]
+    T* mark_out;
+    int64_t mark_stride;
+    double* partials;    // [gridDim.x][4 * VW]
+    int64_t n;
+    int64_t num_tiles;
+    real hscale;
+    int sub_y;
+};
+
+// Write the block's 4*VW column sums into partials[blockIdx.x].
+template <int VW>
+__device__ __forceinline__ void block_reduce_store(double a_re, double a_im, double b_re, double b_im,
+                                                   double* partials) {
+    __shared__ double red[WAVES_PER_BLOCK][4][VW];
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    for (int off = WAVE / 2; off >= VW; off >>= 1) {
+        a_re += __shfl_xor(a_re, off, WAVE);
+        a_im += __shfl_xor(a_im, off, WAVE);
+        b_re += __shfl_xor(b_re, off, WAVE);
+        b_im += __shfl_xor(b_im, off, WAVE);
+    }
+    if (lane < VW) {
+        red[wave][0][lane] = a_re;
+        red[wave][1][lane] = a_im;
+        red[wave][2][lane] = b_re;
+        red[wave][3][lane] = b_im;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 4 * VW; t += BLOCK) {
+        const int q = t / VW, c = t % VW;
+        double s = red[0][q][c];
+        for (int wv = 1; wv < WAVES_PER_BLOCK; ++wv) s += red[wv][q][c];
+        partials[(int64_t)blockIdx.x * 4 * VW + t] = s;
+    }
+}
+
+// Tile index sequence of this workgroup (XCD-aware, see the file header).
+struct TileWalk {
+    int64_t first, stride, end;
+};
+__device__ __forceinline__ TileWalk tile_walk(int64_t num_tiles) {
+    const int nb = gridDim.x;
+    const int b = blockIdx.x;
+    if (nb < 8 || (nb % 8) != 0) return {b, nb, num_tiles};
+    const int region = b % 8;
+    const int local = b / 8;
+    const int per_region_blocks = nb / 8;
+    const int64_t tiles_per_region = (num_tiles + 7) / 8;
+    const int64_t start = region * tiles_per_region;
+    int64_t end = start + tiles_per_region;
+    if (end > num_tiles) end = num_tiles;
+    return {start + local, per_region_blocks, end};
+}
+
+template <class T, int VW, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_step_csr(StepArgs<T> a) {
+    constexpr int RPW = WAVE / VW;                       // rows per wave
+    constexpr int ROWS_PER_TILE = RPW * WAVES_PER_BLOCK;
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    const int rsub = lane / VW;
+    const int col = lane % VW;
+
+    double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
+    const TileWalk tw = tile_walk(a.num_tiles);
+    for (int64_t tile = tw.first; tile < tw.end; tile += tw.stride) {
+        const int64_t row = tile * ROWS_PER_TILE + wave * RPW + rsub;
+        if (row >= a.n) continue;
+        const int32_t k0 = a.indptr[row], k1 = a.indptr[row + 1];
+        T acc = zero(T{});
+        for (int32_t k = k0; k < k1; ++k) {
+            const int64_t c = a.indices[k];
+            acc = fmac(a.vals[k], a.x[c * VW + col], acc);
+        }
+        const int64_t e = row * VW + col;
+        T ynew = scal(acc, a.hscale);
+        if (a.sub_y) ynew = sub(ynew, a.yin[e]);
+        if (a.yout) a.yout[e] = ynew;
+        if (a.copy_out) a.copy_out[col * a.copy_col_stride + row] = ynew;
+        if (a.mark) {
+            const int32_t slot = a.mark[row];
+            if (slot >= 0) a.mark_out[(int64_t)slot * a.mark_stride + col] = ynew;
+        }
+        if constexpr (MODE == MODE_DIAG) {
+            const T xo = a.x[e];
+            a_re += sqabs(xo);
+            const d2 p = cdot(ynew, xo);
+            b_re += p.re;
+            b_im += p.im;
+        } else if constexpr (MODE == MODE_DOT) {
+            const T wv = a.w[e];
+            const d2 p = cdot(wv, a.x[e]);
+            a_re += p.re;
+            a_im += p.im;
+            const d2 q = cdot(wv, ynew);
+            b_re += q.re;
+            b_im += q.im;
+        }
+    }
+    if constexpr (MODE != MODE_NONE) block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials);
+}
+
+// Sliced ELL, one vector (VW = 1): slices of 64 rows, column-major inside the slice with the
+// slice's own width, so lane i of a wave reads element (row 64s+i, k) at slice_ptr[s] + 64k + i.
+template <class T, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_step_sell(StepArgs<T> a, const int64_t* slice_ptr,
+                                                     const int32_t* slice_width) {
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
+    const TileWalk tw = tile_walk(a.num_tiles);
+    for (int64_t tile = tw.first; tile < tw.end; tile += tw.stride) {
+        const int64_t slice = tile * WAVES_PER_BLOCK + wave;
+        const int64_t row = slice * WAVE + lane;
+        if (row >= a.n) continue;
+        const int64_t base = slice_ptr[slice] + lane;
+        const int width = slice_width[slice];
+        T acc = zero(T{});
+        for (int k = 0; k < width; ++k) {
+            const int64_t c = a.indices[base + (int64_t)k * WAVE];
+            acc = fmac(a.vals[base + (int64_t)k * WAVE], a.x[c], acc);
+        }
+        T ynew = scal(acc, a.hscale);
+        if (a.sub_y) ynew = sub(ynew, a.yin[row]);
+        if (a.yout) a.yout[row] = ynew;
+        if (a.copy_out) a.copy_out[row] = ynew;
+        if (a.mark) {
+            const int32_t slot = a.mark[row];
+            if (slot >= 0) a.mark_out[(int64_t)slot * a.mark_stride] = ynew;
+        }
+        if constexpr (MODE == MODE_DIAG) {
+            const T xo = a.x[row];
+            a_re += sqabs(xo);
+            const d2 p = cdot(ynew, xo);
+            b_re += p.re;
+            b_im += p.im;
+        } else if constexpr (MODE == MODE_DOT) {
+            const T wv = a.w[row];
+            const d2 p = cdot(wv, a.x[row]);
+            a_re += p.re;
+            a_im += p.im;
+            const d2 q = cdot(wv, ynew);
+            b_re += q.re;
+            b_im += q.im;
+        }
+    }
+    if constexpr (MODE != MODE_NONE) block_reduce_store<1>(a_re, a_im, b_re, b_im, a.partials);
+}
+
+// sums[s][j] = sum over workgroups g (in order) of partials[s][g][j], for s in [0, nsteps).
+// One block per step; thread t sums the workgroup subset t / width (strided), then the subsets
+// are combined in order through LDS.
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* partials, int64_t num_wg, int width,
+                                                         double* sums) {
+    __shared__ double red[256];
+    const int s = blockIdx.x;
+    const double* p = partials + (int64_t)s * num_wg * width;
+    const int groups = 256 / width;  // width divides 256 (width = 4*VW, VW power of two <= 64)
+    const int t = threadIdx.x;
+    const int j = t % width;
+    const int g = t / width;
+    double acc = 0;
+    if (g < groups) {
+        for (int64_t w = g; w < num_wg; w += groups) acc += p[w * width + j];
+    }
+    red[t] = acc;
+    __syncthreads();
+    if (t < width) {
+        double tot = red[t];
+        for (int q = 1; q < groups; ++q) tot += red[q * width + t];
+        sums[(int64_t)s * width + t] = tot;
+    }
+}
+
+// dst (n x vw, row-major) <- columns [c0, c0 + cols) of src (nvec x n, vector-major), zero padded.
+template <class T>
+__global__ void k_gather_block(const T* src, int64_t n, int64_t c0, int64_t cols, int vw, T* dst) {
+    const int64_t total = n * vw;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / vw;
+        const int c = (int)(e % vw);
+        dst[e] = (c < cols) ? src[(c0 + c) * n + row] : zero(T{});
+    }
+}
+
+// Row-major n x vw block -> vector-major columns [c0, c0+cols) of dst (nvec x n).
+template <class T>
+__global__ void k_scatter_block(const T* src, int64_t n, int vw, int64_t c0, int64_t cols, T* dst) {
+    const int64_t total = n * cols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n;
+        const int64_t row = e % n;
+        dst[(c0 + c) * n + row] = src[row * vw + c];
+    }
+}
+
+// dst[c * col_stride + row] = scale * src[row * vw + c] for c < cols.
+template <class T>
+__global__ void k_scatter_scaled(const T* src, int64_t n, int vw, int64_t cols, double scale, T* dst,
+                                 int64_t col_stride) {
+    using real = typename traits<T>::real;
+    const int64_t total = n * cols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n;
+        const int64_t row = e % n;
+        dst[c * col_stride + row] = scal(src[row * vw + c], (real)scale);
+    }
+}
+
+template <class T>
+__global__ void k_fill(T* dst, int64_t count, T value) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = value;
+}
+
+// y = A x (plain CSR product, one vector), used for op * r and op^H beta preparation.
+template <class T>
+__global__ void k_csr_matvec(const int32_t* indptr, const int32_t* indices, const T* vals, const T* x, int64_t n, T* y) {
+    for (int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; row < n; row += (int64_t)gridDim.x * blockDim.x) {
+        T acc = zero(T{});
+        for (int32_t k = indptr[row]; k < indptr[row + 1]; ++k) acc = fmac(vals[k], x[indices[k]], acc);
+        y[row] = acc;
+    }
+}
+
+// Lanczos (cppcore/include/compute/lanczos.hpp:25-91): tmp = H v1; v0 = tmp - b_prev v0;
+// partial sums of real(conj(tmp) v1).  One partial per block in `partials`.
+__device__ __forceinline__ double real_dot(float a, float b) { return double(a) * double(b); }
+__device__ __forceinline__ double real_dot(double a, double b) { return a * b; }
+template <class R> __device__ __forceinline__ double real_dot(cplx<R> a, cplx<R> b) {
+    return double(a.re) * double(b.re) + double(a.im) * double(b.im);
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[WAVES_PER_BLOCK];
+    for (int off = WAVE / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) s += red[w];
+    return s;
+}
+
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_lanczos_spmv(const int32_t* indptr, const int32_t* indices, const T* vals,
+                                                        const T* v1, T* v0, int64_t n, double b_prev, double* partials) {
+    using real = typename traits<T>::real;
+    double acc = 0;
+    for (int64_t row = (int64_t)blockIdx.x * BLOCK + threadIdx.x; row < n; row += (int64_t)gridDim.x * BLOCK) {
+        T tmp = zero(T{});
+        for (int32_t k = indptr[row]; k < indptr[row + 1]; ++k) tmp = fmac(vals[k], v1[indices[k]], tmp);
+        v0[row] = sub(tmp, scal(v0[row], (real)b_prev));
+        acc += real_dot(tmp, v1[row]);
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// v0 = v0 - a v1; partial sums of |v0|^2
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_lanczos_axpy(const T* v1, T* v0, int64_t n, double a, double* partials) {
+    using real = typename traits<T>::real;
+    double acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const T l = sub(v0[i], scal(v1[i], (real)a));
+        acc += sqabs(l);
+        v0[i] = l;
+    }
+    const double s = block_sum(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <class T>
+__global__ void k_scale(T* v, int64_t n, double f) {
+    using real = typename traits<T>::real;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = scal(v[i], (real)f);
+}
+
+// ------------------------------------------------------------------------------------ launchers
+template <class T>
+static int launch_step_t(const StepLaunch& L, hipStream_t stream) {
+    StepArgs<T> a;
+    a.indptr = L.indptr;
+    a.indices = L.indices;
+    a.vals = static_cast<const T*>(L.vals);
+    a.x = static_cast<const T*>(L.x);
+    a.yin = static_cast<const T*>(L.yin);
+    a.yout = static_cast<T*>(L.yout);
+    a.w = static_cast<const T*>(L.w);
+    a.copy_out = static_cast<T*>(L.copy_out);
+    a.copy_col_stride = L.copy_col_stride;
+    a.mark = L.mark;
+    a.mark_out = static_cast<T*>(L.mark_out);
+    a.mark_stride = L.mark_stride;
+    a.partials = L.partials;
+    a.n = L.n;
+    a.hscale = static_cast<typename traits<T>::real>(L.hscale);
+    a.sub_y = L.sub_y;
+    const dim3 grid(L.num_wg), block(BLOCK);
+    if (L.format == FORMAT_SELL) {
+        if (L.vw != 1) return -1;
+        a.num_tiles = (L.n + BLOCK - 1) / BLOCK;
+        switch (L.mode) {
+            case MODE_DIAG: hipLaunchKernelGGL((k_step_sell<T, MODE_DIAG>), grid, block, 0, stream, a, L.slice_ptr, L.slice_width); break;
+            case MODE_DOT:  hipLaunchKernelGGL((k_step_sell<T, MODE_DOT>), grid, block, 0, stream, a, L.slice_ptr, L.slice_width); break;
+            default:        hipLaunchKernelGGL((k_step_sell<T, MODE_NONE>), grid, block, 0, stream, a, L.slice_ptr, L.slice_width); break;
+        }
+        return 0;
+    }
+    const int64_t rows_per_tile = (WAVE / L.vw) * WAVES_PER_BLOCK;
+    a.num_tiles = (L.n + rows_per_tile - 1) / rows_per_tile;
+#define KPMX_VW_CASE(V)                                                                                      \
+    case V:                                                                                                  \
+        switch (L.mode) {                                                                                    \
+            case MODE_DIAG: hipLaunchKernelGGL((k_step_csr<T, V, MODE_DIAG>), grid, block, 0, stream, a); break; \
+            case MODE_DOT:  hipLaunchKernelGGL((k_step_csr<T, V, MODE_DOT>), grid, block, 0, stream, a); break;  \
+            default:        hipLaunchKernelGGL((k_step_csr<T, V, MODE_NONE>), grid, block, 0, stream, a); break; \
+        }                                                                                                    \
+        return 0;
+    switch (L.vw) {
+        KPMX_VW_CASE(1)
+        KPMX_VW_CASE(2)
+        KPMX_VW_CASE(4)
+        KPMX_VW_CASE(8)
+        KPMX_VW_CASE(16)
+        KPMX_VW_CASE(32)
+        KPMX_VW_CASE(64)
+        default: return -1;
+    }
+#undef KPMX_VW_CASE
+}
+
+int launch_step(const StepLaunch& L, hipStream_t stream) {
+    switch (L.scalar) {
+        case 0: return launch_step_t<float>(L, stream);
+        case 1: return launch_step_t<double>(L, stream);
+        case 2: return launch_step_t<cf32>(L, stream);
+        case 3: return launch_step_t<cf64>(L, stream);
+        default: return -1;
+    }
+}
+
+int64_t step_rows_per_tile(int format, int vw) {
+    return format == FORMAT_SELL ? (int64_t)BLOCK : (int64_t)(WAVE / vw) * WAVES_PER_BLOCK;
+}
+
+void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
+                            hipStream_t stream) {
+    if (nsteps <= 0) return;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)nsteps), dim3(256), 0, stream, partials, num_wg, width, sums);
+}
+
+static dim3 grid_for(int64_t count) {
+    int64_t g = (count + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return dim3((unsigned)g);
+}
+
+template <class T>
+static void gather_t(const void* src, int64_t n, int64_t c0, int64_t cols, int vw, void* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_block<T>, grid_for(n * vw), dim3(256), 0, s, static_cast<const T*>(src), n, c0, cols, vw,
+                       static_cast<T*>(dst));
+}
+template <class T>
+static void scatter_t(const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_block<T>, grid_for(n * cols), dim3(256), 0, s, static_cast<const T*>(src), n, vw, c0, cols,
+                       static_cast<T*>(dst));
+}
+template <class T>
+static void scatter_scaled_t(const void* src, int64_t n, int vw, int64_t cols, double scale, void* dst,
+                             int64_t col_stride, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_scaled<T>, grid_for(n * cols), dim3(256), 0, s, static_cast<const T*>(src), n, vw, cols,
+                       scale, static_cast<T*>(dst), col_stride);
+}
+template <class T>
+static void zero_t(void* dst, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill<T>, grid_for(count), dim3(256), 0, s, static_cast<T*>(dst), count, zero(T{}));
+}
+template <class T>
+static void matvec_t(const int32_t* indptr, const int32_t* indices, const void* vals, const void* x, int64_t n, void* y,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_csr_matvec<T>, grid_for(n), dim3(256), 0, s, indptr, indices, static_cast<const T*>(vals),
+                       static_cast<const T*>(x), n, static_cast<T*>(y));
+}
+
+#define KPMX_DISPATCH(scalar, fn, ...)                       \
+    switch (scalar) {                                        \
+        case 0: fn<float>(__VA_ARGS__); break;               \
+        case 1: fn<double>(__VA_ARGS__); break;              \
+        case 2: fn<cf32>(__VA_ARGS__); break;                \
+        default: fn<cf64>(__VA_ARGS__); break;               \
+    }
+
+void launch_gather_block(int scalar, const void* src, int64_t n, int64_t c0, int64_t cols, int vw, void* dst, hipStream_t s) {
+    KPMX_DISPATCH(scalar, gather_t, src, n, c0, cols, vw, dst, s)
+}
+void launch_scatter_block(int scalar, const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst, hipStream_t s) {
+    KPMX_DISPATCH(scalar, scatter_t, src, n, vw, c0, cols, dst, s)
+}
+void launch_scatter_scaled(int scalar, const void* src, int64_t n, int vw, int64_t cols, double scale, void* dst,
+                           int64_t col_stride, hipStream_t s) {
+    KPMX_DISPATCH(scalar, scatter_scaled_t, src, n, vw, cols, scale, dst, col_stride, s)
+}
+void launch_zero(int scalar, void* dst, int64_t count, hipStream_t s) { KPMX_DISPATCH(scalar, zero_t, dst, count, s) }
+void launch_zero_f64(double* dst, int64_t count, hipStream_t s) { zero_t<double>(dst, count, s); }
+void launch_csr_matvec(int scalar, const int32_t* indptr, const int32_t* indices, const void* vals, const void* x, int64_t n,
+                       void* y, hipStream_t s) {
+    KPMX_DISPATCH(scalar, matvec_t, indptr, indices, vals, x, n, y, s)
+}
+
+template <class T>
+static void lanczos_spmv_t(const int32_t* indptr, const int32_t* indices, const void* vals, const void* v1, void* v0,
+                           int64_t n, double b_prev, double* partials, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_lanczos_spmv<T>, dim3(nblocks), dim3(BLOCK), 0, s, indptr, indices, static_cast<const T*>(vals),
+                       static_cast<const T*>(v1), static_cast<T*>(v0), n, b_prev, partials);
+}
+template <class T>
+static void lanczos_axpy_t(const void* v1, void* v0, int64_t n, double a, double* partials, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_lanczos_axpy<T>, dim3(nblocks), dim3(BLOCK), 0, s, static_cast<const T*>(v1), static_cast<T*>(v0),
+                       n, a, partials);
+}
+template <class T>
+static void scale_t(void* v, int64_t n, double f, hipStream_t s) {
+    hipLaunchKernelGGL(k_scale<T>, grid_for(n), dim3(256), 0, s, static_cast<T*>(v), n, f);
+}
+void launch_lanczos_spmv(int scalar, const int32_t* indptr, const int32_t* indices, const void* vals, const void* v1,
+                         void* v0, int64_t n, double b_prev, double* partials, int nblocks, hipStream_t s) {
+    KPMX_DISPATCH(scalar, lanczos_spmv_t, indptr, indices, vals, v1, v0, n, b_prev, partials, nblocks, s)
+}
+void launch_lanczos_axpy(int scalar, const void* v1, void* v0, int64_t n, double a, double* partials, int nblocks,
+                         hipStream_t s) {
+    KPMX_DISPATCH(scalar, lanczos_axpy_t, v1, v0, n, a, partials, nblocks, s)
+}
+void launch_scale(int scalar, void* v, int64_t n, double f, hipStream_t s) { KPMX_DISPATCH(scalar, scale_t, v, n, f, s) }
+
+}  // namespace kpmx

@@ -1,0 +1,129 @@
+"""Direct, vectorised generators of the tight-binding Hamiltonians used by the benchmarks.
+
+They produce the same matrices the reference's model builder makes for these simple cases, in
+the reference's site order (sublattice-major, then a2, then a1 -- cppcore/include/system/
+Foundation.hpp:167), without going through the general `Model` machinery, so that multi-million
+site systems are built in seconds:
+
+- `honeycomb_supercell`: `pb.Model(graphene.monolayer(), pb.primitive(n1, n2),
+  pb.translational_symmetry())` plus an optional Anderson onsite term;
+- `square_flake`: a one-orbital square lattice L1 x L2 with open edges plus Anderson disorder;
+- `honeycomb_flake`: graphene cells within a rectangle, with dangling sites (fewer than 2
+  neighbours) removed like `Lattice.min_neighbors = 2` does.
+
+Graphene constants follow pybinding/repository/graphene/constants.py:3-8 and the hopping table of
+pybinding/repository/graphene/lattice.py:36-40.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+A_CC = 0.142        # nm, carbon-carbon distance
+A_LAT = 0.24595     # nm, graphene lattice constant
+T_GRAPHENE = -2.8   # eV
+
+# graphene.monolayer() hoppings from A(i, j) to B(i + di, j + dj)
+_HONEYCOMB_HOPS = ((0, 0), (1, -1), (0, -1))
+
+
+def _anderson(n, W, seed):
+    if not W:
+        return None
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-W / 2, W / 2, size=n)
+
+
+def _assemble(n, rows, cols, vals, onsite, dtype):
+    r = np.concatenate([rows, cols])
+    c = np.concatenate([cols, rows])
+    v = np.concatenate([vals, np.conj(vals)])
+    if onsite is not None:
+        d = np.arange(n)
+        r = np.concatenate([r, d])
+        c = np.concatenate([c, d])
+        v = np.concatenate([v, onsite])
+    h = sp.csr_matrix((v.astype(dtype), (r, c)), shape=(n, n))
+    h.sum_duplicates()
+    h.sort_indices()
+    return h
+
+
+def honeycomb_supercell(n1, n2, t=T_GRAPHENE, disorder=0.0, seed=2, dtype=np.float64):
+    """Periodic graphene supercell of n1 x n2 cells (2 n1 n2 sites), optional Anderson onsite
+    energies uniform in [-W/2, W/2] (numpy PCG64 `seed`)."""
+    i, j = np.meshgrid(np.arange(n1), np.arange(n2), indexing="xy")
+    i = i.ravel()
+    j = j.ravel()
+    ncell = n1 * n2
+    a_idx = j * n1 + i
+    rows, cols = [], []
+    for di, dj in _HONEYCOMB_HOPS:
+        bi = (i + di) % n1
+        bj = (j + dj) % n2
+        rows.append(a_idx)
+        cols.append(ncell + bj * n1 + bi)
+    rows = np.concatenate(rows)
+    cols = np.concatenate(cols)
+    vals = np.full(rows.size, t)
+    return _assemble(2 * ncell, rows, cols, vals, _anderson(2 * ncell, disorder, seed), dtype)
+
+
+def square_flake(l1, l2, t=-1.0, disorder=0.0, seed=None, dtype=np.float64, rng=None):
+    """Square lattice l1 x l2 (one orbital, nearest neighbours `t`) with open edges, optional
+    Anderson onsite energies uniform in [-W/2, W/2]."""
+    n = l1 * l2
+    idx = np.arange(n).reshape(l2, l1)
+    rows = np.concatenate([idx[:, :-1].ravel(), idx[:-1, :].ravel()])
+    cols = np.concatenate([idx[:, 1:].ravel(), idx[1:, :].ravel()])
+    vals = np.full(rows.size, t)
+    onsite = None
+    if disorder:
+        g = rng if rng is not None else np.random.default_rng(seed)
+        onsite = g.uniform(-disorder / 2, disorder / 2, size=n)
+    return _assemble(n, rows, cols, vals, onsite, dtype)
+
+
+def honeycomb_flake(width, height, t=T_GRAPHENE, dtype=np.float64, return_positions=False):
+    """Graphene flake: sites of the monolayer lattice (a1 = [a, 0], a2 = [a/2, a sqrt(3)/2],
+    A at [0, -a_cc/2], B at [0, a_cc/2]) inside the centred `width` x `height` rectangle, dangling
+    sites with fewer than 2 neighbours removed repeatedly."""
+    a1 = np.array([A_LAT, 0.0])
+    a2 = np.array([A_LAT / 2, A_LAT / 2 * np.sqrt(3)])
+    sub = np.array([[0.0, -A_CC / 2], [0.0, A_CC / 2]])
+    # cell index range covering the rectangle
+    inv = np.linalg.inv(np.stack([a1, a2], axis=1))
+    corners = np.array([[sx * width / 2, sy * height / 2] for sx in (-1, 1) for sy in (-1, 1)])
+    lc = corners @ inv.T
+    lo = np.floor(lc.min(axis=0)).astype(int) - 1
+    hi = np.ceil(lc.max(axis=0)).astype(int) + 1
+    n1, n2 = hi - lo + 1
+    i, j = np.meshgrid(np.arange(n1), np.arange(n2), indexing="xy")
+    i = i.ravel()
+    j = j.ravel()
+    base = (lo[0] + i)[:, None] * a1 + (lo[1] + j)[:, None] * a2
+    pos = np.concatenate([base + sub[0], base + sub[1]])
+    ncell = n1 * n2
+    inside = (np.abs(pos[:, 0]) < width / 2) & (np.abs(pos[:, 1]) < height / 2)
+    a_idx = j * n1 + i
+    edges = []
+    for di, dj in _HONEYCOMB_HOPS:
+        bi, bj = i + di, j + dj
+        ok = (bi >= 0) & (bi < n1) & (bj >= 0) & (bj < n2)
+        edges.append(np.stack([a_idx[ok], ncell + bj[ok] * n1 + bi[ok]], axis=1))
+    edges = np.concatenate(edges)
+    valid = inside.copy()
+    while True:
+        e_ok = valid[edges[:, 0]] & valid[edges[:, 1]]
+        deg = np.bincount(edges[e_ok, 0], minlength=2 * ncell) + np.bincount(edges[e_ok, 1], minlength=2 * ncell)
+        drop = valid & (deg < 2)
+        if not drop.any():
+            break
+        valid &= ~drop
+    new = -np.ones(2 * ncell, dtype=np.int64)
+    new[valid] = np.arange(valid.sum())
+    e_ok = valid[edges[:, 0]] & valid[edges[:, 1]]
+    rows = new[edges[e_ok, 0]]
+    cols = new[edges[e_ok, 1]]
+    h = _assemble(int(valid.sum()), rows, cols, np.full(rows.size, t), None, dtype)
+    if return_positions:
+        return h, pos[valid]
+    return h

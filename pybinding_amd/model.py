@@ -1,0 +1,83 @@
+"""Tight-binding model container used by `kpm()` (pybinding/model.py:14-162).
+
+`Model` holds the Hamiltonian CSR and the `System` (site positions, sublattice of each site,
+orbitals per site) that the KPM position-based methods need.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+__all__ = ['Model', 'System']
+
+
+class System:
+    """Site positions and sublattice ids of a built model (pybinding/system.py)."""
+
+    def __init__(self, x, y, z=None, sublattices=None, sublattice_names=None, norb=None):
+        self.x = np.asarray(x, dtype=np.float32)
+        self.y = np.asarray(y, dtype=np.float32)
+        self.z = np.zeros_like(self.x) if z is None else np.asarray(z, dtype=np.float32)
+        n = self.x.size
+        self.sublattices = np.zeros(n, dtype=np.int32) if sublattices is None else np.asarray(sublattices, np.int32)
+        self.sublattice_names = list(sublattice_names or ["A"])
+        self.norb = np.ones(n, dtype=np.int32) if norb is None else np.asarray(norb, np.int32)
+        self._offsets = np.concatenate([[0], np.cumsum(self.norb)])
+
+    @property
+    def num_sites(self):
+        return self.x.size
+
+    @property
+    def positions(self):
+        return self.x, self.y, self.z
+
+    @property
+    def is_multiorbital(self):
+        return bool(np.any(self.norb > 1))
+
+    @property
+    def sublattices_str(self):
+        return np.array(self.sublattice_names, dtype=object)[self.sublattices]
+
+    @property
+    def expanded_positions(self):
+        """Positions repeated once per orbital (System::expanded_positions)"""
+        rep = self.norb
+        return np.repeat(self.x, rep), np.repeat(self.y, rep), np.repeat(self.z, rep)
+
+    @property
+    def hamiltonian_size(self):
+        return int(self._offsets[-1])
+
+    def find_nearest(self, position, sublattice=""):
+        """Index of the site nearest to `position` (System::find_nearest: float distances,
+        the first site wins ties), optionally restricted to one sublattice."""
+        p = np.zeros(3, dtype=np.float32)
+        p[:len(position)] = np.asarray(position, dtype=np.float32)
+        d = (self.x - p[0]) ** 2 + (self.y - p[1]) ** 2 + (self.z - p[2]) ** 2
+        if sublattice:
+            sid = self.sublattice_names.index(sublattice)
+            d = np.where(self.sublattices == sid, d, np.inf)
+        return int(np.argmin(d))
+
+    def to_hamiltonian_indices(self, site):
+        """Hamiltonian indices of all orbitals of `site`"""
+        return list(range(int(self._offsets[site]), int(self._offsets[site + 1])))
+
+    def __getitem__(self, mask):
+        idx = np.nonzero(mask)[0] if np.asarray(mask).dtype == bool else np.asarray(mask)
+        s = System(self.x[idx], self.y[idx], self.z[idx], self.sublattices[idx], self.sublattice_names,
+                   self.norb[idx])
+        s.sub = s.sublattices_str
+        return s
+
+
+class Model:
+    """A built tight-binding model: Hamiltonian + System."""
+
+    def __init__(self, hamiltonian, system=None):
+        self.hamiltonian = sp.csr_matrix(hamiltonian)
+        self.system = system
+
+    @property
+    def raw_hamiltonian(self):
+        return self.hamiltonian

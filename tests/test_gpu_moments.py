@@ -1,0 +1,215 @@
+"""GPU parity: the HIP engine (through its C-ABI) against the oracle on identical seeded inputs.
+
+Tolerances (normwise, |mu_n| <= mu_0 makes them meaningful for every n):
+    fp64 / c128:  max_n |dmu_n| <= 1e-10 * |mu_0|
+    fp32 / c64 :  max_n |dmu_n| <= 1e-4  * |mu_0|
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import abi
+from pybinding_amd import generators
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.dtype(np.float64): 1e-10, np.dtype(np.complex128): 1e-10,
+       np.dtype(np.float32): 1e-4, np.dtype(np.complex64): 1e-4}
+DTYPES = [np.float64, np.float32, np.complex128, np.complex64]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = abi.Ctx(0)
+    yield c
+    c.close()
+
+
+def scaled_random(n, dtype, seed=0, density=None):
+    """A random Hermitian sparse matrix scaled into (-1, 1) (times 2, the H2 convention)."""
+    rng = np.random.default_rng(seed)
+    density = density or min(1.0, 6.0 / n)
+    a = sp.random(n, n, density=density, random_state=rng, format="csr")
+    a = a + a.T
+    if np.dtype(dtype).kind == "c":
+        b = sp.random(n, n, density=density / 2, random_state=rng, format="csr")
+        a = a + 1j * (b - b.T)
+    bound = abs(a).sum(axis=1).max()  # Gershgorin
+    h2 = sp.csr_matrix(a * (2.0 / (1.01 * bound))).astype(dtype)
+    h2.sort_indices()
+    return h2
+
+
+def graphene_h2(n1, n2, dtype, disorder=0.5):
+    h = generators.honeycomb_supercell(n1, n2, disorder=disorder, dtype=np.float64)
+    a = 0.5 * 2 * (3 * 2.8 + disorder / 2) * 1.01
+    return sp.csr_matrix(h * (2 / a)).astype(dtype)
+
+
+def normwise(mu, ref):
+    mu = np.atleast_2d(mu)
+    ref = np.atleast_2d(ref)
+    return np.max(np.abs(mu - ref) / np.abs(ref[:, :1]))
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("nvec", [1, 3, 8, 64, 70])
+def test_diagonal_block_vs_oracle(ctx, oracle, dtype, nvec):
+    dtype = np.dtype(dtype)
+    h2 = scaled_random(2000, dtype, seed=nvec)
+    r0 = oracle.random_starter(2000, nvec, dtype)
+    M = 130
+    mat = ctx.mat(h2)
+    blk = ctx.block(r0)
+    mu = abi.moments_diagonal(mat, blk, M)
+    blk.close()
+    mat.close()
+    ref = oracle.diag_moments(h2, r0, M)
+    print(dtype, nvec, normwise(mu, ref))
+    assert normwise(mu, ref) <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+def test_diagonal_sell_single_vector(ctx, oracle, dtype):
+    dtype = np.dtype(dtype)
+    h2 = graphene_h2(40, 30, dtype)
+    r0 = oracle.random_starter(h2.shape[0], 1, dtype)
+    mat = ctx.mat(h2, fmt=1)
+    blk = ctx.block(r0)
+    mu = abi.moments_diagonal(mat, blk, 258)
+    ref = oracle.diag_moments(h2, r0, 258)
+    blk.close()
+    mat.close()
+    assert normwise(mu, ref) <= TOL[dtype]
+
+
+def test_diagonal_graphene_long_fp64(ctx, oracle):
+    """2048 moments on a disordered periodic graphene supercell, 8 Rademacher vectors."""
+    h2 = graphene_h2(64, 48, np.float64)
+    r0 = oracle.random_starter(h2.shape[0], 8, np.float64)
+    mat = ctx.mat(h2)
+    blk = ctx.block(r0)
+    mu = abi.moments_diagonal(mat, blk, 2050)
+    ref = oracle.diag_moments(h2, r0, 2050)
+    blk.close()
+    mat.close()
+    assert normwise(mu, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("with_op", [False, True])
+def test_generic_vs_oracle(ctx, oracle, dtype, with_op):
+    dtype = np.dtype(dtype)
+    n = 1500
+    h2 = scaled_random(n, dtype, seed=3)
+    rng = np.random.default_rng(4)
+    alpha = oracle.random_starter(n, 1, dtype)[0]
+    beta = rng.standard_normal(n).astype(dtype)
+    op = sp.random(n, n, density=3.0 / n, random_state=5, format="csr").astype(dtype) if with_op else None
+    mat = ctx.mat(h2)
+    blk = ctx.block(alpha[None, :])
+    mu = abi.moments_generic(mat, blk, 101, beta, op)
+    ref = oracle.offdiag_moments(h2, alpha, 101, "generic", beta=beta, op=op)
+    blk.close()
+    mat.close()
+    scale = np.abs(ref).max()
+    assert np.abs(mu - ref).max() <= TOL[dtype] * scale * 10
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex64])
+def test_multi_unit_vs_oracle(ctx, oracle, dtype):
+    dtype = np.dtype(dtype)
+    n = 1200
+    h2 = scaled_random(n, dtype, seed=6)
+    e = np.zeros(n, dtype=dtype)
+    e[17] = 1
+    dest = [3, 17, 500, 3, 1199]
+    mat = ctx.mat(h2)
+    blk = ctx.block(e[None, :])
+    mu = abi.moments_multi_unit(mat, blk, 80, dest)
+    ref = oracle.offdiag_moments(h2, e, 80, "multi_unit", dest=dest)
+    blk.close()
+    mat.close()
+    assert np.abs(mu - ref).max() <= TOL[dtype] * 10
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
+@pytest.mark.parametrize("with_op", [False, True])
+def test_dense_vs_oracle(ctx, oracle, dtype, with_op):
+    dtype = np.dtype(dtype)
+    n = 700
+    h2 = scaled_random(n, dtype, seed=7)
+    r0 = oracle.random_starter(n, 2, dtype)
+    op = sp.random(n, n, density=4.0 / n, random_state=8, format="csr").astype(dtype) if with_op else None
+    mat = ctx.mat(h2)
+    opm = ctx.mat(op) if op is not None else None
+    blk = ctx.block(r0)
+    d = abi.moments_dense(mat, blk, 40, opm)
+    for c in range(2):
+        ref = oracle.offdiag_moments(h2, r0[c], 40, "dense", op=op)
+        assert np.abs(d[c] - ref).max() <= TOL[dtype] * np.abs(ref).max() * 10
+    blk.close()
+    mat.close()
+    if opm is not None:
+        opm.close()
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.float64])
+def test_kubo_contraction_vs_fp64_gemm(ctx, oracle, dtype):
+    """mu_nm = (1/R) sum_r L_r R_r^H against an FP64 numpy GEMM of oracle panels; the FP32
+    split-precision MFMA contraction is reported with its error against the same oracle."""
+    dtype = np.dtype(dtype)
+    n, M, R = 900, 64, 3
+    h2 = scaled_random(n, dtype, seed=9)
+    x = np.random.default_rng(10).uniform(-5, 5, n).astype(np.float32)
+    y = np.random.default_rng(11).uniform(-5, 5, n).astype(np.float32)
+    vl = oracle.velocity(h2, x).astype(dtype)
+    vr = oracle.velocity(h2, y).astype(dtype)
+    r0 = oracle.random_starter(n, R, dtype)
+    ref = np.zeros((M, M), dtype=np.complex128)
+    for k in range(R):
+        L = oracle.offdiag_moments(h2, vl @ r0[k], M, "dense")
+        Rr = oracle.offdiag_moments(h2, r0[k], M, "dense", op=vr)
+        ref += L.astype(np.complex128) @ Rr.astype(np.complex128).conj().T
+    ref /= R
+    mat, ml, mr = ctx.mat(h2), ctx.mat(vl), ctx.mat(vr)
+    blk = ctx.block(r0)
+    mu64 = abi.kubo_moments(mat, ml, mr, blk, M, mode=0)
+    mu32 = abi.kubo_moments(mat, ml, mr, blk, M, mode=1)
+    for h in (blk, mat, ml, mr):
+        h.close()
+    scale = np.abs(ref).max()
+    err64 = np.abs(mu64 - ref).max() / scale
+    err32 = np.abs(mu32 - ref).max() / scale
+    print("kubo fp64 err %.3e  fp32-split err %.3e" % (err64, err32))
+    assert err64 <= 1e-11
+    assert err32 <= 1e-5
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
+def test_lanczos_bounds(ctx, dtype):
+    h = generators.honeycomb_supercell(12, 10, disorder=1.0, dtype=np.float64)
+    if np.dtype(dtype).kind == "c":
+        h = h.astype(np.complex128)
+    h = h.astype(dtype)
+    ev = np.linalg.eigvalsh(h.toarray().astype(np.complex128))
+    mat = ctx.mat(h)
+    lo, hi, loops = abi.lanczos_bounds(mat, 0.002)
+    mat.close()
+    assert loops > 0
+    assert abs(lo - ev[0]) <= 1e-3 * abs(ev[0])
+    assert abs(hi - ev[-1]) <= 1e-3 * abs(ev[-1])
+
+
+def test_errors_are_reported(ctx):
+    h2 = scaled_random(100, np.float64)
+    mat = ctx.mat(h2)
+    blk = ctx.block(np.ones((1, 50)))
+    with pytest.raises(RuntimeError, match="Size mismatch"):
+        abi.moments_diagonal(mat, blk, 10)
+    blk.close()
+    blk = ctx.block(np.ones((1, 100)))
+    with pytest.raises(RuntimeError, match="even"):
+        abi.moments_diagonal(mat, blk, 11)
+    blk.close()
+    mat.close()
