@@ -39,19 +39,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(n1, n2, disorder, dtype):
+def build_workload(n1, n2, disorder, dtype, order="bfs"):
     from pybinding_amd import generators
     from pybinding_amd._native import load_module
+    import scipy.sparse as sp
     native = load_module()
     t0 = time.time()
     h = generators.honeycomb_supercell(n1, n2, disorder=disorder, seed=2, dtype=dtype)
     log("built H: n=%d nnz=%d (%.1fs)" % (h.shape[0], h.nnz, time.time() - t0))
     t0 = time.time()
-    ip, ix, dv, rmap, borders, _, _, (a, b) = native.optimize(h.shape, h.indptr, h.indices, h.data, (-8.9, 8.9),
-                                                               True, [0], [0])
-    import scipy.sparse as sp
+    if order == "hilbert":
+        q = generators.honeycomb_cell_order(n1, n2)
+        inv = np.argsort(q)
+        hp = h[inv][:, inv].tocsr()
+        hp.sort_indices()
+        ip, ix, dv, _, borders, _, _, (a, b) = native.optimize(hp.shape, hp.indptr, hp.indices, hp.data, (-8.9, 8.9),
+                                                               False, [0], [0])
+        rmap = q.astype(np.int32)
+    else:
+        ip, ix, dv, rmap, borders, _, _, (a, b) = native.optimize(h.shape, h.indptr, h.indices, h.data, (-8.9, 8.9),
+                                                                   True, [0], [0])
     h2 = sp.csr_matrix((dv, ix, ip), shape=h.shape)
-    log("optimized H2 (scale a=%.6f b=%.3g, BFS reorder, %d slices) (%.1fs)" % (a, b, len(borders), time.time() - t0))
+    log("optimized H2 (scale a=%.6f b=%.3g, %s order, %d slices) (%.1fs)" % (a, b, order, len(borders), time.time() - t0))
     return h2, rmap, borders, native
 
 
@@ -95,10 +104,24 @@ def main():
     ap.add_argument("--vectors", type=int, default=64)
     ap.add_argument("--cells", type=int, nargs=2, default=(2048, 1024))
     ap.add_argument("--dtype", choices=["float64", "float32"], default="float64")
-    ap.add_argument("--format", choices=["CSR", "SELL"], default="CSR")
+    ap.add_argument("--format", choices=["CSR", "SELL"], default="SELL")
+    ap.add_argument("--order", choices=["bfs", "hilbert"], default="hilbert",
+                    help="site order of H2: the reference's BFS reorder, or cells along a Hilbert curve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-vector (plain) comparison")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 2 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import torch
+        a = torch.ones(2**28, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8}))
+        return
 
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU path (vectors split across GPUs "
@@ -108,7 +131,7 @@ def main():
     from pybinding_amd import abi
     dtype = np.dtype(args.dtype)
     s = dtype.itemsize
-    h2, rmap, borders, native = build_workload(args.cells[0], args.cells[1], 0.5, dtype)
+    h2, rmap, borders, native = build_workload(args.cells[0], args.cells[1], 0.5, dtype, args.order)
     n, nnz = h2.shape[0], h2.nnz
     M, R = args.moments, args.vectors
     r0 = starters(native, n, R, dtype, rmap)
@@ -117,6 +140,8 @@ def main():
     mat = ctx.mat(h2, fmt=1 if args.format == "SELL" else 0)
     blk = ctx.block(r0)
     ctx.synchronize()
+    variant = abi.autotune_diagonal(mat, blk)
+    log("autotuned step kernel variant: %d" % variant)
 
     for _ in range(args.warmup):
         abi.moments_diagonal_run(mat, blk, M)
@@ -191,7 +216,8 @@ def main():
         "data": "synthetic (seeded disorder, reference mt19937 starters)",
         "config": {"workload": "periodic honeycomb %dx%d cells (N=%d, nnz=%d), Anderson W=1 eV, %d moments, "
                                "%d random vectors as one block SpMM" % (args.cells[0], args.cells[1], n, nnz, M, R),
-                   "matrix_format": args.format, "moments": M, "vectors": R},
+                   "matrix_format": args.format, "site_order": args.order, "kernel_variant": variant,
+                   "moments": M, "vectors": R},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None},
         "cpu_baseline": cpu,

@@ -96,6 +96,12 @@ int kpmx_block_destroy(kpmx_block* b);
  * mu: host, num_vectors x num_moments, scalar type of h2. */
 int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu);
 
+/* Autotuner (no counterpart in the reference, whose CPU kernels have one fixed shape): time the
+ * tile-shape variants of the fused diagonal step kernel on this matrix and starter block and keep
+ * the fastest for every later recurrence of that scalar type and vector width on the context.
+ * variant (optional): the chosen variant index. */
+int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant);
+
 /* Same recurrence, results kept as raw per-step sums on the device; for timing loops.
  * Leaves the sums readable through kpmx_moments_diagonal_fetch. */
 int kpmx_moments_diagonal_run(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments);

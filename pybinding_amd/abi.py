@@ -116,6 +116,13 @@ def moments_diagonal(mat, block, num_moments):
     return out
 
 
+def autotune_diagonal(mat, block):
+    """Pick the fastest tile shape of the diagonal step kernel for this matrix and block."""
+    v = ctypes.c_int()
+    _check(lib().kpmx_autotune_diagonal(mat.h, block.h, ctypes.byref(v)))
+    return v.value
+
+
 def moments_diagonal_run(mat, block, num_moments):
     """Run the diagonal recurrence and keep the raw sums on the device (no host transfer)."""
     _check(lib().kpmx_moments_diagonal_run(mat.h, block.h, ctypes.c_int64(num_moments)))

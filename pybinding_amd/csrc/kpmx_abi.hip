@@ -99,6 +99,7 @@ struct kpmx_ctx {
     hipStream_t stream = nullptr;
     Workspace partials, sums, work0, work1, aux0, aux1, aux2, panel_l, panel_r;
     hipEvent_t t0 = nullptr, t1 = nullptr;
+    int tune[4][7] = {};  // tile-shape variant per (scalar, log2 vw), chosen by kpmx_autotune_diagonal
 };
 
 struct kpmx_mat {
@@ -111,6 +112,7 @@ struct kpmx_mat {
     void* vals = nullptr;
     // SELL-64 copy (format == SELL)
     int64_t nslices = 0;
+    int max_width = 0;
     int64_t* slice_ptr = nullptr;
     int32_t* slice_width = nullptr;
     int32_t* sell_indices = nullptr;
@@ -167,18 +169,20 @@ StepLaunch base_launch(const kpmx_mat* h2, int vw, bool use_sell) {
     L.scalar = h2->scalar;
     L.vw = vw;
     L.n = h2->rows;
-    if (use_sell && vw == 1 && h2->format == KPMX_FMT_SELL) {
+    if (use_sell && (vw == 1 || vw >= 4) && h2->format == KPMX_FMT_SELL) {
         L.format = FORMAT_SELL;
         L.indices = h2->sell_indices;
         L.vals = h2->sell_vals;
         L.slice_ptr = h2->slice_ptr;
         L.slice_width = h2->slice_width;
+        L.max_width = h2->max_width;
     } else {
         L.format = FORMAT_CSR;
         L.indptr = h2->indptr;
         L.indices = h2->indices;
         L.vals = h2->vals;
     }
+    L.num_cu = h2->ctx->num_cu;
     L.num_wg = grid_size(h2->ctx, (h2->rows + step_rows_per_tile(L.format, vw) - 1) / step_rows_per_tile(L.format, vw));
     return L;
 }
@@ -191,9 +195,15 @@ void launch(const StepLaunch& L, hipStream_t s) {
 // Run the Chebyshev recurrence r_{k+1} = H2 r_k - r_{k-1} over one tile for `steps` launches
 // (k = 0 is the first step r_1 = 0.5 H2 r_0).  `per_step` customises each launch (collection
 // pointers) and the partials of reduction modes are summed into `sums` (steps x 4vw doubles).
+int log2i(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
 template <class PerStep, class PostStep>
 void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, const void* w, double* sums,
-                    PerStep per_step, PostStep post_step) {
+                    PerStep per_step, PostStep post_step, int tune = -1) {
     kpmx_ctx* ctx = h2->ctx;
     const int64_t n = h2->rows;
     const size_t sb = scalar_bytes(h2->scalar);
@@ -203,6 +213,8 @@ void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, 
     StepLaunch L = base_launch(h2, t.vw, true);
     L.mode = mode;
     L.w = w;
+    L.tune = tune >= 0 ? tune : ctx->tune[h2->scalar][log2i(t.vw)];
+    L.num_wg = step_grid(L);  // the SELL block kernel runs a resident grid
     const int width = 4 * t.vw;
     int64_t chunk = 1;
     double* partials = nullptr;
@@ -509,6 +521,7 @@ int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t
                 }
             }
             m->nslices = ns;
+            m->max_width = ns ? *std::max_element(swidth.begin(), swidth.end()) : 0;
             m->slice_ptr = static_cast<int64_t*>(dev_alloc((ns + 1) * sizeof(int64_t)));
             m->slice_width = static_cast<int32_t*>(dev_alloc(ns * sizeof(int32_t)));
             m->sell_indices = static_cast<int32_t*>(dev_alloc(total * sizeof(int32_t)));
@@ -997,6 +1010,39 @@ int kpmx_ctx_timer_stop(kpmx_ctx* ctx, double* ms) {
         float f = 0;
         KPMX_HIP(hipEventElapsedTime(&f, ctx->t0, ctx->t1));
         *ms = (double)f;
+    });
+}
+
+int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant) {
+    return guarded([&] {
+        check_pair(h2, r0);
+        kpmx_ctx* ctx = h2->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const Tile& t = r0->tiles.front();
+        const int64_t steps = 9;
+        double* sums = static_cast<double*>(ctx->sums.get((size_t)steps * 4 * t.vw * sizeof(double)));
+        if (!ctx->t0) KPMX_HIP(hipEventCreate(&ctx->t0));
+        if (!ctx->t1) KPMX_HIP(hipEventCreate(&ctx->t1));
+        int best = 0;
+        float best_ms = 0;
+        for (int v = 0; v < NUM_TUNE_VARIANTS; ++v) {
+            for (int rep = 0; rep < 2; ++rep) {  // first repetition warms caches and code
+                KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
+                run_recurrence(h2, t, MODE_DIAG, steps, nullptr, sums, [](StepLaunch&, int64_t) {},
+                               [](StepLaunch const&, int64_t) {}, v);
+                KPMX_HIP(hipEventRecord(ctx->t1, ctx->stream));
+                KPMX_HIP(hipEventSynchronize(ctx->t1));
+            }
+            float ms = 0;
+            KPMX_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+            if (v == 0 || ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        for (const Tile& tt : r0->tiles)
+            if (tt.vw == t.vw) ctx->tune[h2->scalar][log2i(t.vw)] = best;
+        if (variant) *variant = best;
     });
 }
 }  // extern "C"

@@ -58,4 +58,20 @@ template <class R> __device__ __forceinline__ d2 cdot(cplx<R> a, cplx<R> b) {
             double(a.re) * double(b.im) - double(a.im) * double(b.re)};
 }
 
+// Non-temporal (streaming) loads and stores: data touched once per step, kept from evicting
+// reused data in L2.
+__device__ __forceinline__ float nt_load(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ double nt_load(const double* p) { return __builtin_nontemporal_load(p); }
+template <class R> __device__ __forceinline__ cplx<R> nt_load(const cplx<R>* p) {
+    const R* q = reinterpret_cast<const R*>(p);
+    return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
+__device__ __forceinline__ void nt_store(float v, float* p) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void nt_store(double v, double* p) { __builtin_nontemporal_store(v, p); }
+template <class R> __device__ __forceinline__ void nt_store(cplx<R> v, cplx<R>* p) {
+    R* q = reinterpret_cast<R*>(p);
+    __builtin_nontemporal_store(v.re, q);
+    __builtin_nontemporal_store(v.im, q + 1);
+}
+
 }  // namespace kpmx

@@ -13,12 +13,14 @@ struct StepLaunch {
     int format = FORMAT_CSR;
     int vw = 1;              // vectors per row (power of two <= 64)
     int mode = MODE_DIAG;
-    int num_wg = 0;
+    int num_wg = 0;          // grid size; 0 = resident grid chosen by the launcher (SELL block kernel)
+    int num_cu = 256;
     const int32_t* indptr = nullptr;   // CSR row pointers (CSR format)
     const int32_t* indices = nullptr;  // CSR or SELL column indices
     const void* vals = nullptr;
     const int64_t* slice_ptr = nullptr;  // SELL
     const int32_t* slice_width = nullptr;
+    int max_width = 0;                   // widest SELL slice
     const void* x = nullptr;
     const void* yin = nullptr;
     void* yout = nullptr;
@@ -32,9 +34,13 @@ struct StepLaunch {
     int64_t n = 0;
     double hscale = 1.0;
     int sub_y = 1;
+    bool query_grid = false;  // launch_step returns the resident grid instead of launching
+    int tune = 0;             // tile-shape variant of the SELL block kernel (0 default, 1 smaller, 2 larger)
 };
 
 int launch_step(const StepLaunch& L, hipStream_t stream);
+int step_grid(const StepLaunch& L);
+constexpr int NUM_TUNE_VARIANTS = 3;
 int64_t step_rows_per_tile(int format, int vw);
 void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
                             hipStream_t stream);

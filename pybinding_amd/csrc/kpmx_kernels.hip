@@ -194,29 +194,171 @@ __global__ __launch_bounds__(BLOCK) void k_step_sell(StepArgs<T> a, const int64_
     if constexpr (MODE != MODE_NONE) block_reduce_store<1>(a_re, a_im, b_re, b_im, a.partials);
 }
 
+// Sliced ELL, block of VW >= 4 vectors.  Same matrix layout as above (64-row slices, column-major
+// inside a slice).  A workgroup iteration covers a tile of TR = 4 * RW rows (RW = max(4, 64/VW)
+// rows per wave, i.e. U = RW / (64/VW) row groups of 64/VW rows), so the rows in flight on one
+// XCD -- resident workgroups x TR -- stay a band of a few thousand rows whose gathered x rows fit
+// in that XCD's L2.  The tile's first KMAX columns (indices and values) are staged in LDS with one
+// coalesced load, columns beyond a slice's width padded with (own row, 0); every global load of
+// the tile's rows -- y, own x and U*KMAX gathered x rows per lane -- is issued before the first
+// multiply.  y is streamed with non-temporal loads and stores so that it does not evict x from
+// L2.  Columns beyond KMAX (wider slices) fall back to a loop over global memory.
+template <class T, int VW, int MODE, int KMAX, int RW>
+__global__ __launch_bounds__(BLOCK) void k_step_sellb(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
+                                                      const int32_t* __restrict__ slice_width) {
+    constexpr int RPW = WAVE / VW;                 // rows per row group
+    constexpr int U = RW / RPW;                    // row groups per wave per tile
+    constexpr int TR = RW * WAVES_PER_BLOCK;       // rows per tile (divides 64)
+    constexpr int TPS = WAVE / TR;                 // tiles per slice
+    static_assert(VW >= 4 && RW % RPW == 0 && WAVE % TR == 0, "bad block configuration");
+    __shared__ int32_t s_idx[KMAX * TR];
+    __shared__ T s_val[KMAX * TR];
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    const int rsub = lane / VW;
+    const int col = lane % VW;
+    const int32_t* __restrict__ sidx = a.indices;
+    const T* __restrict__ sval = a.vals;
+    const T* __restrict__ x = a.x;
+    double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
+    const TileWalk tw = tile_walk(a.num_tiles);
+    // Software pipeline: the next tile's slice header, indices and values are loaded into
+    // registers while the current tile's gathers are in flight, then moved to LDS.
+    constexpr int E = (KMAX * TR + BLOCK - 1) / BLOCK;  // staged entries per thread
+    int32_t p_idx[E];
+    T p_val[E];
+    auto prefetch = [&](int64_t tile, int64_t& pbase, int& pwidth) {
+        const int64_t slc = tile / TPS;
+        const int rr0 = (int)(tile % TPS) * TR;
+        pbase = slice_ptr[slc];
+        pwidth = slice_width[slc];
+        const int kk = pwidth < KMAX ? pwidth : KMAX;
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * BLOCK;
+            const int k = e / TR, r = e % TR;
+            if (e < KMAX * TR && k < kk) {
+                p_idx[q] = sidx[pbase + (int64_t)k * WAVE + rr0 + r];
+                p_val[q] = sval[pbase + (int64_t)k * WAVE + rr0 + r];
+            } else {
+                const int64_t own = slc * WAVE + rr0 + r;
+                p_idx[q] = (int32_t)(own < a.n ? own : a.n - 1);
+                p_val[q] = zero(T{});
+            }
+        }
+    };
+    int64_t next_base = 0;
+    int next_width = 0;
+    if (tw.first < tw.end) prefetch(tw.first, next_base, next_width);
+    for (int64_t tile = tw.first; tile < tw.end; tile += tw.stride) {
+        const int64_t slice = tile / TPS;
+        const int r0 = (int)(tile % TPS) * TR;  // first row of the tile inside its slice
+        const int64_t base = next_base;
+        const int width = next_width;
+        __syncthreads();  // the previous tile's LDS entries are no longer read
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * BLOCK;
+            if (e < KMAX * TR) {
+                s_idx[e] = p_idx[q];
+                s_val[e] = p_val[q];
+            }
+        }
+        __syncthreads();
+        int rin[U];
+        int64_t row[U];
+        bool ok[U];
+        T yv[U], xo[U];
+        T xg[U][KMAX];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rin[u] = wave * RW + u * RPW + rsub;  // row inside the tile
+            row[u] = slice * WAVE + r0 + rin[u];
+            ok[u] = row[u] < a.n;
+            const int64_t e = (ok[u] ? row[u] : a.n - 1) * VW + col;
+            yv[u] = a.sub_y ? nt_load(&a.yin[e]) : zero(T{});
+            if (MODE == MODE_DIAG) xo[u] = x[e];
+        }
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u) xg[u][k] = x[(int64_t)s_idx[k * TR + rin[u]] * VW + col];
+        // the next tile's header and indices go out while this tile's gathers are in flight
+        if (tile + tw.stride < tw.end) prefetch(tile + tw.stride, next_base, next_width);
+        T acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc[u] = zero(T{});
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) acc[u] = fmac(s_val[k * TR + rin[u]], xg[u][k], acc[u]);
+        }
+        for (int k = KMAX; k < width; ++k) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t off = base + (int64_t)k * WAVE + r0 + rin[u];
+                acc[u] = fmac(sval[off], x[(int64_t)sidx[off] * VW + col], acc[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!ok[u]) continue;
+            const int64_t e = row[u] * VW + col;
+            T ynew = scal(acc[u], a.hscale);
+            if (a.sub_y) ynew = sub(ynew, yv[u]);
+            if (a.yout) nt_store(ynew, &a.yout[e]);
+            if (a.copy_out) a.copy_out[col * a.copy_col_stride + row[u]] = ynew;
+            if (a.mark) {
+                const int32_t slot = a.mark[row[u]];
+                if (slot >= 0) a.mark_out[(int64_t)slot * a.mark_stride + col] = ynew;
+            }
+            if constexpr (MODE == MODE_DIAG) {
+                a_re += sqabs(xo[u]);
+                const d2 p = cdot(ynew, xo[u]);
+                b_re += p.re;
+                b_im += p.im;
+            } else if constexpr (MODE == MODE_DOT) {
+                const T wv = a.w[e];
+                const d2 p = cdot(wv, x[e]);
+                a_re += p.re;
+                a_im += p.im;
+                const d2 q = cdot(wv, ynew);
+                b_re += q.re;
+                b_im += q.im;
+            }
+        }
+    }
+    if constexpr (MODE != MODE_NONE) block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials);
+}
+
+// Rows per wave per tile of the SELL block kernel: the tunable choices for a vector width
+// (tune index 0 is the default).  Each must be a multiple of 64/VW and at most 16.
+template <int VW>
+constexpr int sellb_rw(int tune) {
+    constexpr int RPW = WAVE / VW;
+    constexpr int base = RPW > 4 ? RPW : 4;
+    return tune == 1 ? (base > RPW ? base / 2 : base) : tune == 2 ? (base * 2 <= 16 ? base * 2 : base) : base;
+}
+
 // sums[s][j] = sum over workgroups g (in order) of partials[s][g][j], for s in [0, nsteps).
-// One block per step; thread t sums the workgroup subset t / width (strided), then the subsets
-// are combined in order through LDS.
+// Block (s, chunk) covers columns [64 chunk, 64 chunk + 64) of step s: 4 groups of 64 threads
+// sum interleaved workgroup subsets, then the 4 subtotals are combined in order through LDS.
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* partials, int64_t num_wg, int width,
                                                          double* sums) {
-    __shared__ double red[256];
-    const int s = blockIdx.x;
+    __shared__ double red[4][64];
+    const int chunks = (width + 63) / 64;
+    const int s = blockIdx.x / chunks;
+    const int chunk = blockIdx.x % chunks;
     const double* p = partials + (int64_t)s * num_wg * width;
-    const int groups = 256 / width;  // width divides 256 (width = 4*VW, VW power of two <= 64)
     const int t = threadIdx.x;
-    const int j = t % width;
-    const int g = t / width;
+    const int cl = t % 64, g = t / 64;
+    const int j = chunk * 64 + cl;
     double acc = 0;
-    if (g < groups) {
-        for (int64_t w = g; w < num_wg; w += groups) acc += p[w * width + j];
+    if (j < width) {
+        for (int64_t w = g; w < num_wg; w += 4) acc += p[w * width + j];
     }
-    red[t] = acc;
+    red[g][cl] = acc;
     __syncthreads();
-    if (t < width) {
-        double tot = red[t];
-        for (int q = 1; q < groups; ++q) tot += red[q * width + t];
-        sums[(int64_t)s * width + t] = tot;
-    }
+    if (g == 0 && j < width) sums[(int64_t)s * width + j] = ((red[0][cl] + red[1][cl]) + red[2][cl]) + red[3][cl];
 }
 
 // dst (n x vw, row-major) <- columns [c0, c0 + cols) of src (nvec x n, vector-major), zero padded.
@@ -327,6 +469,59 @@ __global__ void k_scale(T* v, int64_t n, double f) {
 }
 
 // ------------------------------------------------------------------------------------ launchers
+
+// Launch the SELL block kernel with a grid of exactly the workgroups that can be resident
+// (occupancy x CUs, a multiple of 8), so that every workgroup is live from the start and the
+// XCD-aware tile walk advances all of them through their band together.
+template <class K>
+static int resident_grid(K kernel, int num_cu, int64_t num_tiles) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, BLOCK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    int64_t g = (int64_t)per_cu * (num_cu > 0 ? num_cu : 256);
+    if (g > num_tiles) g = num_tiles;
+    if (g >= 8) g -= g % 8;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+template <class T, int VW, int MODE, int KMAX, int RW>
+static int launch_sellb_rw(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
+    auto kernel = k_step_sellb<T, VW, MODE, KMAX, RW>;
+    constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
+    a.num_tiles = (L.n + tr - 1) / tr;
+    if (L.query_grid) return resident_grid(kernel, L.num_cu, a.num_tiles);
+    const int grid = L.num_wg > 0 ? L.num_wg : resident_grid(kernel, L.num_cu, a.num_tiles);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, L.slice_ptr, L.slice_width);
+    return 0;
+}
+
+template <class T, int VW, int MODE, int KMAX>
+static int launch_sellb_k(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
+    // the diagonal (hot) path carries the tuned variants, the other modes use the default shape
+    const int tune = (MODE == MODE_DIAG) ? L.tune : 0;
+    switch (tune) {
+        case 1: return launch_sellb_rw<T, VW, MODE, KMAX, sellb_rw<VW>(1)>(a, L, stream);
+        case 2: return launch_sellb_rw<T, VW, MODE, KMAX, sellb_rw<VW>(2)>(a, L, stream);
+        default: return launch_sellb_rw<T, VW, MODE, KMAX, sellb_rw<VW>(0)>(a, L, stream);
+    }
+}
+
+template <class T, int VW>
+static int launch_sellb(StepArgs<T> a, const StepLaunch& L, bool narrow, hipStream_t stream) {
+    if (narrow) {
+        switch (L.mode) {
+            case MODE_DIAG: return launch_sellb_k<T, VW, MODE_DIAG, 4>(a, L, stream);
+            case MODE_DOT: return launch_sellb_k<T, VW, MODE_DOT, 4>(a, L, stream);
+            default: return launch_sellb_k<T, VW, MODE_NONE, 4>(a, L, stream);
+        }
+    }
+    switch (L.mode) {
+        case MODE_DIAG: return launch_sellb_k<T, VW, MODE_DIAG, 8>(a, L, stream);
+        case MODE_DOT: return launch_sellb_k<T, VW, MODE_DOT, 8>(a, L, stream);
+        default: return launch_sellb_k<T, VW, MODE_NONE, 8>(a, L, stream);
+    }
+}
+
 template <class T>
 static int launch_step_t(const StepLaunch& L, hipStream_t stream) {
     StepArgs<T> a;
@@ -347,6 +542,17 @@ static int launch_step_t(const StepLaunch& L, hipStream_t stream) {
     a.hscale = static_cast<typename traits<T>::real>(L.hscale);
     a.sub_y = L.sub_y;
     const dim3 grid(L.num_wg), block(BLOCK);
+    if (L.format == FORMAT_SELL && L.vw >= 4) {
+        const bool narrow = L.max_width <= 4;
+        switch (L.vw) {
+            case 4: return launch_sellb<T, 4>(a, L, narrow, stream);
+            case 8: return launch_sellb<T, 8>(a, L, narrow, stream);
+            case 16: return launch_sellb<T, 16>(a, L, narrow, stream);
+            case 32: return launch_sellb<T, 32>(a, L, narrow, stream);
+            case 64: return launch_sellb<T, 64>(a, L, narrow, stream);
+            default: return -1;
+        }
+    }
     if (L.format == FORMAT_SELL) {
         if (L.vw != 1) return -1;
         a.num_tiles = (L.n + BLOCK - 1) / BLOCK;
@@ -380,6 +586,14 @@ static int launch_step_t(const StepLaunch& L, hipStream_t stream) {
 #undef KPMX_VW_CASE
 }
 
+// Resident grid of the SELL block kernel for this configuration (L.num_wg otherwise).
+int step_grid(const StepLaunch& L) {
+    if (!(L.format == FORMAT_SELL && L.vw >= 4)) return L.num_wg;
+    StepLaunch q = L;
+    q.query_grid = true;
+    return launch_step(q, nullptr);
+}
+
 int launch_step(const StepLaunch& L, hipStream_t stream) {
     switch (L.scalar) {
         case 0: return launch_step_t<float>(L, stream);
@@ -391,13 +605,17 @@ int launch_step(const StepLaunch& L, hipStream_t stream) {
 }
 
 int64_t step_rows_per_tile(int format, int vw) {
-    return format == FORMAT_SELL ? (int64_t)BLOCK : (int64_t)(WAVE / vw) * WAVES_PER_BLOCK;
+    if (format == FORMAT_SELL) return vw >= 4 ? (int64_t)(WAVE / vw > 4 ? WAVE / vw : 4) * WAVES_PER_BLOCK : (int64_t)BLOCK;
+    // (the SELL block kernel's own tile size depends on the tuned shape; its grid comes from step_grid)
+    return (int64_t)(WAVE / vw) * WAVES_PER_BLOCK;
 }
 
 void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
                             hipStream_t stream) {
     if (nsteps <= 0) return;
-    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)nsteps), dim3(256), 0, stream, partials, num_wg, width, sums);
+    const int chunks = (width + 63) / 64;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)(nsteps * chunks)), dim3(256), 0, stream, partials, num_wg, width,
+                       sums);
 }
 
 static dim3 grid_for(int64_t count) {

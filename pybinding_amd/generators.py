@@ -25,6 +25,43 @@ T_GRAPHENE = -2.8   # eV
 _HONEYCOMB_HOPS = ((0, 0), (1, -1), (0, -1))
 
 
+def hilbert_index(ix, iy, order):
+    """Position along the Hilbert curve of the 2^order x 2^order grid for integer cells (ix, iy)
+    (the classic rotate-and-reflect construction, vectorised)."""
+    ix = np.asarray(ix, dtype=np.int64).copy()
+    iy = np.asarray(iy, dtype=np.int64).copy()
+    d = np.zeros_like(ix)
+    s = 1 << (order - 1)
+    while s > 0:
+        rx = ((ix & s) > 0).astype(np.int64)
+        ry = ((iy & s) > 0).astype(np.int64)
+        d += s * s * ((3 * rx) ^ ry)
+        # rotate the quadrant: reflect when (rx, ry) = (1, 0), transpose when ry = 0 (only the
+        # bits below s matter from here on, so reflecting within s is enough)
+        flip = ry == 0
+        refl = flip & (rx == 1)
+        ix = np.where(refl, s - 1 - ix, ix)
+        iy = np.where(refl, s - 1 - iy, iy)
+        ix, iy = np.where(flip, iy, ix), np.where(flip, ix, iy)
+        s >>= 1
+    return d
+
+
+def honeycomb_cell_order(n1, n2):
+    """A locality order of the honeycomb supercell's sites: cells along a Hilbert curve, the two
+    sublattice sites of a cell adjacent.  Returns q with q[original index] = new index."""
+    i, j = np.meshgrid(np.arange(n1), np.arange(n2), indexing="xy")
+    order = int(np.ceil(np.log2(max(n1, n2, 2))))
+    key = hilbert_index(i.ravel(), j.ravel(), order)
+    cell_rank = np.empty(n1 * n2, dtype=np.int64)
+    cell_rank[np.argsort(key, kind="stable")] = np.arange(n1 * n2)
+    ncell = n1 * n2
+    q = np.empty(2 * ncell, dtype=np.int64)
+    q[:ncell] = 2 * cell_rank
+    q[ncell:] = 2 * cell_rank + 1
+    return q
+
+
 def _anderson(n, W, seed):
     if not W:
         return None

@@ -53,13 +53,14 @@ def normwise(mu, ref):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("nvec", [1, 3, 8, 64, 70])
-def test_diagonal_block_vs_oracle(ctx, oracle, dtype, nvec):
+@pytest.mark.parametrize("nvec", [1, 2, 3, 8, 16, 32, 64, 70])
+@pytest.mark.parametrize("fmt", [0, 1], ids=["csr", "sell"])
+def test_diagonal_block_vs_oracle(ctx, oracle, dtype, nvec, fmt):
     dtype = np.dtype(dtype)
-    h2 = scaled_random(2000, dtype, seed=nvec)
-    r0 = oracle.random_starter(2000, nvec, dtype)
+    h2 = scaled_random(2000 + 37 * nvec, dtype, seed=nvec)  # ragged last slice
+    r0 = oracle.random_starter(h2.shape[0], nvec, dtype)
     M = 130
-    mat = ctx.mat(h2)
+    mat = ctx.mat(h2, fmt=fmt)
     blk = ctx.block(r0)
     mu = abi.moments_diagonal(mat, blk, M)
     blk.close()
