@@ -221,6 +221,22 @@ class _ComputeProgressReporter:
             self.done += delta
 
 
+def _site_order(model, site_order):
+    """The GPU site order for whole-system calculations (see pybinding_amd.ordering)."""
+    if site_order is None or isinstance(site_order, str) and site_order == "bfs":
+        return None
+    if not isinstance(site_order, str):
+        return np.asarray(site_order, dtype=np.int32)
+    if site_order != "auto":
+        raise ValueError("site_order must be 'auto', 'bfs' or a permutation")
+    system = getattr(model, "system", None)
+    if system is None:
+        return None
+    from .ordering import locality_order
+    x, y, _ = system.expanded_positions if system.is_multiorbital else system.positions
+    return locality_order(x, y)
+
+
 def kpm(model, energy_range=None, kernel="default", num_threads="auto", silent=False, **kwargs):
     """The Kernel Polynomial Method on the GPU (pybinding/chebyshev.py:339-378)
 
@@ -238,9 +254,12 @@ def kpm(model, energy_range=None, kernel="default", num_threads="auto", silent=F
         Don't show any progress messages.
     **kwargs
         matrix_format ("ELL" or "CSR"), optimal_size, interleaved, lanczos_precision,
-        device (GPU index), kubo_precision ("fp64" or "fp32_split").
+        device (GPU index), kubo_precision ("fp64" or "fp32_split"),
+        site_order ("auto": a Hilbert-curve order of the model's site positions for whole-system
+        calculations, "bfs": the reference's BFS reorder only, or an explicit permutation).
     """
     native = load_module()
+    order = _site_order(model, kwargs.pop("site_order", "auto"))
     h = _as_hamiltonian(model)
     h.sort_indices()
     if kernel == "default":
@@ -251,6 +270,8 @@ def kpm(model, energy_range=None, kernel="default", num_threads="auto", silent=F
         kwargs.pop("progress_callback", None)
     impl = native.kpm(h.shape, h.indptr.astype(np.int32), h.indices.astype(np.int32), h.data,
                       tuple(energy_range) if energy_range is not None else (0.0, 0.0), kernel, **kwargs)
+    if order is not None:
+        impl.set_locality_order(order)
     return KPM(impl, model if not scipy.sparse.issparse(model) else None)
 
 

@@ -223,6 +223,8 @@ public:
     Csr<T> h2;
     SliceMap slice_map;
     std::vector<int32_t> reorder_map;  // original index -> optimized index (empty: identity)
+    std::vector<int32_t> locality_order;  // optional site order for whole-system calculations
+    bool used_locality = false;
     uint64_t version = 0;
     Chrono timer;
 
@@ -231,13 +233,20 @@ public:
 
     idx_t size() const { return original->rows; }
 
-    void optimize_for(Indices const& idx, Scale<> scale) {
-        if (has_idx && original_idx == idx) return;
+    // `whole_system`: the calculation never uses the slice map (DOS, moments, conductivity); the
+    // GPU then prefers `locality_order` (e.g. sites along a space-filling curve) when one is set,
+    // because the moments do not depend on the site order and a compact order keeps the gathered
+    // vector rows of a sweep in L2.  Otherwise the reference's BFS reorder is used.
+    void optimize_for(Indices const& idx, Scale<> scale, bool whole_system = false) {
+        const bool loc = whole_system && is_reordered && !locality_order.empty();
+        if (has_idx && original_idx == idx && loc == used_locality) return;
         timer.tic();
-        if (is_reordered) create_reordered(idx, scale);
+        if (loc) create_ordered(idx, scale, locality_order);
+        else if (is_reordered) create_reordered(idx, scale);
         else create_scaled(idx, scale);
         timer.toc();
         original_idx = idx;
+        used_locality = loc;
         has_idx = true;
         ++version;
     }
@@ -298,6 +307,17 @@ public:
     size_t vector_memory() const { return (size_t)size() * sizeof(T); }
 
 private:
+    // Scale and permute by a given order: new index of original site i is order[i].
+    void create_ordered(Indices const& idx, Scale<> s, std::vector<int32_t> const& order) {
+        create_scaled(idx, s);
+        reorder_map = order;
+        h2 = reorder(h2);
+        optimized_idx = Indices();
+        for (auto v : idx.src) optimized_idx.src.push_back(reorder_map[(size_t)v]);
+        for (auto v : idx.dest) optimized_idx.dest.push_back(reorder_map[(size_t)v]);
+        slice_map = SliceMap(h2.rows);
+    }
+
     void create_scaled(Indices const& idx, Scale<> s) {
         const Scale<R> scale(s);
         auto const& h = *original;

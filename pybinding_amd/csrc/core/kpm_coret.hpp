@@ -57,7 +57,7 @@ public:
                             Csr<cd> const& op) override {
         auto alg = cfg_.algorithm;
         alg.optimal_size = false;
-        oh_.optimize_for({0, 0}, scaling_factors());
+        oh_.optimize_for({0, 0}, scaling_factors(), true);
         stats_.reset(num_moments, oh_, alg);
         auto starter = constant_starter(oh_, alpha);
         const idx_t M = round_num_moments(num_moments);
@@ -99,7 +99,7 @@ public:
         const idx_t M = cfg_.kernel.required_num_moments(broadening / scale.a);
         auto alg = cfg_.algorithm;
         alg.optimal_size = false;
-        oh_.optimize_for({0, 0}, scale);
+        oh_.optimize_for({0, 0}, scale, true);
         stats_.reset(M, oh_, alg, num_random);
         auto starter = random_starter(oh_);
         BatchDiagonalMoments<T> m{M, num_random, true, {}};
@@ -112,7 +112,7 @@ public:
     std::vector<cd> dos_moments(idx_t num_moments, idx_t num_random) override {
         auto alg = cfg_.algorithm;
         alg.optimal_size = false;
-        oh_.optimize_for({0, 0}, scaling_factors());
+        oh_.optimize_for({0, 0}, scaling_factors(), true);
         stats_.reset(num_moments, oh_, alg, num_random);
         auto starter = random_starter(oh_);
         BatchDiagonalMoments<T> m{num_moments, num_random, true, {}};
@@ -152,7 +152,7 @@ public:
         const idx_t M = cfg_.kernel.required_num_moments(broadening / scale.a);
         auto alg = cfg_.algorithm;
         alg.optimal_size = false;
-        oh_.optimize_for({0, 0}, scale);
+        oh_.optimize_for({0, 0}, scale, true);
         stats_.reset(M, oh_, alg, num_random);
         const Csr<T> vl = velocity(h_, left), vr = velocity(h_, right);
         stats_.moments_timer.tic();
@@ -167,6 +167,11 @@ public:
         for (idx_t i = 0; i < num_points; ++i)
             samples[(size_t)i] = (num_points == 1) ? max_ : min_ + (max_ - min_) * (double)i / (double)(num_points - 1);
         return kubo_bastin(mu, M, chemical_potential, samples, temperature, scale);
+    }
+
+    void set_locality_order(std::vector<int32_t> order) override {
+        if (!order.empty() && (idx_t)order.size() != h_.rows) throw std::invalid_argument("site order has the wrong size");
+        oh_.locality_order = std::move(order);
     }
 
     // Raw access for the engine's own tools

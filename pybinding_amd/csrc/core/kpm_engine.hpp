@@ -435,6 +435,7 @@ public:
     virtual double min_energy() = 0;
     virtual double max_energy() = 0;
     virtual int lanczos_loops() const = 0;
+    virtual void set_locality_order(std::vector<int32_t> order) = 0;
 };
 
 }  // namespace kpmx_host

@@ -92,6 +92,7 @@ PYBIND11_MODULE(_kpmx, m) {
         .def_property_readonly("dtype", [](KPMCore& k) { return k.dtype; })
         .def_property_readonly("size", [](KPMCore& k) { return k.core->size(); })
         .def_property_readonly("is_complex", [](KPMCore& k) { return k.core->is_complex(); })
+        .def("set_locality_order", [](KPMCore& k, std::vector<int32_t> order) { k.core->set_locality_order(order); })
         .def("moments",
              [](KPMCore& k, idx_t num_moments, std::vector<cd> alpha, std::vector<cd> beta, py::object op) {
                  return as_array(k.core->moments(num_moments, alpha, beta, csr_cd(op)));

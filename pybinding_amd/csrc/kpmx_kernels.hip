@@ -203,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sell(StepArgs<T> a, const int64_
 // the tile's rows -- y, own x and U*KMAX gathered x rows per lane -- is issued before the first
 // multiply.  y is streamed with non-temporal loads and stores so that it does not evict x from
 // L2.  Columns beyond KMAX (wider slices) fall back to a loop over global memory.
-template <class T, int VW, int MODE, int KMAX, int RW>
+template <class T, int VW, int MODE, int KMAX, int RW, bool OFF32>
 __global__ __launch_bounds__(BLOCK) void k_step_sellb(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
                                                       const int32_t* __restrict__ slice_width) {
     constexpr int RPW = WAVE / VW;                 // rows per row group
@@ -282,7 +282,15 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellb(StepArgs<T> a, const int64
 #pragma unroll
         for (int k = 0; k < KMAX; ++k)
 #pragma unroll
-            for (int u = 0; u < U; ++u) xg[u][k] = x[(int64_t)s_idx[k * TR + rin[u]] * VW + col];
+            for (int u = 0; u < U; ++u) {
+                if constexpr (OFF32) {
+                    // the whole x block is below 4 GiB: a 32-bit byte offset from the uniform base
+                    const uint32_t off = ((uint32_t)s_idx[k * TR + rin[u]] * VW + col) * (uint32_t)sizeof(T);
+                    xg[u][k] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(x) + off);
+                } else {
+                    xg[u][k] = x[(int64_t)s_idx[k * TR + rin[u]] * VW + col];
+                }
+            }
         // the next tile's header and indices go out while this tile's gathers are in flight
         if (tile + tw.stride < tw.end) prefetch(tile + tw.stride, next_base, next_width);
         T acc[U];
@@ -484,15 +492,22 @@ static int resident_grid(K kernel, int num_cu, int64_t num_tiles) {
     return (int)g;
 }
 
-template <class T, int VW, int MODE, int KMAX, int RW>
-static int launch_sellb_rw(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
-    auto kernel = k_step_sellb<T, VW, MODE, KMAX, RW>;
+template <class T, int VW, int MODE, int KMAX, int RW, bool OFF32>
+static int launch_sellb_off(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
+    auto kernel = k_step_sellb<T, VW, MODE, KMAX, RW, OFF32>;
     constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
     a.num_tiles = (L.n + tr - 1) / tr;
     if (L.query_grid) return resident_grid(kernel, L.num_cu, a.num_tiles);
     const int grid = L.num_wg > 0 ? L.num_wg : resident_grid(kernel, L.num_cu, a.num_tiles);
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, L.slice_ptr, L.slice_width);
     return 0;
+}
+
+template <class T, int VW, int MODE, int KMAX, int RW>
+static int launch_sellb_rw(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
+    const bool off32 = (uint64_t)L.n * VW * sizeof(T) < ((uint64_t)1 << 32);
+    return off32 ? launch_sellb_off<T, VW, MODE, KMAX, RW, true>(a, L, stream)
+                 : launch_sellb_off<T, VW, MODE, KMAX, RW, false>(a, L, stream);
 }
 
 template <class T, int VW, int MODE, int KMAX>

@@ -17,34 +17,14 @@ pybinding/repository/graphene/lattice.py:36-40.
 import numpy as np
 import scipy.sparse as sp
 
+from .ordering import hilbert_index
+
 A_CC = 0.142        # nm, carbon-carbon distance
 A_LAT = 0.24595     # nm, graphene lattice constant
 T_GRAPHENE = -2.8   # eV
 
 # graphene.monolayer() hoppings from A(i, j) to B(i + di, j + dj)
 _HONEYCOMB_HOPS = ((0, 0), (1, -1), (0, -1))
-
-
-def hilbert_index(ix, iy, order):
-    """Position along the Hilbert curve of the 2^order x 2^order grid for integer cells (ix, iy)
-    (the classic rotate-and-reflect construction, vectorised)."""
-    ix = np.asarray(ix, dtype=np.int64).copy()
-    iy = np.asarray(iy, dtype=np.int64).copy()
-    d = np.zeros_like(ix)
-    s = 1 << (order - 1)
-    while s > 0:
-        rx = ((ix & s) > 0).astype(np.int64)
-        ry = ((iy & s) > 0).astype(np.int64)
-        d += s * s * ((3 * rx) ^ ry)
-        # rotate the quadrant: reflect when (rx, ry) = (1, 0), transpose when ry = 0 (only the
-        # bits below s matter from here on, so reflecting within s is enough)
-        flip = ry == 0
-        refl = flip & (rx == 1)
-        ix = np.where(refl, s - 1 - ix, ix)
-        iy = np.where(refl, s - 1 - iy, iy)
-        ix, iy = np.where(flip, iy, ix), np.where(flip, ix, iy)
-        s >>= 1
-    return d
 
 
 def honeycomb_cell_order(n1, n2):

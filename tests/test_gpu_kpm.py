@@ -104,3 +104,19 @@ def test_lanczos_bounds_used_when_no_range():
     lo, hi = kpm.impl.energy_range
     assert lo == pytest.approx(ev[0], rel=2e-3)
     assert hi == pytest.approx(ev[-1], rel=2e-3)
+
+
+@pytest.mark.parametrize("order", ["auto", "bfs"])
+def test_site_order_does_not_change_moments(oracle, order):
+    """The Hilbert-curve site order (whole-system calculations) and the reference's BFS order give
+    the same moments as the unpermuted oracle."""
+    h, pos = generators.honeycomb_flake(8, 6, return_positions=True)
+    d = np.random.default_rng(5).uniform(-0.4, 0.4, h.shape[0])
+    h = (h + sp.diags(d)).tocsr()
+    model = pb.Model(h, pb.model.System(pos[:, 0], pos[:, 1]))
+    kpm = pb.kpm(model, energy_range=(-9, 9), silent=True, site_order=order)
+    mu = kpm.dos_moments(130, 4)
+    a, b = oracle.scale_factors(-9, 9)
+    h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
+    ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], 4, np.float64), 130).mean(axis=0)
+    assert np.max(np.abs(mu - ref)) <= 1e-10 * abs(ref[0])
