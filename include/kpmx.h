@@ -56,6 +56,7 @@ extern "C" {
 typedef struct kpmx_ctx kpmx_ctx;
 typedef struct kpmx_mat kpmx_mat;
 typedef struct kpmx_block kpmx_block;
+typedef struct kpmx_batch kpmx_batch;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* kpmx_last_error(void);
@@ -130,6 +131,25 @@ int kpmx_moments_dense(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_mom
  * mode: KPMX_KUBO_FP64 or KPMX_KUBO_FP32_SPLIT.  mu_nm: host, M x M, scalar type of h2. */
 int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr, const kpmx_block* r0,
                       int64_t num_moments, int mode, void* mu_nm);
+
+/* Batched independent systems -- many disorder realizations of uneven size advanced together.
+ * No single counterpart in the reference: it is what a caller of the reference does by running
+ * one `KPM.calc_dos` (cppcore/src/kpm/Core.cpp:74-90) per realization, each with its own bounds
+ * and its own default-seeded random starter; here every realization's diagonal moments come from
+ * one block-diagonal recurrence.
+ *   upload:   the block-diagonal matrix of the already scaled H2 of every system (CSR, global
+ *             indices; system s owns rows [row_offsets[s], row_offsets[s+1]) and may only couple
+ *             to them).
+ *   starters: num_vectors (1..64) starters per system, host, vector-major num_vectors x N.
+ *   run:      all num_moments / 2 steps; use_graph != 0 captures the whole step sequence in one
+ *             HIP graph on the first run for this num_moments and replays it afterwards.
+ *   fetch:    mu: host, num_systems x num_vectors x num_moments (mu_0 halved). */
+int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int64_t* row_offsets, int64_t nnz,
+                      const int32_t* indptr, const int32_t* indices, const void* values, kpmx_batch** out);
+int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0);
+int kpmx_batch_run(kpmx_batch* b, int64_t num_moments, int use_graph);
+int kpmx_batch_fetch(kpmx_batch* b, int64_t num_moments, void* mu);
+int kpmx_batch_destroy(kpmx_batch* b);
 
 /* Extreme eigenvalues of the (unscaled) Hamiltonian `h` by the Lanczos procedure, converged to
  * `precision_percent` (cppcore/include/compute/lanczos.hpp:93-154, used by Bounds::compute_bounds,

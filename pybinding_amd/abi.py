@@ -176,3 +176,45 @@ def lanczos_bounds(mat, precision_percent=0.002):
     _check(lib().kpmx_lanczos_bounds(mat.h, ctypes.c_double(precision_percent), ctypes.byref(lo), ctypes.byref(hi),
                                      ctypes.byref(loops)))
     return lo.value, hi.value, loops.value
+
+
+class Batch:
+    """Batched independent systems (kpmx_batch_*): `h2s` are the already scaled H2 of each
+    system; they are laid out block-diagonally."""
+
+    def __init__(self, ctx, h2s):
+        import scipy.sparse as sp
+        h2s = [sp.csr_matrix(h) for h in h2s]
+        self.dtype = np.dtype(h2s[0].dtype)
+        self.sizes = [h.shape[0] for h in h2s]
+        self.row_offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        big = sp.block_diag(h2s, format="csr", dtype=self.dtype)
+        big.sort_indices()
+        self.indptr = np.ascontiguousarray(big.indptr, dtype=np.int32)
+        self.indices = np.ascontiguousarray(big.indices, dtype=np.int32)
+        self.data = np.ascontiguousarray(big.data)
+        self.n = big.shape[0]
+        self.nvec = 0
+        self.h = ctypes.c_void_p()
+        _check(lib().kpmx_batch_upload(ctx.h, ctypes.c_int(SCALAR[self.dtype]), ctypes.c_int64(len(h2s)),
+                                       _p(self.row_offsets), ctypes.c_int64(len(self.data)), _p(self.indptr),
+                                       _p(self.indices), _p(self.data), ctypes.byref(self.h)))
+
+    def starters(self, r0):
+        """r0: (num_vectors, N) over the concatenated systems."""
+        r0 = np.ascontiguousarray(np.atleast_2d(r0), dtype=self.dtype)
+        self.nvec = r0.shape[0]
+        _check(lib().kpmx_batch_starters(self.h, ctypes.c_int64(self.nvec), _p(r0)))
+
+    def run(self, num_moments, use_graph=True):
+        _check(lib().kpmx_batch_run(self.h, ctypes.c_int64(num_moments), ctypes.c_int(1 if use_graph else 0)))
+
+    def fetch(self, num_moments):
+        out = np.empty((len(self.sizes), self.nvec, num_moments), dtype=self.dtype)
+        _check(lib().kpmx_batch_fetch(self.h, ctypes.c_int64(num_moments), _p(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().kpmx_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()

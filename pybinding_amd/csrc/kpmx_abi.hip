@@ -148,6 +148,52 @@ struct kpmx_block {
     }
 };
 
+// Batched independent systems (block-diagonal H2, one block per system, each starting on a
+// 64-row slice boundary) with their starter vectors and device workspaces.
+struct kpmx_batch {
+    kpmx_ctx* ctx = nullptr;
+    int scalar = 1;
+    int64_t num_sys = 0;
+    std::vector<int64_t> row_off, pad_off;  // unpadded / padded first row of every system (S + 1)
+    int64_t n_pad = 0, nslices = 0;
+    int max_width = 0;
+    int64_t* slice_ptr = nullptr;
+    int32_t* slice_width = nullptr;
+    int32_t* slice_sys = nullptr;
+    int32_t* sidx = nullptr;
+    void* svals = nullptr;
+    int num_wg = 0;
+    int64_t num_slots = 0;
+    int64_t* tile_begin = nullptr;
+    int64_t* slot_begin = nullptr;
+    int64_t* slot_first = nullptr;
+    // starters and workspaces
+    int64_t nvec = 0;
+    int vw = 4;
+    void* r0 = nullptr;
+    void* w0 = nullptr;
+    void* w1 = nullptr;
+    double* partials = nullptr;
+    double* sums = nullptr;
+    int64_t sums_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t graph_moments = 0;
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_moments = 0;
+    }
+    ~kpmx_batch() {
+        drop_graph();
+        for (void* p : {(void*)slice_ptr, (void*)slice_width, (void*)slice_sys, (void*)sidx, svals, (void*)tile_begin,
+                        (void*)slot_begin, (void*)slot_first, r0, w0, w1, (void*)partials, (void*)sums})
+            dev_free(p);
+    }
+};
+
 namespace {
 
 int pick_vw(int64_t remaining) {
@@ -1043,6 +1089,269 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
         for (const Tile& tt : r0->tiles)
             if (tt.vw == t.vw) ctx->tune[h2->scalar][log2i(t.vw)] = best;
         if (variant) *variant = best;
+    });
+}
+
+int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int64_t* row_offsets, int64_t nnz,
+                      const int32_t* indptr, const int32_t* indices, const void* values, kpmx_batch** out) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx && out && row_offsets && indptr, "null argument");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        KPMX_REQUIRE(num_systems > 0, "no systems");
+        KPMX_REQUIRE(row_offsets[0] == 0, "row offsets must start at 0");
+        const int64_t n = row_offsets[num_systems];
+        KPMX_REQUIRE(n > 0 && n < ((int64_t)1 << 31) && indptr[n] == nnz && (nnz == 0 || (indices && values)),
+                     "inconsistent CSR arrays");
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const size_t sb = scalar_bytes(scalar);
+        auto b = std::make_unique<kpmx_batch>();
+        b->ctx = ctx;
+        b->scalar = scalar;
+        b->num_sys = num_systems;
+        b->row_off.assign(row_offsets, row_offsets + num_systems + 1);
+        b->pad_off.assign(num_systems + 1, 0);
+        for (int64_t sy = 0; sy < num_systems; ++sy) {
+            const int64_t ns = row_offsets[sy + 1] - row_offsets[sy];
+            KPMX_REQUIRE(ns > 0, "empty system");
+            b->pad_off[sy + 1] = b->pad_off[sy] + (ns + 63) / 64 * 64;
+        }
+        b->n_pad = b->pad_off[num_systems];
+        const int64_t ns_tot = b->n_pad / 64;
+        b->nslices = ns_tot;
+        // SELL-64 over padded rows; column indices remapped into the padded numbering
+        std::vector<int64_t> sptr(ns_tot + 1, 0);
+        std::vector<int32_t> swidth(ns_tot, 0), ssys(ns_tot, 0);
+        std::vector<int64_t> slice_first_row(ns_tot);  // unpadded row of lane 0 (or -1)
+        for (int64_t sy = 0; sy < num_systems; ++sy) {
+            for (int64_t pr = b->pad_off[sy]; pr < b->pad_off[sy + 1]; pr += 64) {
+                const int64_t sl = pr / 64;
+                ssys[sl] = (int32_t)sy;
+                int32_t w = 0;
+                for (int lane = 0; lane < 64; ++lane) {
+                    const int64_t r = row_offsets[sy] + (pr - b->pad_off[sy]) + lane;
+                    if (r < row_offsets[sy + 1]) {
+                        for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k)
+                            KPMX_REQUIRE(indices[k] >= row_offsets[sy] && indices[k] < row_offsets[sy + 1],
+                                         "the matrix is not block diagonal over the given systems");
+                        w = std::max<int32_t>(w, indptr[r + 1] - indptr[r]);
+                    }
+                }
+                swidth[sl] = w;
+            }
+        }
+        for (int64_t sl = 0; sl < ns_tot; ++sl) sptr[sl + 1] = sptr[sl] + (int64_t)swidth[sl] * 64;
+        const int64_t total = sptr[ns_tot];
+        std::vector<int32_t> sidx((size_t)std::max<int64_t>(total, 1));
+        std::vector<char> svals((size_t)std::max<int64_t>(total, 1) * sb, 0);
+        const char* v = static_cast<const char*>(values);
+        for (int64_t sy = 0; sy < num_systems; ++sy) {
+            const int64_t shift = b->pad_off[sy] - row_offsets[sy];
+            for (int64_t pr = b->pad_off[sy]; pr < b->pad_off[sy + 1]; ++pr) {
+                const int64_t sl = pr / 64, lane = pr % 64;
+                const int64_t r = pr - shift;
+                const bool real_row = r < row_offsets[sy + 1];
+                for (int32_t k = 0; k < swidth[sl]; ++k) {
+                    const int64_t dst = sptr[sl] + (int64_t)k * 64 + lane;
+                    if (real_row && indptr[r] + k < indptr[r + 1]) {
+                        sidx[dst] = (int32_t)(indices[indptr[r] + k] + shift);
+                        std::memcpy(&svals[dst * sb], v + (size_t)(indptr[r] + k) * sb, sb);
+                    } else {
+                        sidx[dst] = (int32_t)pr;
+                    }
+                }
+            }
+        }
+        b->max_width = ns_tot ? *std::max_element(swidth.begin(), swidth.end()) : 0;
+        // contiguous slice ranges per workgroup and the partial slot numbering
+        b->num_wg = (int)std::max<int64_t>(1, std::min<int64_t>(ns_tot, (int64_t)ctx->num_cu * 4));
+        std::vector<int64_t> tb(b->num_wg + 1), sbeg(b->num_wg);
+        for (int w = 0; w <= b->num_wg; ++w) tb[w] = ns_tot * w / b->num_wg;
+        std::vector<int64_t> slot_sys;
+        for (int w = 0; w < b->num_wg; ++w) {
+            sbeg[w] = (int64_t)slot_sys.size();
+            if (tb[w] == tb[w + 1]) continue;
+            for (int sy = ssys[tb[w]]; sy <= ssys[tb[w + 1] - 1]; ++sy) slot_sys.push_back(sy);
+        }
+        b->num_slots = (int64_t)slot_sys.size();
+        std::vector<int64_t> sfirst(num_systems + 1, b->num_slots);
+        for (int64_t p = b->num_slots - 1; p >= 0; --p) sfirst[slot_sys[p]] = p;
+        for (int64_t sy = num_systems - 1; sy >= 0; --sy) sfirst[sy] = std::min(sfirst[sy], sfirst[sy + 1]);
+        auto up = [&](void* dst, const void* src, size_t bytes) {
+            if (bytes) KPMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        };
+        b->slice_ptr = static_cast<int64_t*>(dev_alloc((ns_tot + 1) * sizeof(int64_t)));
+        b->slice_width = static_cast<int32_t*>(dev_alloc(ns_tot * sizeof(int32_t)));
+        b->slice_sys = static_cast<int32_t*>(dev_alloc(ns_tot * sizeof(int32_t)));
+        b->sidx = static_cast<int32_t*>(dev_alloc(sidx.size() * sizeof(int32_t)));
+        b->svals = dev_alloc(svals.size());
+        b->tile_begin = static_cast<int64_t*>(dev_alloc(tb.size() * sizeof(int64_t)));
+        b->slot_begin = static_cast<int64_t*>(dev_alloc(sbeg.size() * sizeof(int64_t)));
+        b->slot_first = static_cast<int64_t*>(dev_alloc(sfirst.size() * sizeof(int64_t)));
+        up(b->slice_ptr, sptr.data(), sptr.size() * sizeof(int64_t));
+        up(b->slice_width, swidth.data(), swidth.size() * sizeof(int32_t));
+        up(b->slice_sys, ssys.data(), ssys.size() * sizeof(int32_t));
+        up(b->sidx, sidx.data(), sidx.size() * sizeof(int32_t));
+        up(b->svals, svals.data(), svals.size());
+        up(b->tile_begin, tb.data(), tb.size() * sizeof(int64_t));
+        up(b->slot_begin, sbeg.data(), sbeg.size() * sizeof(int64_t));
+        up(b->slot_first, sfirst.data(), sfirst.size() * sizeof(int64_t));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        *out = b.release();
+    });
+}
+
+int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0) {
+    return guarded([&] {
+        KPMX_REQUIRE(b && r0, "null argument");
+        KPMX_REQUIRE(num_vectors >= 1 && num_vectors <= 64, "1 to 64 vectors per system");
+        kpmx_ctx* ctx = b->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const size_t sb = scalar_bytes(b->scalar);
+        const int vw = std::max(4, pick_vw(num_vectors));
+        if (b->vw != vw || !b->r0) {
+            KPMX_HIP(hipStreamSynchronize(ctx->stream));
+            dev_free(b->r0);
+            dev_free(b->w0);
+            dev_free(b->w1);
+            b->r0 = b->w0 = b->w1 = nullptr;
+            b->drop_graph();
+            b->r0 = dev_alloc((size_t)b->n_pad * vw * sb);
+            b->w0 = dev_alloc((size_t)b->n_pad * vw * sb);
+            b->w1 = dev_alloc((size_t)b->n_pad * vw * sb);
+        }
+        b->vw = vw;
+        b->nvec = num_vectors;
+        // padded vector-major copy on the host, then one gather into the row-major block
+        const int64_t n = b->row_off.back();
+        std::vector<char> padded((size_t)num_vectors * b->n_pad * sb, 0);
+        const char* src = static_cast<const char*>(r0);
+        for (int64_t c = 0; c < num_vectors; ++c)
+            for (int64_t sy = 0; sy < b->num_sys; ++sy)
+                std::memcpy(&padded[((size_t)c * b->n_pad + b->pad_off[sy]) * sb], src + ((size_t)c * n + b->row_off[sy]) * sb,
+                            (size_t)(b->row_off[sy + 1] - b->row_off[sy]) * sb);
+        void* staging = ctx->aux0.get(padded.size());
+        KPMX_HIP(hipMemcpyAsync(staging, padded.data(), padded.size(), hipMemcpyHostToDevice, ctx->stream));
+        launch_gather_block(b->scalar, staging, b->n_pad, 0, num_vectors, vw, b->r0, ctx->stream);
+        KPMX_HIP(hipGetLastError());
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+namespace {
+void batch_enqueue(kpmx_batch* b, int64_t steps) {
+    kpmx_ctx* ctx = b->ctx;
+    const int width = 4 * b->vw;
+    BatchedLaunch B;
+    B.scalar = b->scalar;
+    B.vw = b->vw;
+    B.num_wg = b->num_wg;
+    B.max_width = b->max_width;
+    B.indices = b->sidx;
+    B.vals = b->svals;
+    B.slice_ptr = b->slice_ptr;
+    B.slice_width = b->slice_width;
+    B.slice_sys = b->slice_sys;
+    B.tile_begin = b->tile_begin;
+    B.slot_begin = b->slot_begin;
+    B.partials = b->partials;
+    B.n = b->n_pad;
+    for (int64_t k = 0; k < steps; ++k) {
+        if (k == 0) {
+            B.x = b->r0;
+            B.yin = nullptr;
+            B.yout = b->w1;
+            B.hscale = 0.5;
+            B.sub_y = 0;
+        } else {
+            const bool odd = (k % 2) == 1;
+            B.x = odd ? b->w1 : b->w0;
+            B.yout = odd ? b->w0 : b->w1;
+            B.yin = (k == 1) ? b->r0 : B.yout;
+            B.hscale = 1.0;
+            B.sub_y = 1;
+        }
+        if (launch_batched_step(B, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported batched configuration"};
+        launch_reduce_segments(b->partials, b->slot_first, b->num_sys, width, b->sums + (size_t)k * b->num_sys * width,
+                               ctx->stream);
+    }
+}
+}  // namespace
+
+int kpmx_batch_run(kpmx_batch* b, int64_t M, int use_graph) {
+    return guarded([&] {
+        KPMX_REQUIRE(b, "null batch");
+        KPMX_REQUIRE(b->r0, "no starters uploaded");
+        KPMX_REQUIRE(M >= 2 && M % 2 == 0, "num_moments must be even and >= 2");
+        kpmx_ctx* ctx = b->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int64_t steps = M / 2;
+        const int width = 4 * b->vw;
+        if (!b->partials) b->partials = static_cast<double*>(dev_alloc((size_t)b->num_slots * 64 * 4 * sizeof(double)));
+        if (b->sums_steps != steps || !b->sums) {
+            KPMX_HIP(hipStreamSynchronize(ctx->stream));
+            dev_free(b->sums);
+            b->sums = static_cast<double*>(dev_alloc((size_t)steps * b->num_sys * 64 * 4 * sizeof(double)));
+            b->sums_steps = steps;
+            b->drop_graph();
+        }
+        (void)width;
+        if (!use_graph) {
+            batch_enqueue(b, steps);
+            KPMX_HIP(hipGetLastError());
+            return;
+        }
+        if (b->graph_moments != M || !b->graph_exec) {
+            b->drop_graph();
+            KPMX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            try {
+                batch_enqueue(b, steps);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(ctx->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            KPMX_HIP(hipStreamEndCapture(ctx->stream, &b->graph));
+            KPMX_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+            b->graph_moments = M;
+        }
+        KPMX_HIP(hipGraphLaunch(b->graph_exec, ctx->stream));
+    });
+}
+
+int kpmx_batch_fetch(kpmx_batch* b, int64_t M, void* mu) {
+    return guarded([&] {
+        KPMX_REQUIRE(b && mu, "null argument");
+        KPMX_REQUIRE(b->sums && b->sums_steps == M / 2, "no finished run for this number of moments");
+        kpmx_ctx* ctx = b->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int width = 4 * b->vw;
+        const int64_t steps = M / 2;
+        std::vector<double> s((size_t)steps * b->num_sys * width);
+        KPMX_HIP(hipMemcpyAsync(s.data(), b->sums, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        const bool cplx = is_complex(b->scalar);
+        // per system: the diagonal collector over that system's sums; mu is S x V x M
+        std::vector<double> per((size_t)steps * width);
+        for (int64_t sy = 0; sy < b->num_sys; ++sy) {
+            for (int64_t k = 0; k < steps; ++k)
+                std::copy(&s[((size_t)k * b->num_sys + sy) * width], &s[((size_t)k * b->num_sys + sy) * width] + width,
+                          &per[(size_t)k * width]);
+            const size_t off = (size_t)sy * b->nvec * M * (cplx ? 2 : 1);
+            if (b->scalar == 0 || b->scalar == 2)
+                diag_collect<float>(per, b->vw, b->nvec, M, cplx, static_cast<float*>(mu) + off, 0);
+            else
+                diag_collect<double>(per, b->vw, b->nvec, M, cplx, static_cast<double*>(mu) + off, 0);
+        }
+    });
+}
+
+int kpmx_batch_destroy(kpmx_batch* b) {
+    return guarded([&] {
+        if (!b) return;
+        (void)hipSetDevice(b->ctx->device);
+        (void)hipStreamSynchronize(b->ctx->stream);
+        delete b;
     });
 }
 }  // extern "C"

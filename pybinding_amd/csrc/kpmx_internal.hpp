@@ -38,6 +38,27 @@ struct StepLaunch {
     int tune = 0;             // tile-shape variant of the SELL block kernel (0 default, 1 smaller, 2 larger)
 };
 
+struct BatchedLaunch {
+    int scalar = 1, vw = 4, num_wg = 0, max_width = 0;
+    const int32_t* indices = nullptr;
+    const void* vals = nullptr;
+    const int64_t* slice_ptr = nullptr;
+    const int32_t* slice_width = nullptr;
+    const int32_t* slice_sys = nullptr;
+    const int64_t* tile_begin = nullptr;  // num_wg + 1
+    const int64_t* slot_begin = nullptr;  // num_wg
+    const void* x = nullptr;
+    const void* yin = nullptr;
+    void* yout = nullptr;
+    double* partials = nullptr;
+    int64_t n = 0;
+    double hscale = 1.0;
+    int sub_y = 1;
+};
+int launch_batched_step(const BatchedLaunch& B, hipStream_t stream);
+void launch_reduce_segments(const double* partials, const int64_t* slot_first, int64_t num_sys, int width, double* sums,
+                            hipStream_t stream);
+
 int launch_step(const StepLaunch& L, hipStream_t stream);
 int step_grid(const StepLaunch& L);
 constexpr int NUM_TUNE_VARIANTS = 3;

@@ -53,10 +53,11 @@ struct StepArgs {
     int sub_y;
 };
 
-// Write the block's 4*VW column sums into partials[blockIdx.x].
+// Write the block's 4*VW column sums into partials[slot] (slot defaults to blockIdx.x).
 template <int VW>
 __device__ __forceinline__ void block_reduce_store(double a_re, double a_im, double b_re, double b_im,
-                                                   double* partials) {
+                                                   double* partials, int64_t slot = -1) {
+    if (slot < 0) slot = blockIdx.x;
     __shared__ double red[WAVES_PER_BLOCK][4][VW];
     const int lane = threadIdx.x % WAVE;
     const int wave = threadIdx.x / WAVE;
@@ -77,8 +78,9 @@ __device__ __forceinline__ void block_reduce_store(double a_re, double a_im, dou
         const int q = t / VW, c = t % VW;
         double s = red[0][q][c];
         for (int wv = 1; wv < WAVES_PER_BLOCK; ++wv) s += red[wv][q][c];
-        partials[(int64_t)blockIdx.x * 4 * VW + t] = s;
+        partials[slot * 4 * VW + t] = s;
     }
+    __syncthreads();  // `red` may be reused by a following call
 }
 
 // Tile index sequence of this workgroup (XCD-aware, see the file header).
@@ -345,6 +347,110 @@ constexpr int sellb_rw(int tune) {
     constexpr int RPW = WAVE / VW;
     constexpr int base = RPW > 4 ? RPW : 4;
     return tune == 1 ? (base > RPW ? base / 2 : base) : tune == 2 ? (base * 2 <= 16 ? base * 2 : base) : base;
+}
+
+// Batched independent systems: a block-diagonal SELL-64 matrix whose diagonal blocks (one per
+// system) start on slice boundaries, with VW >= 4 vectors per system.  Workgroup b owns the
+// contiguous tiles [tile_begin[b], tile_begin[b+1]) (one 64-row slice per tile), so it meets each
+// system in one stretch; the diagonal-moment sums are flushed to a new partial slot whenever the
+// system changes (slot numbering precomputed on the host: slot_begin[b] is the first slot of
+// workgroup b), and k_reduce_segments sums the slots of every system in order.
+template <class T, int VW, int KMAX>
+__global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ slice_width,
+                                                        const int32_t* __restrict__ slice_sys,
+                                                        const int64_t* __restrict__ tile_begin,
+                                                        const int64_t* __restrict__ slot_begin) {
+    constexpr int RPW = WAVE / VW;       // rows per row group
+    constexpr int RW = 16;               // rows per wave (the tile is one 64-row slice)
+    constexpr int U = RW / RPW;
+    constexpr int TR = 64;
+    static_assert(VW >= 4 && RW % RPW == 0, "bad block configuration");
+    __shared__ int32_t s_idx[KMAX * TR];
+    __shared__ T s_val[KMAX * TR];
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    const int rsub = lane / VW;
+    const int col = lane % VW;
+    const int32_t* __restrict__ sidx = a.indices;
+    const T* __restrict__ sval = a.vals;
+    const T* __restrict__ x = a.x;
+    const int64_t t0 = tile_begin[blockIdx.x], t1 = tile_begin[blockIdx.x + 1];
+    if (t0 >= t1) return;  // uniform: the whole workgroup leaves
+    int64_t slot = slot_begin[blockIdx.x];
+    int cur = slice_sys[t0];
+    double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
+    for (int64_t slice = t0; slice < t1; ++slice) {
+        const int sys = slice_sys[slice];
+        if (sys != cur) {
+            block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials, slot);
+            ++slot;
+            cur = sys;
+            a_re = a_im = b_re = b_im = 0;
+        }
+        const int64_t base = slice_ptr[slice];
+        const int width = slice_width[slice];
+        const int kst = width < KMAX ? width : KMAX;
+        __syncthreads();
+        for (int e = threadIdx.x; e < KMAX * TR; e += BLOCK) {
+            const int k = e / TR, r = e % TR;
+            if (k < kst) {
+                s_idx[e] = sidx[base + (int64_t)k * WAVE + r];
+                s_val[e] = sval[base + (int64_t)k * WAVE + r];
+            } else {
+                s_idx[e] = (int32_t)(slice * WAVE + r);
+                s_val[e] = zero(T{});
+            }
+        }
+        __syncthreads();
+        int rin[U];
+        int64_t row[U];
+        T yv[U], xo[U], xg[U][KMAX];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rin[u] = wave * RW + u * RPW + rsub;
+            row[u] = slice * WAVE + rin[u];
+            const int64_t e = row[u] * VW + col;
+            yv[u] = a.sub_y ? nt_load(&a.yin[e]) : zero(T{});
+            xo[u] = x[e];
+        }
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u) xg[u][k] = x[(int64_t)s_idx[k * TR + rin[u]] * VW + col];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            T acc = zero(T{});
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) acc = fmac(s_val[k * TR + rin[u]], xg[u][k], acc);
+            for (int k = KMAX; k < width; ++k) {
+                const int64_t off = base + (int64_t)k * WAVE + rin[u];
+                acc = fmac(sval[off], x[(int64_t)sidx[off] * VW + col], acc);
+            }
+            const int64_t e = row[u] * VW + col;
+            T ynew = scal(acc, a.hscale);
+            if (a.sub_y) ynew = sub(ynew, yv[u]);
+            nt_store(ynew, &a.yout[e]);
+            a_re += sqabs(xo[u]);
+            const d2 p = cdot(ynew, xo[u]);
+            b_re += p.re;
+            b_im += p.im;
+        }
+    }
+    block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials, slot);
+}
+
+// sums[s][j] = sum over the slots [slot_first[s], slot_first[s+1]) of partials[slot][j], in order.
+__global__ void k_reduce_segments(const double* partials, const int64_t* slot_first, int64_t num_sys, int width,
+                                  double* sums) {
+    const int64_t total = num_sys * width;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t sy = e / width;
+        const int j = (int)(e % width);
+        double acc = 0;
+        for (int64_t p = slot_first[sy]; p < slot_first[sy + 1]; ++p) acc += partials[p * width + j];
+        sums[e] = acc;
+    }
 }
 
 // sums[s][j] = sum over workgroups g (in order) of partials[s][g][j], for s in [0, nsteps).
@@ -716,5 +822,55 @@ void launch_lanczos_axpy(int scalar, const void* v1, void* v0, int64_t n, double
     KPMX_DISPATCH(scalar, lanczos_axpy_t, v1, v0, n, a, partials, nblocks, s)
 }
 void launch_scale(int scalar, void* v, int64_t n, double f, hipStream_t s) { KPMX_DISPATCH(scalar, scale_t, v, n, f, s) }
+
+template <class T, int VW>
+static void batched_t(const BatchedLaunch& B, hipStream_t stream) {
+    StepArgs<T> a{};
+    a.indices = B.indices;
+    a.vals = static_cast<const T*>(B.vals);
+    a.x = static_cast<const T*>(B.x);
+    a.yin = static_cast<const T*>(B.yin);
+    a.yout = static_cast<T*>(B.yout);
+    a.partials = B.partials;
+    a.n = B.n;
+    a.hscale = static_cast<typename traits<T>::real>(B.hscale);
+    a.sub_y = B.sub_y;
+    if (B.max_width <= 4)
+        hipLaunchKernelGGL((k_step_batched<T, VW, 4>), dim3(B.num_wg), dim3(BLOCK), 0, stream, a, B.slice_ptr,
+                           B.slice_width, B.slice_sys, B.tile_begin, B.slot_begin);
+    else
+        hipLaunchKernelGGL((k_step_batched<T, VW, 8>), dim3(B.num_wg), dim3(BLOCK), 0, stream, a, B.slice_ptr,
+                           B.slice_width, B.slice_sys, B.tile_begin, B.slot_begin);
+}
+
+template <class T>
+static int batched_vw(const BatchedLaunch& B, hipStream_t stream) {
+    switch (B.vw) {
+        case 4: batched_t<T, 4>(B, stream); return 0;
+        case 8: batched_t<T, 8>(B, stream); return 0;
+        case 16: batched_t<T, 16>(B, stream); return 0;
+        case 32: batched_t<T, 32>(B, stream); return 0;
+        case 64: batched_t<T, 64>(B, stream); return 0;
+        default: return -1;
+    }
+}
+
+int launch_batched_step(const BatchedLaunch& B, hipStream_t stream) {
+    switch (B.scalar) {
+        case 0: return batched_vw<float>(B, stream);
+        case 1: return batched_vw<double>(B, stream);
+        case 2: return batched_vw<cf32>(B, stream);
+        case 3: return batched_vw<cf64>(B, stream);
+        default: return -1;
+    }
+}
+
+void launch_reduce_segments(const double* partials, const int64_t* slot_first, int64_t num_sys, int width, double* sums,
+                            hipStream_t stream) {
+    int64_t g = (num_sys * width + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_reduce_segments, dim3((unsigned)g), dim3(256), 0, stream, partials, slot_first, num_sys, width, sums);
+}
 
 }  // namespace kpmx
