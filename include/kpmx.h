@@ -57,6 +57,7 @@ typedef struct kpmx_ctx kpmx_ctx;
 typedef struct kpmx_mat kpmx_mat;
 typedef struct kpmx_block kpmx_block;
 typedef struct kpmx_batch kpmx_batch;
+typedef struct kpmx_tape kpmx_tape;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* kpmx_last_error(void);
@@ -131,6 +132,25 @@ int kpmx_moments_dense(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_mom
  * mode: KPMX_KUBO_FP64 or KPMX_KUBO_FP32_SPLIT.  mu_nm: host, M x M, scalar type of h2. */
 int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr, const kpmx_block* r0,
                       int64_t num_moments, int mode, void* mu_nm);
+
+/* Adjoint gradient of the diagonal moments (no counterpart in the reference).
+ * For L = sum_n w[n] mu_n with mu_n the moments of kpmx_moments_diagonal averaged over the vectors
+ * of r0 (mu_0 halved), returns dL/dH2 for every stored entry of h2 (grad: host, nnz doubles, the
+ * CSR order of the upload), by the reverse (adjoint) Chebyshev recurrence
+ *   lambda_j = g_j + H2 lambda_{j+1} - lambda_{j+2},   dL/dH2 += lambda_{j+1} v_j^T,
+ * fused with the gradient accumulation over the sparsity pattern.  checkpoint: steps between stored
+ * vector pairs, the vectors in between recomputed; <= 0 keeps every vector (no recomputation).
+ * Real H2 (KPMX_F32 / KPMX_F64) uploaded with KPMX_FMT_SELL; at most 64 vectors.
+ * mu (optional): host, num_vectors x num_moments, the forward moments. */
+int kpmx_adjoint_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, const double* w,
+                          int64_t checkpoint, double* grad, void* mu);
+/* The same in two halves, for callers (autograd) that learn w after the forward pass: the forward
+ * pass returns the moments and a tape holding the stored vectors on the device; the backward pass
+ * takes w and returns the gradient; the tape can be run backward more than once. */
+int kpmx_adjoint_forward(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, int64_t checkpoint, void* mu,
+                         kpmx_tape** out);
+int kpmx_adjoint_backward(kpmx_tape* tape, const double* w, double* grad);
+int kpmx_tape_destroy(kpmx_tape* tape);
 
 /* Batched independent systems -- many disorder realizations of uneven size advanced together.
  * No single counterpart in the reference: it is what a caller of the reference does by running

@@ -218,3 +218,37 @@ class Batch:
         if self.h:
             lib().kpmx_batch_destroy(self.h)
             self.h = ctypes.c_void_p()
+
+
+class Tape:
+    """Forward record of an adjoint calculation (kpmx_adjoint_forward)."""
+
+    def __init__(self, mat, block, num_moments, checkpoint=64):
+        self.mat = mat
+        self.num_moments = num_moments
+        self.mu = np.empty((block.nvec, num_moments), dtype=mat.dtype)
+        self.h = ctypes.c_void_p()
+        _check(lib().kpmx_adjoint_forward(mat.h, block.h, ctypes.c_int64(num_moments), ctypes.c_int64(checkpoint),
+                                          _p(self.mu), ctypes.byref(self.h)))
+
+    def backward(self, w):
+        """dL/dH2 (CSR order of the uploaded matrix) for L = sum_n w[n] * mean over vectors of mu_n."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        grad = np.empty(len(self.mat.data), dtype=np.float64)
+        _check(lib().kpmx_adjoint_backward(self.h, _p(w), _p(grad)))
+        return grad
+
+    def close(self):
+        if self.h:
+            lib().kpmx_tape_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+
+def adjoint_diagonal(mat, block, num_moments, w, checkpoint=64):
+    """(dL/dH2, moments) in one call (kpmx_adjoint_diagonal)."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    grad = np.empty(len(mat.data), dtype=np.float64)
+    mu = np.empty((block.nvec, num_moments), dtype=mat.dtype)
+    _check(lib().kpmx_adjoint_diagonal(mat.h, block.h, ctypes.c_int64(num_moments), _p(w), ctypes.c_int64(checkpoint),
+                                       _p(grad), _p(mu)))
+    return grad, mu

@@ -117,6 +117,8 @@ struct kpmx_mat {
     int32_t* slice_width = nullptr;
     int32_t* sell_indices = nullptr;
     void* sell_vals = nullptr;
+    int64_t sell_total = 0;
+    std::vector<int64_t> sell_pos;  // SELL storage position of every CSR entry
     ~kpmx_mat() {
         dev_free(indptr);
         dev_free(indices);
@@ -191,6 +193,25 @@ struct kpmx_batch {
         for (void* p : {(void*)slice_ptr, (void*)slice_width, (void*)slice_sys, (void*)sidx, svals, (void*)tile_begin,
                         (void*)slot_begin, (void*)slot_first, r0, w0, w1, (void*)partials, (void*)sums})
             dev_free(p);
+    }
+};
+
+// Forward record of an adjoint calculation: every vector v_0..v_S, or checkpoint pairs
+// (v_cC, v_cC+1) plus a segment buffer for the recomputed vectors.
+struct kpmx_tape {
+    const kpmx_mat* h2 = nullptr;
+    int64_t M = 0, S = 0, C = 0, nvec = 0, nslots = 0, nck = 0;
+    int vw = 4;
+    size_t blk = 0;
+    bool store_all = true;
+    char* slots = nullptr;
+    char* cks = nullptr;
+    void* padded = nullptr;
+    char* slot(int64_t j) const { return slots + (size_t)j * blk; }
+    ~kpmx_tape() {
+        dev_free(slots);
+        dev_free(cks);
+        dev_free(padded);
     }
 };
 
@@ -566,6 +587,11 @@ int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t
                     }
                 }
             }
+            m->sell_pos.assign((size_t)nnz, 0);
+            for (int64_t r = 0; r < rows; ++r)
+                for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k)
+                    m->sell_pos[(size_t)k] = sptr[r / 64] + (int64_t)(k - indptr[r]) * 64 + (r % 64);
+            m->sell_total = total;
             m->nslices = ns;
             m->max_width = ns ? *std::max_element(swidth.begin(), swidth.end()) : 0;
             m->slice_ptr = static_cast<int64_t*>(dev_alloc((ns + 1) * sizeof(int64_t)));
@@ -1354,4 +1380,208 @@ int kpmx_batch_destroy(kpmx_batch* b) {
         delete b;
     });
 }
+
+int kpmx_adjoint_forward(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, int64_t checkpoint, void* mu,
+                         kpmx_tape** out) {
+    return guarded([&] {
+        check_pair(h2, r0);
+        KPMX_REQUIRE(out, "null output");
+        KPMX_REQUIRE(M >= 4 && M % 2 == 0, "num_moments must be even and >= 4");
+        KPMX_REQUIRE(h2->scalar == KPMX_F32 || h2->scalar == KPMX_F64, "the adjoint supports real Hamiltonians only");
+        KPMX_REQUIRE(h2->format == KPMX_FMT_SELL, "the adjoint needs the matrix uploaded in SELL format");
+        KPMX_REQUIRE(r0->tiles.size() == 1, "the adjoint takes at most 64 vectors per block");
+        kpmx_ctx* ctx = h2->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int64_t n = h2->rows;
+        const size_t sb = scalar_bytes(h2->scalar);
+        auto tp = std::make_unique<kpmx_tape>();
+        tp->h2 = h2;
+        tp->M = M;
+        tp->nvec = r0->nvec;
+        const Tile& t0 = r0->tiles.front();
+        Tile t = t0;
+        if (t0.vw < 4) {  // the adjoint kernel works on blocks of >= 4 columns: pad with zero columns
+            t.vw = 4;
+            tp->padded = dev_alloc((size_t)n * 4 * sb);
+            void* vm = ctx->aux0.get((size_t)n * t0.vw * sb);
+            launch_scatter_block(h2->scalar, t0.data, n, t0.vw, 0, t0.vw, vm, ctx->stream);
+            launch_gather_block(h2->scalar, vm, n, 0, t0.vw, 4, tp->padded, ctx->stream);
+            KPMX_HIP(hipGetLastError());
+            t.data = tp->padded;
+        }
+        const int vw = t.vw;
+        tp->vw = vw;
+        const size_t blk = (size_t)n * vw * sb;
+        tp->blk = blk;
+        const int64_t S = M / 2;  // vectors v_0 .. v_S
+        tp->S = S;
+        tp->C = (checkpoint <= 0 || checkpoint >= S) ? S : std::max<int64_t>(2, checkpoint);
+        const int64_t C = tp->C;
+        tp->store_all = (C >= S);
+        tp->nslots = tp->store_all ? S + 1 : C + 2;
+        tp->slots = static_cast<char*>(dev_alloc((size_t)tp->nslots * blk));
+        tp->nck = tp->store_all ? 0 : (S + C - 1) / C;
+        if (tp->nck) tp->cks = static_cast<char*>(dev_alloc((size_t)tp->nck * 2 * blk));
+        double* sums = static_cast<double*>(ctx->sums.get((size_t)S * 4 * vw * sizeof(double)));
+        kpmx_tape* T = tp.get();
+        if (T->store_all) {
+            KPMX_HIP(hipMemcpyAsync(T->slot(0), t.data, blk, hipMemcpyDeviceToDevice, ctx->stream));
+            run_recurrence(
+                h2, t, MODE_DIAG, S, nullptr, sums,
+                [&](StepLaunch& L, int64_t k) {
+                    L.x = T->slot(k);
+                    L.yin = (k == 0) ? nullptr : T->slot(k - 1);
+                    L.yout = T->slot(k + 1);
+                },
+                [](StepLaunch const&, int64_t) {});
+        } else {
+            KPMX_HIP(hipMemcpyAsync(T->cks, t.data, blk, hipMemcpyDeviceToDevice, ctx->stream));
+            run_recurrence(h2, t, MODE_DIAG, S, nullptr, sums, [](StepLaunch&, int64_t) {},
+                           [&](StepLaunch const& L, int64_t k) {
+                               // after step k, L.yout holds v_{k+1}: keep the pairs (v_cC, v_cC+1)
+                               const int64_t j = k + 1;
+                               if (j % C == 0 && j / C < T->nck)
+                                   KPMX_HIP(hipMemcpyAsync(T->cks + (size_t)(j / C) * 2 * blk, L.yout, blk,
+                                                           hipMemcpyDeviceToDevice, ctx->stream));
+                               if (j % C == 1 && j / C < T->nck)
+                                   KPMX_HIP(hipMemcpyAsync(T->cks + ((size_t)(j / C) * 2 + 1) * blk, L.yout, blk,
+                                                           hipMemcpyDeviceToDevice, ctx->stream));
+                           });
+        }
+        std::vector<double> hs((size_t)S * 4 * vw);
+        KPMX_HIP(hipMemcpyAsync(hs.data(), sums, hs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        if (mu) {
+            if (h2->scalar == KPMX_F32) diag_collect<float>(hs, vw, r0->nvec, M, false, static_cast<float*>(mu), 0);
+            else diag_collect<double>(hs, vw, r0->nvec, M, false, static_cast<double*>(mu), 0);
+        }
+        *out = tp.release();
+    });
+}
+
+int kpmx_adjoint_backward(kpmx_tape* T, const double* w, double* grad) {
+    return guarded([&] {
+        KPMX_REQUIRE(T && w && grad, "null argument");
+        const kpmx_mat* h2 = T->h2;
+        kpmx_ctx* ctx = h2->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        const int64_t n = h2->rows, S = T->S, C = T->C;
+        const size_t blk = T->blk;
+        const int vw = T->vw;
+        // L = sum_j alpha_j A_j + beta_j B_j with A_j = |v_j|^2 and B_j = <v_j+1|v_j>, from the
+        // collector relations mu_0 = A_0/2, mu_1 = B_0, mu_2k = 2 A_k - A_0, mu_2k+1 = 2 B_k - B_0,
+        // averaged over the R vectors of the block
+        const double inv_r = 1.0 / (double)T->nvec;
+        std::vector<double> alpha((size_t)S + 1, 0.0), beta((size_t)S + 1, 0.0);
+        alpha[0] = 0.5 * w[0];
+        beta[0] = w[1];
+        for (int64_t k = 1; k < S; ++k) {
+            alpha[0] -= w[2 * k];
+            beta[0] -= w[2 * k + 1];
+            alpha[(size_t)k] = 2.0 * w[2 * k];
+            beta[(size_t)k] = 2.0 * w[2 * k + 1];
+        }
+        for (auto& v : alpha) v *= inv_r;
+        for (auto& v : beta) v *= inv_r;
+
+        const int64_t gsize = std::max<int64_t>(h2->sell_total, 1);
+        double* dgrad = static_cast<double*>(ctx->aux2.get((size_t)gsize * sizeof(double)));
+        launch_zero_f64(dgrad, gsize, ctx->stream);
+        void* lamA = ctx->work0.get(blk);  // lambda_{j+2}, overwritten with lambda_j
+        void* lamB = ctx->work1.get(blk);  // lambda_{j+1}
+        launch_zero(h2->scalar, lamA, n * vw, ctx->stream);
+        launch_zero(h2->scalar, lamB, n * vw, ctx->stream);
+        AdjointLaunch A;
+        A.scalar = h2->scalar;
+        A.vw = vw;
+        A.num_cu = ctx->num_cu;
+        A.indices = h2->sell_indices;
+        A.vals = h2->sell_vals;
+        A.slice_ptr = h2->slice_ptr;
+        A.slice_width = h2->slice_width;
+        A.grad = dgrad;
+        A.n = n;
+        StepLaunch R = base_launch(h2, vw, true);
+        R.mode = MODE_NONE;
+        R.num_wg = step_grid(R);
+        // slots 0.. hold v_{s0}, v_{s0+1}, ... of the current segment (recomputed from its checkpoint)
+        auto recompute = [&](int64_t s0, int64_t s1) {
+            const int64_t c = s0 / C;
+            KPMX_HIP(hipMemcpyAsync(T->slot(0), T->cks + (size_t)c * 2 * blk, blk, hipMemcpyDeviceToDevice, ctx->stream));
+            KPMX_HIP(hipMemcpyAsync(T->slot(1), T->cks + ((size_t)c * 2 + 1) * blk, blk, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+            for (int64_t j = s0 + 1; j <= std::min<int64_t>(s1, S - 1); ++j) {  // v_{j+1} = H2 v_j - v_{j-1}
+                R.x = T->slot(j - s0);
+                R.yin = T->slot(j - s0 - 1);
+                R.yout = T->slot(j - s0 + 1);
+                R.hscale = 1.0;
+                R.sub_y = 1;
+                launch(R, ctx->stream);
+            }
+        };
+        // adjoint steps j = S .. 1 (lambda_{S+1} = lambda_{S+2} = 0)
+        int64_t s1 = S;
+        for (;;) {
+            const int64_t s0 = T->store_all ? 0 : ((s1 - 1) / C) * C;
+            if (!T->store_all) recompute(s0, s1);
+            const int64_t base_j = T->store_all ? 0 : s0;
+            for (int64_t j = s1; j > s0; --j) {
+                // lambda_j = g_j + H2 lambda_{j+1} - lambda_{j+2};  grad += lambda_{j+1} v_j^T
+                A.lam1 = lamB;
+                A.lam2 = lamA;
+                A.lam_out = lamA;
+                A.v_j = T->slot(j - base_j);
+                A.v_next = (j < S) ? T->slot(j + 1 - base_j) : T->slot(j - base_j);
+                A.v_prev = T->slot(j - 1 - base_j);
+                A.ca = 2.0 * alpha[(size_t)j];
+                A.cb = (j < S) ? beta[(size_t)j] : 0.0;
+                A.cc = beta[(size_t)(j - 1)];
+                A.gscale = (j < S) ? 1.0 : 0.0;
+                A.update = 1;
+                if (launch_adjoint_step(A, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported adjoint configuration"};
+                KPMX_HIP(hipGetLastError());
+                std::swap(lamA, lamB);  // lamB <- lambda_j, lamA <- lambda_{j+1}
+            }
+            if (s0 == 0) break;
+            s1 = s0;
+        }
+        // v_1 = 0.5 H2 v_0:  grad += 0.5 lambda_1 v_0^T   (lamB holds lambda_1)
+        if (!T->store_all) recompute(0, 0);
+        A.lam1 = lamB;
+        A.lam2 = lamA;
+        A.lam_out = lamA;
+        A.v_j = A.v_next = A.v_prev = T->slot(0);
+        A.ca = A.cb = A.cc = 0.0;
+        A.gscale = 0.5;
+        A.update = 0;
+        if (launch_adjoint_step(A, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported adjoint configuration"};
+        KPMX_HIP(hipGetLastError());
+        std::vector<double> hg((size_t)gsize);
+        KPMX_HIP(hipMemcpyAsync(hg.data(), dgrad, hg.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        for (int64_t k = 0; k < h2->nnz; ++k) grad[k] = hg[(size_t)h2->sell_pos[(size_t)k]];
+    });
+}
+
+int kpmx_tape_destroy(kpmx_tape* T) {
+    return guarded([&] {
+        if (!T) return;
+        (void)hipSetDevice(T->h2->ctx->device);
+        (void)hipStreamSynchronize(T->h2->ctx->stream);
+        delete T;
+    });
+}
+
+int kpmx_adjoint_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, const double* w, int64_t checkpoint,
+                          double* grad, void* mu) {
+    kpmx_tape* T = nullptr;
+    int rc = kpmx_adjoint_forward(h2, r0, M, checkpoint, mu, &T);
+    if (rc != KPMX_OK) return rc;
+    rc = kpmx_adjoint_backward(T, w, grad);
+    const std::string err = g_last_error;
+    kpmx_tape_destroy(T);
+    if (rc != KPMX_OK) g_last_error = err;
+    return rc;
+}
+
 }  // extern "C"

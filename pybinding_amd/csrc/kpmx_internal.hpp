@@ -59,6 +59,25 @@ int launch_batched_step(const BatchedLaunch& B, hipStream_t stream);
 void launch_reduce_segments(const double* partials, const int64_t* slot_first, int64_t num_sys, int width, double* sums,
                             hipStream_t stream);
 
+struct AdjointLaunch {
+    int scalar = 1, vw = 4, num_cu = 256;
+    const int32_t* indices = nullptr;
+    const void* vals = nullptr;
+    const int64_t* slice_ptr = nullptr;
+    const int32_t* slice_width = nullptr;
+    const void* lam1 = nullptr;
+    const void* lam2 = nullptr;
+    void* lam_out = nullptr;
+    const void* v_prev = nullptr;
+    const void* v_j = nullptr;
+    const void* v_next = nullptr;
+    double ca = 0, cb = 0, cc = 0, gscale = 1;
+    int update = 1;
+    double* grad = nullptr;
+    int64_t n = 0;
+};
+int launch_adjoint_step(const AdjointLaunch& A, hipStream_t stream);
+
 int launch_step(const StepLaunch& L, hipStream_t stream);
 int step_grid(const StepLaunch& L);
 constexpr int NUM_TUNE_VARIANTS = 3;

@@ -453,6 +453,72 @@ __global__ void k_reduce_segments(const double* partials, const int64_t* slot_fi
     }
 }
 
+// Adjoint Chebyshev step for real H2 (symmetric), block of VW >= 4 vectors, SELL-64 layout:
+//   update:    lam_out = H2 lam1 - lam2 + ca v_j + cb v_next + cc v_prev      (lam_out may alias lam2)
+//   gradient:  grad[e] += gscale * sum_cols lam1[row] * v_j[col(e)]           for every stored entry e
+// lam1 = lambda_{j+1}, lam2 = lambda_{j+2}; the gradient of the stored entry e = (row, col) of H2
+// is kept in SELL order (grad has the SELL storage size).  Each entry belongs to one row, which
+// one wave owns per launch, so the accumulation needs no atomics and is deterministic.
+template <class T>
+struct AdjArgs {
+    const int32_t* indices;
+    const T* vals;
+    const T* lam1;
+    const T* lam2;
+    T* lam_out;
+    const T* v_prev;
+    const T* v_j;
+    const T* v_next;
+    double ca, cb, cc, gscale;
+    int update;
+    double* grad;
+    int64_t n;
+    int64_t num_tiles;
+};
+
+template <class T, int VW, int KMAX>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int64_t* __restrict__ slice_ptr,
+                                                         const int32_t* __restrict__ slice_width) {
+    constexpr int RPW = WAVE / VW;
+    constexpr int RW = 16;  // a workgroup tile is one 64-row slice
+    constexpr int U = RW / RPW;
+    static_assert(VW >= 4, "bad block configuration");
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    const int rsub = lane / VW;
+    const int col = lane % VW;
+    const TileWalk tw = tile_walk(a.num_tiles);
+    for (int64_t slice = tw.first; slice < tw.end; slice += tw.stride) {
+        const int64_t base = slice_ptr[slice];
+        const int width = slice_width[slice];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int rin = wave * RW + u * RPW + rsub;
+            const int64_t row = slice * WAVE + rin;
+            const bool ok = row < a.n;
+            const int64_t e = (ok ? row : a.n - 1) * VW + col;
+            const T l1 = a.lam1[e];
+            T acc = zero(T{});
+            for (int k = 0; k < width; ++k) {
+                const int64_t off = base + (int64_t)k * WAVE + rin;
+                const int64_t c = a.indices[off];
+                if (a.update) acc = fmac(a.vals[off], a.lam1[c * VW + col], acc);
+                // gradient of entry off: sum over the VW columns of lam1[row] * v_j[c]
+                double g = (double)(l1 * a.v_j[c * VW + col]);
+                for (int w = VW / 2; w >= 1; w >>= 1) g += __shfl_xor(g, w, WAVE);
+                if (col == 0 && ok) a.grad[off] += a.gscale * g;
+            }
+            if (a.update && ok) {
+                T lnew = sub(acc, a.lam2[e]);
+                lnew = fmac((T)a.ca, a.v_j[e], lnew);
+                if (a.cb != 0.0) lnew = fmac((T)a.cb, a.v_next[e], lnew);
+                if (a.cc != 0.0) lnew = fmac((T)a.cc, a.v_prev[e], lnew);
+                a.lam_out[e] = lnew;
+            }
+        }
+    }
+}
+
 // sums[s][j] = sum over workgroups g (in order) of partials[s][g][j], for s in [0, nsteps).
 // Block (s, chunk) covers columns [64 chunk, 64 chunk + 64) of step s: 4 groups of 64 threads
 // sum interleaved workgroup subsets, then the 4 subtotals are combined in order through LDS.
@@ -871,6 +937,52 @@ void launch_reduce_segments(const double* partials, const int64_t* slot_first, i
     if (g < 1) g = 1;
     if (g > 65536) g = 65536;
     hipLaunchKernelGGL(k_reduce_segments, dim3((unsigned)g), dim3(256), 0, stream, partials, slot_first, num_sys, width, sums);
+}
+
+template <class T, int VW>
+static void adjoint_t(const AdjointLaunch& A, hipStream_t stream) {
+    AdjArgs<T> a{};
+    a.indices = A.indices;
+    a.vals = static_cast<const T*>(A.vals);
+    a.lam1 = static_cast<const T*>(A.lam1);
+    a.lam2 = static_cast<const T*>(A.lam2);
+    a.lam_out = static_cast<T*>(A.lam_out);
+    a.v_prev = static_cast<const T*>(A.v_prev);
+    a.v_j = static_cast<const T*>(A.v_j);
+    a.v_next = static_cast<const T*>(A.v_next);
+    a.ca = A.ca;
+    a.cb = A.cb;
+    a.cc = A.cc;
+    a.gscale = A.gscale;
+    a.update = A.update;
+    a.grad = A.grad;
+    a.n = A.n;
+    a.num_tiles = (A.n + WAVE - 1) / WAVE;
+    int64_t g = a.num_tiles < (int64_t)A.num_cu * 8 ? a.num_tiles : (int64_t)A.num_cu * 8;
+    if (g >= 8) g -= g % 8;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL((k_adjoint_sellb<T, VW, 8>), dim3((unsigned)g), dim3(BLOCK), 0, stream, a, A.slice_ptr,
+                       A.slice_width);
+}
+
+template <class T>
+static int adjoint_vw(const AdjointLaunch& A, hipStream_t stream) {
+    switch (A.vw) {
+        case 4: adjoint_t<T, 4>(A, stream); return 0;
+        case 8: adjoint_t<T, 8>(A, stream); return 0;
+        case 16: adjoint_t<T, 16>(A, stream); return 0;
+        case 32: adjoint_t<T, 32>(A, stream); return 0;
+        case 64: adjoint_t<T, 64>(A, stream); return 0;
+        default: return -1;
+    }
+}
+
+int launch_adjoint_step(const AdjointLaunch& A, hipStream_t stream) {
+    switch (A.scalar) {
+        case 0: return adjoint_vw<float>(A, stream);
+        case 1: return adjoint_vw<double>(A, stream);
+        default: return -1;  // real Hamiltonians only
+    }
 }
 
 }  // namespace kpmx
