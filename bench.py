@@ -138,6 +138,95 @@ def batched_realizations(steps=2):
             "workload": "256 square flakes, %d sites, 1024 moments, 4 vectors each, one HIP graph" % n}
 
 
+def adjoint_cost(steps=2):
+    """Strained graphene flake of ~1M sites (161.8 nm square, Gaussian bump h = 1 nm, sigma = 5 nm),
+    fp64, 1024 moments, 16 vectors, checkpoint every 64 steps: time of one forward moment
+    computation vs one adjoint gradient (forward with checkpoints + recompute + adjoint)."""
+    from pybinding_amd import abi, generators
+    from pybinding_amd.autograd import TightBinding
+    t0 = time.time()
+    n, bonds, hop, onsite = generators.strained_graphene_flake(161.8, 161.8)
+    M, R = 1024, 16
+    tb = TightBinding(n, bonds, (-9.0, 9.0), M, R, checkpoint=64)
+    h2 = tb.scaled(onsite, hop)
+    mat = tb.ctx.mat(h2, fmt=1)
+    # loss weights: the Jackson-damped DOS integrated against a Gaussian window at 0.5 eV, 0.1 eV wide
+    a, b = tb.scale
+    xq = np.cos(np.pi * (np.arange(4096) + 0.5) / 4096)
+    win = np.exp(-0.5 * ((a * xq + b - 0.5) / 0.1) ** 2)
+    g = ((M + 1 - np.arange(M)) * np.cos(np.pi * np.arange(M) / (M + 1)) +
+         np.sin(np.pi * np.arange(M) / (M + 1)) / np.tan(np.pi / (M + 1))) / (M + 1)
+    w = g * (2 / np.pi) * np.pi / 4096 * (np.cos(np.arange(M)[:, None] * np.arccos(xq)[None, :]) @ win)
+    log("adjoint: %d sites, %d bonds (%.1fs setup)" % (n, len(bonds), time.time() - t0))
+    abi.moments_diagonal_run(mat, tb.block, M)
+    tb.ctx.synchronize()
+    tb.ctx.timer_start()
+    for _ in range(steps):
+        abi.moments_diagonal_run(mat, tb.block, M)
+    fwd = tb.ctx.timer_stop() / steps
+    grad, _ = abi.adjoint_diagonal(mat, tb.block, M, w, checkpoint=64)
+    tb.ctx.timer_start()
+    for _ in range(steps):
+        grad, _ = abi.adjoint_diagonal(mat, tb.block, M, w, checkpoint=64)
+    adj = tb.ctx.timer_stop() / steps
+    tb.ctx.timer_start()
+    for _ in range(steps):
+        grad_all, _ = abi.adjoint_diagonal(mat, tb.block, M, w, checkpoint=0)
+    adj_all = tb.ctx.timer_stop() / steps
+    assert np.all(np.isfinite(grad))
+    mat.close()
+    tb.close()
+    log("adjoint: forward %.1f ms, gradient %.1f ms (checkpoint 64) = %.2fx, %.1f ms storing all = %.2fx"
+        % (fwd, adj, adj / fwd, adj_all, adj_all / fwd))
+    return {"forward_ms": fwd, "gradient_ms_checkpoint64": adj, "ratio_checkpoint64": adj / fwd,
+            "gradient_ms_store_all": adj_all, "ratio_store_all": adj_all / fwd,
+            "workload": "strained graphene flake, %d sites, 1024 moments, 16 vectors, fp64" % n}
+
+
+def kubo_contraction(n1=724, n2=724, M=512, R=16):
+    """Kubo-Bastin mu_nm of a Peierls graphene supercell (B = 20 T, W = 1 eV, complex128),
+    512 x 512 moments, 16 vectors: device time of the FP64 and FP32 split-precision MFMA paths,
+    and the FP32 result's error against the FP64 one."""
+    from pybinding_amd import abi, generators
+    from pybinding_amd._native import load_module
+    import scipy.sparse as sp
+    native = load_module()
+    t0 = time.time()
+    h, x, y = generators.peierls_honeycomb_supercell(n1, n2, 20.0, disorder=1.0)
+    n = h.shape[0]
+    ip, ix, dv, rmap, _, _, _, (a, b) = native.optimize(h.shape, h.indptr, h.indices, h.data, (-9.5, 9.5), True,
+                                                        [0], [0])
+    h2 = sp.csr_matrix((dv, ix, ip), shape=h.shape)
+    # velocity v_ij = (x_i - x_j) H_ij in the optimized order
+    perm = np.argsort(rmap)
+    xo = x.astype(np.float32)[perm]
+    hh = h2.tocoo()
+    v = sp.csr_matrix(((xo[hh.row] - xo[hh.col]).astype(np.float64) * hh.data * (a / 2), (hh.row, hh.col)),
+                      shape=h.shape)
+    r0 = native.random_starters(n, R, "complex128", rmap)
+    ctx = abi.Ctx(0)
+    mat = ctx.mat(h2, fmt=1)
+    vm = ctx.mat(v)
+    blk = ctx.block(r0)
+    log("kubo: %d sites (%.1fs setup)" % (n, time.time() - t0))
+    out = {}
+    results = {}
+    for mode, name in ((0, "fp64"), (1, "fp32_split")):
+        ctx.timer_start()
+        results[name] = abi.kubo_moments(mat, vm, vm, blk, M, mode=mode)
+        out[name + "_s"] = ctx.timer_stop() / 1000
+    err = np.abs(results["fp32_split"] - results["fp64"]).max() / np.abs(results["fp64"]).max()
+    flops = 8.0 * M * M * n * R
+    out.update({"contraction_flop": flops, "fp32_split_max_rel_error_vs_fp64": float(err),
+                "workload": "Peierls graphene %dx%d cells (%d sites), B=20 T, W=1 eV, %dx%d moments, %d vectors"
+                            % (n1, n2, n, M, M, R)})
+    log("kubo: fp64 %.2f s, fp32-split %.2f s, error %.2e" % (out["fp64_s"], out["fp32_split_s"], err))
+    for hnd in (blk, vm, mat):
+        hnd.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +242,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-vector (plain) comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-realizations run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient cost run")
+    ap.add_argument("--no-kubo", action="store_true", help="skip the Kubo contraction run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 2 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -235,6 +326,10 @@ def main():
 
     if not args.no_batched:
         extra["batched_realizations"] = batched_realizations()
+    if not args.no_adjoint:
+        extra["adjoint_gradient"] = adjoint_cost()
+    if not args.no_kubo:
+        extra["kubo_contraction"] = kubo_contraction()
 
     cpu = None
     if not args.no_cpu_baseline:

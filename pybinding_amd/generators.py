@@ -144,3 +144,52 @@ def honeycomb_flake(width, height, t=T_GRAPHENE, dtype=np.float64, return_positi
     if return_positions:
         return h, pos[valid]
     return h
+
+
+def honeycomb_positions(n1, n2):
+    """Site positions (x, y) of `honeycomb_supercell(n1, n2)` in its site order (float64)."""
+    i, j = np.meshgrid(np.arange(n1), np.arange(n2), indexing="xy")
+    i = i.ravel()
+    j = j.ravel()
+    x = i * A_LAT + j * A_LAT / 2
+    y = j * A_LAT / 2 * np.sqrt(3)
+    return np.concatenate([x, x]), np.concatenate([y - A_CC / 2, y + A_CC / 2])
+
+
+def peierls_honeycomb_supercell(n1, n2, field, disorder=0.0, seed=5, t=T_GRAPHENE):
+    """Periodic graphene supercell with a perpendicular magnetic field `field` (tesla) through
+    Peierls phases exp(i c (y1 + y2)/2 B (x1 - x2)), c = 1e-18 * 2 pi / phi0 with the reference's
+    float constants (pybinding/repository/graphene/modifiers.py:54-73), plus Anderson disorder.
+    Returns (H complex128 CSR, x, y)."""
+    x, y = honeycomb_positions(n1, n2)
+    i, j = np.meshgrid(np.arange(n1), np.arange(n2), indexing="xy")
+    i = i.ravel()
+    j = j.ravel()
+    ncell = n1 * n2
+    a_idx = j * n1 + i
+    rows, cols = [], []
+    for di, dj in _HONEYCOMB_HOPS:
+        rows.append(a_idx)
+        cols.append(ncell + ((j + dj) % n2) * n1 + (i + di) % n1)
+    rows = np.concatenate(rows)
+    cols = np.concatenate(cols)
+    hbar = np.float32(6.58211899e-16)
+    phi0 = np.float32(2) * np.float32(np.pi) * hbar
+    const = 1e-18 * 2 * np.pi / float(phi0)
+    vp_x = 0.5 * field * (y[rows] + y[cols])
+    vals = t * np.exp(1j * const * vp_x * (x[rows] - x[cols]))
+    return _assemble(2 * ncell, rows, cols, vals, _anderson(2 * ncell, disorder, seed), np.complex128), x, y
+
+
+def strained_graphene_flake(width, height, bump_height=1.0, bump_sigma=5.0):
+    """Graphene flake with a Gaussian bump z = h exp(-r^2 / sigma^2) and strained hoppings
+    t = -2.8 exp(-3.37 (d / a_cc - 1)) (pybinding/repository/graphene/modifiers.py:76-82,
+    118-137).  Returns (num_sites, bonds (E, 2), hoppings (E,), onsite zeros (N,))."""
+    h, pos = honeycomb_flake(width, height, return_positions=True)
+    coo = sp.triu(h, k=1).tocoo()
+    bonds = np.stack([coo.row, coo.col], axis=1)
+    z = bump_height * np.exp(-(pos[:, 0] ** 2 + pos[:, 1] ** 2) / bump_sigma ** 2)
+    d = np.sqrt((pos[bonds[:, 0], 0] - pos[bonds[:, 1], 0]) ** 2 + (pos[bonds[:, 0], 1] - pos[bonds[:, 1], 1]) ** 2
+                + (z[bonds[:, 0]] - z[bonds[:, 1]]) ** 2)
+    hop = T_GRAPHENE * np.exp(-3.37 * (d / A_CC - 1))
+    return h.shape[0], bonds, hop, np.zeros(h.shape[0])

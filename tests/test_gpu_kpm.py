@@ -120,3 +120,34 @@ def test_site_order_does_not_change_moments(oracle, order):
     h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
     ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], 4, np.float64), 130).mean(axis=0)
     assert np.max(np.abs(mu - ref)) <= 1e-10 * abs(ref[0])
+
+
+def test_conductivity_matches_oracle(oracle):
+    """calc_conductivity against the oracle: dense panels L = T_n(H~) v_x r and R = v_x T_m(H~) r,
+    mu_nm = mean_r L R^H, 2-D Jackson damping, Kubo-Bastin reconstruction."""
+    h, pos = generators.honeycomb_flake(5, 4, return_positions=True)
+    d = np.random.default_rng(7).uniform(-0.5, 0.5, h.shape[0])
+    h = (h + sp.diags(d)).tocsr()
+    n = h.shape[0]
+    model = pb.Model(h, pb.model.System(pos[:, 0], pos[:, 1]))
+    kpm = pb.kpm(model, energy_range=(-9, 9), silent=True)
+    mu_e = np.linspace(-2, 2, 9)
+    sigma = kpm.calc_conductivity(mu_e, broadening=0.8, temperature=0, num_random=3, num_points=50)
+    a, b = oracle.scale_factors(-9, 9)
+    M = oracle.required_num_moments("jackson", 0.8 / a)
+    h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
+    x = pos[:, 0].astype(np.float32)
+    v = oracle.velocity(h, x)
+    r0 = oracle.random_starter(n, 3, np.float64)
+    mu = np.zeros((M, M))
+    for k in range(3):
+        L = oracle.offdiag_moments(h2, v @ r0[k], M, "dense")
+        R = oracle.offdiag_moments(h2, r0[k], M, "dense", op=v)
+        mu += L @ R.T
+    mu /= 3
+    g = oracle.jackson(M)
+    mu = mu * g[:, None] * g[None, :]
+    lo, hi = float(np.float32(-9)), float(np.float32(9))
+    samples = np.linspace(lo, hi, 50)
+    ref = oracle.kubo_bastin(mu, mu_e, samples, 0.0, a, b).real
+    np.testing.assert_allclose(sigma.data, ref, rtol=1e-8, atol=1e-10 * np.abs(ref).max())
