@@ -247,6 +247,41 @@ def required_num_moments(kind, scaled_broadening, lam=4.0):
     return round_num_moments(int(num / scaled_broadening) + 1)
 
 
+def lanczos_bounds(h, precision_percent=0.002, seed=0, loop_limit=1000):
+    """Lanczos min/max eigenvalues (cppcore/include/compute/lanczos.hpp:93-154): three-term
+    recurrence, eigenvalues of the growing tridiagonal matrix, stop when both extremes change by
+    less than `precision_percent` % between two iterations.  The start vector is numpy's, not
+    Eigen's Random, so the result agrees with the reference's to the stopping precision only.
+    Returns (min, max, loops).  Pinned through the reference's LDOS/DOS fixtures, which use
+    these bounds for their scaling."""
+    from scipy.linalg import eigvalsh_tridiagonal
+    h = h.tocsr().astype(np.complex128 if np.iscomplexobj(h.data) else np.float64)
+    rng = np.random.default_rng(seed)
+    v1 = rng.uniform(-1, 1, h.shape[0])
+    if np.iscomplexobj(h.data):
+        v1 = v1 + 1j * rng.uniform(-1, 1, h.shape[0])
+    v1 /= np.linalg.norm(v1)
+    v0 = np.zeros_like(v1)
+    alpha, beta = [], []
+    prev_min, prev_max = np.inf, -np.inf
+    prec = precision_percent / 100
+    for i in range(loop_limit):
+        b_prev = beta[-1] if beta else 0.0
+        w = h @ v1 - b_prev * v0
+        a = float(np.real(np.vdot(v1, w)))
+        w = w - a * v1
+        b = float(np.linalg.norm(w))
+        v0, v1 = v1, w / b
+        alpha.append(a)
+        beta.append(b)
+        ev = eigvalsh_tridiagonal(np.array(alpha), np.array(beta[:-1])) if len(alpha) > 1 else np.array(alpha)
+        mn, mx = ev.min(), ev.max()
+        if abs((prev_min - mn) / mn) < prec and abs((prev_max - mx) / mx) < prec:
+            return float(mn), float(mx), i
+        prev_min, prev_max = mn, mx
+    raise RuntimeError("Lanczos algorithm did not converge for the min/max eigenvalues.")
+
+
 def bfs_reorder(h, src, b_nonzero=False):
     """BFS reorder from `src[0]` and slice borders (OptimizedHamiltonian.cpp:74-152).
     Returns (reorder_map, borders): reorder_map[original] = optimized index.

@@ -49,14 +49,16 @@ class System:
         return int(self._offsets[-1])
 
     def find_nearest(self, position, sublattice=""):
-        """Index of the site nearest to `position` (System::find_nearest: float distances,
-        the first site wins ties), optionally restricted to one sublattice."""
+        """Index of the site nearest to `position` (System::find_nearest,
+        cppcore/src/system/System.cpp:67-81: float32 distances, the first site wins ties),
+        optionally restricted to one sublattice."""
         p = np.zeros(3, dtype=np.float32)
         p[:len(position)] = np.asarray(position, dtype=np.float32)
-        d = (self.x - p[0]) ** 2 + (self.y - p[1]) ** 2 + (self.z - p[2]) ** 2
+        dx, dy, dz = self.x - p[0], self.y - p[1], self.z - p[2]
+        d = np.sqrt(dx * dx + dy * dy + dz * dz)
         if sublattice:
             sid = self.sublattice_names.index(sublattice)
-            d = np.where(self.sublattices == sid, d, np.inf)
+            d = np.where(self.sublattices == sid, d, np.float32(np.inf))
         return int(np.argmin(d))
 
     def to_hamiltonian_indices(self, site):
@@ -72,11 +74,25 @@ class System:
 
 
 class Model:
-    """A built tight-binding model: Hamiltonian + System."""
+    """A built tight-binding model: Hamiltonian + System.
 
-    def __init__(self, hamiltonian, system=None):
-        self.hamiltonian = sp.csr_matrix(hamiltonian)
-        self.system = system
+    `Model(lattice, shape, *modifiers)` builds it like `pb.Model` (pybinding/model.py:14-70) in
+    the reference's site order; `Model(hamiltonian, system)` wraps an existing CSR matrix.
+    """
+
+    def __init__(self, hamiltonian, *args):
+        from .lattice import Lattice, build
+        if isinstance(hamiltonian, Lattice):
+            if not args:
+                raise RuntimeError("a shape is needed to build a finite model")
+            self.lattice = hamiltonian
+            self.shape = args[0]
+            self.modifiers = list(args[1:])
+            self.hamiltonian, self.system = build(self.lattice, self.shape, *self.modifiers)
+        else:
+            self.lattice = None
+            self.hamiltonian = sp.csr_matrix(hamiltonian)
+            self.system = args[0] if args else None
 
     @property
     def raw_hamiltonian(self):
