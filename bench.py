@@ -227,6 +227,17 @@ def kubo_contraction(n1=724, n2=724, M=512, R=16):
     return out
 
 
+def counted_traffic(key):
+    """HBM bytes per step launch measured by rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, each
+    scaled by its copy-kernel calibration factor) for this exact workload and kernel variant, read
+    from profiles/hbm_traffic.json; None when that workload was not counted."""
+    try:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")) as f:
+            return json.load(f)[key]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +315,8 @@ def main():
     log("GPU: %.1f ms per moment computation, %.4f ms per step launch, %.1f GB/s algorithmic"
         % (gpu_ms / args.steps, step_ms, achieved / 1e9))
 
+    traffic_key = "%dx%d %s v%d %s %s variant%d" % (args.cells[0], args.cells[1], dtype.name, R, args.format,
+                                                 args.order, variant)
     extra = {}
     if not args.no_plain:
         # the plain single-vector path (one vector, SELL-64 kernel) on the same matrix, timed over
@@ -344,9 +357,9 @@ def main():
     ctx.close()
 
     line = {
-        "metric": "KPM GNNZ*moments/s (block SpMM KPM DOS moments)",
+        "metric": "KPM GNNZ\u00b7moments/s and % HBM roofline; batched/adjoint vs plain GPU",
         "value": value,
-        "unit": "GNNZ*moments/s",
+        "unit": "GNNZ\u00b7moments/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -361,7 +374,7 @@ def main():
                    "matrix_format": args.format, "site_order": args.order, "kernel_variant": variant,
                    "moments": M, "vectors": R},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None},
+                     "frac": achieved / HBM_PEAK, "traffic": counted_traffic(traffic_key)},
         "cpu_baseline": cpu,
     }
     line.update(extra)

