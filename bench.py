@@ -229,8 +229,9 @@ def kubo_contraction(n1=724, n2=724, M=512, R=16):
 
 def counted_traffic(key):
     """HBM bytes per step launch measured by rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, each
-    scaled by its copy-kernel calibration factor) for this exact workload and kernel variant, read
-    from profiles/hbm_traffic.json; None when that workload was not counted."""
+    scaled by its copy-kernel calibration factor) for this exact workload, read from
+    profiles/hbm_traffic.json (the autotuner's tile-shape variants move the same bytes); None when
+    that workload was not counted."""
     try:
         with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")) as f:
             return json.load(f)[key]["bytes_per_launch"]
@@ -263,10 +264,17 @@ def main():
         import torch
         a = torch.ones(2**28, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
+        b.copy_(a)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8}))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 5
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8,
+                          "copy_ms": ms, "copy_GBps": 2 * a.numel() * 8 / (ms / 1000) / 1e9}))
         return
 
     if args.gpus != 1:
@@ -315,8 +323,7 @@ def main():
     log("GPU: %.1f ms per moment computation, %.4f ms per step launch, %.1f GB/s algorithmic"
         % (gpu_ms / args.steps, step_ms, achieved / 1e9))
 
-    traffic_key = "%dx%d %s v%d %s %s variant%d" % (args.cells[0], args.cells[1], dtype.name, R, args.format,
-                                                 args.order, variant)
+    traffic_key = "%dx%d %s v%d %s %s" % (args.cells[0], args.cells[1], dtype.name, R, args.format, args.order)
     extra = {}
     if not args.no_plain:
         # the plain single-vector path (one vector, SELL-64 kernel) on the same matrix, timed over

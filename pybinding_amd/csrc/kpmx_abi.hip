@@ -1501,16 +1501,25 @@ int kpmx_adjoint_backward(kpmx_tape* T, const double* w, double* grad) {
         A.slice_width = h2->slice_width;
         A.grad = dgrad;
         A.n = n;
+        A.max_width = h2->max_width;
         StepLaunch R = base_launch(h2, vw, true);
         R.mode = MODE_NONE;
         R.num_wg = step_grid(R);
+        // With C_j = <v_j|H2|v_j>, B_0 = C_0 / 2 and B_j = C_j - B_j-1 (j >= 1, from
+        // v_j+1 = H2 v_j - v_j-1), so L = sum_j alpha_j A_j + sum_{j>=1} gamma_j C_j + gamma_0 C_0 / 2
+        // with gamma_j = sum_{i>=j} (-1)^(i-j) beta_i.  v_S no longer enters, and
+        //   lambda_j = 2 alpha_j v_j + 2 gamma_j H2 v_j + H2 lambda_j+1 - lambda_j+2     (j = S-1 .. 1)
+        //   dL/dH2   = sum_{j>=1} (lambda_j+1 + gamma_j v_j) v_j^T + (lambda_1 + gamma_0 v_0) v_0^T / 2
+        std::vector<double> gamma((size_t)S + 1, 0.0);
+        for (int64_t j = S - 1; j >= 0; --j) gamma[(size_t)j] = beta[(size_t)j] - gamma[(size_t)j + 1];
+
         // slots 0.. hold v_{s0}, v_{s0+1}, ... of the current segment (recomputed from its checkpoint)
         auto recompute = [&](int64_t s0, int64_t s1) {
             const int64_t c = s0 / C;
             KPMX_HIP(hipMemcpyAsync(T->slot(0), T->cks + (size_t)c * 2 * blk, blk, hipMemcpyDeviceToDevice, ctx->stream));
             KPMX_HIP(hipMemcpyAsync(T->slot(1), T->cks + ((size_t)c * 2 + 1) * blk, blk, hipMemcpyDeviceToDevice,
                                     ctx->stream));
-            for (int64_t j = s0 + 1; j <= std::min<int64_t>(s1, S - 1); ++j) {  // v_{j+1} = H2 v_j - v_{j-1}
+            for (int64_t j = s0 + 1; j < s1; ++j) {  // v_{j+1} = H2 v_j - v_{j-1}, up to v_{s1}
                 R.x = T->slot(j - s0);
                 R.yin = T->slot(j - s0 - 1);
                 R.yout = T->slot(j - s0 + 1);
@@ -1519,24 +1528,21 @@ int kpmx_adjoint_backward(kpmx_tape* T, const double* w, double* grad) {
                 launch(R, ctx->stream);
             }
         };
-        // adjoint steps j = S .. 1 (lambda_{S+1} = lambda_{S+2} = 0)
-        int64_t s1 = S;
+        // adjoint steps j = S-1 .. 1 (lambda_S = lambda_S+1 = 0)
+        int64_t s1 = S - 1;
         for (;;) {
             const int64_t s0 = T->store_all ? 0 : ((s1 - 1) / C) * C;
             if (!T->store_all) recompute(s0, s1);
             const int64_t base_j = T->store_all ? 0 : s0;
             for (int64_t j = s1; j > s0; --j) {
-                // lambda_j = g_j + H2 lambda_{j+1} - lambda_{j+2};  grad += lambda_{j+1} v_j^T
                 A.lam1 = lamB;
                 A.lam2 = lamA;
                 A.lam_out = lamA;
                 A.v_j = T->slot(j - base_j);
-                A.v_next = (j < S) ? T->slot(j + 1 - base_j) : T->slot(j - base_j);
-                A.v_prev = T->slot(j - 1 - base_j);
                 A.ca = 2.0 * alpha[(size_t)j];
-                A.cb = (j < S) ? beta[(size_t)j] : 0.0;
-                A.cc = beta[(size_t)(j - 1)];
-                A.gscale = (j < S) ? 1.0 : 0.0;
+                A.chv = 2.0 * gamma[(size_t)j];
+                A.cg = gamma[(size_t)j];
+                A.gscale = 1.0;
                 A.update = 1;
                 if (launch_adjoint_step(A, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported adjoint configuration"};
                 KPMX_HIP(hipGetLastError());
@@ -1545,13 +1551,14 @@ int kpmx_adjoint_backward(kpmx_tape* T, const double* w, double* grad) {
             if (s0 == 0) break;
             s1 = s0;
         }
-        // v_1 = 0.5 H2 v_0:  grad += 0.5 lambda_1 v_0^T   (lamB holds lambda_1)
+        // v_1 = 0.5 H2 v_0 and C_0 / 2:  grad += 0.5 (lambda_1 + gamma_0 v_0) v_0^T   (lamB holds lambda_1)
         if (!T->store_all) recompute(0, 0);
         A.lam1 = lamB;
         A.lam2 = lamA;
         A.lam_out = lamA;
-        A.v_j = A.v_next = A.v_prev = T->slot(0);
-        A.ca = A.cb = A.cc = 0.0;
+        A.v_j = T->slot(0);
+        A.ca = A.chv = 0.0;
+        A.cg = gamma[0];
         A.gscale = 0.5;
         A.update = 0;
         if (launch_adjoint_step(A, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported adjoint configuration"};

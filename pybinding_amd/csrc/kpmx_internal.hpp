@@ -68,11 +68,9 @@ struct AdjointLaunch {
     const void* lam1 = nullptr;
     const void* lam2 = nullptr;
     void* lam_out = nullptr;
-    const void* v_prev = nullptr;
     const void* v_j = nullptr;
-    const void* v_next = nullptr;
-    double ca = 0, cb = 0, cc = 0, gscale = 1;
-    int update = 1;
+    double ca = 0, chv = 0, cg = 0, gscale = 1;
+    int update = 1, max_width = 0;
     double* grad = nullptr;
     int64_t n = 0;
 };

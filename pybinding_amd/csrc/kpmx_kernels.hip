@@ -212,7 +212,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellb(StepArgs<T> a, const int64
     constexpr int U = RW / RPW;                    // row groups per wave per tile
     constexpr int TR = RW * WAVES_PER_BLOCK;       // rows per tile (divides 64)
     constexpr int TPS = WAVE / TR;                 // tiles per slice
-    static_assert(VW >= 4 && RW % RPW == 0 && WAVE % TR == 0, "bad block configuration");
+    static_assert(VW >= 4 && KMAX % 4 == 0 && RW % RPW == 0 && WAVE % TR == 0, "bad block configuration");
     __shared__ int32_t s_idx[KMAX * TR];
     __shared__ T s_val[KMAX * TR];
     const int lane = threadIdx.x % WAVE;
@@ -454,11 +454,15 @@ __global__ void k_reduce_segments(const double* partials, const int64_t* slot_fi
 }
 
 // Adjoint Chebyshev step for real H2 (symmetric), block of VW >= 4 vectors, SELL-64 layout:
-//   update:    lam_out = H2 lam1 - lam2 + ca v_j + cb v_next + cc v_prev      (lam_out may alias lam2)
-//   gradient:  grad[e] += gscale * sum_cols lam1[row] * v_j[col(e)]           for every stored entry e
-// lam1 = lambda_{j+1}, lam2 = lambda_{j+2}; the gradient of the stored entry e = (row, col) of H2
-// is kept in SELL order (grad has the SELL storage size).  Each entry belongs to one row, which
-// one wave owns per launch, so the accumulation needs no atomics and is deterministic.
+//   update:    lam_out = H2 lam1 - lam2 + ca v_j + chv (H2 v_j)
+//   gradient:  grad[e] += gscale * sum_cols (lam1[row] + cg v_j[row]) * v_j[col(e)]   for every stored entry e
+// lam1 = lambda_{j+1}, lam2 = lambda_{j+2} (lam_out may alias lam2).  The driver writes the loss
+// over A_j = <v_j|v_j> and C_j = <v_j|H2|v_j> (the B_j = <v_j+1|v_j> terms become alternating
+// sums of C's through v_j+1 = H2 v_j - v_j-1), so the source term 2 alpha_j v_j + 2 gamma_j H2 v_j
+// and the explicit gradient gamma_j v_j v_j^T need only v_j, whose gathers the H2 v_j product
+// shares.  The gradient of entry e = (row, col) is kept in SELL order; every entry belongs to one
+// row owned by one wave per launch, so the accumulation needs no atomics and is deterministic.
+// Tiles, LDS staging and the gather schedule follow k_step_sellb.
 template <class T>
 struct AdjArgs {
     const int32_t* indices;
@@ -466,54 +470,170 @@ struct AdjArgs {
     const T* lam1;
     const T* lam2;
     T* lam_out;
-    const T* v_prev;
     const T* v_j;
-    const T* v_next;
-    double ca, cb, cc, gscale;
+    double ca, chv, cg, gscale;
     int update;
     double* grad;
     int64_t n;
     int64_t num_tiles;
 };
 
-template <class T, int VW, int KMAX>
+template <int VW>
+__device__ __forceinline__ double reduce4(double p0, double p1, double p2, double p3, int col) {
+    constexpr int H = VW / 2, Q = VW / 4;
+    const bool hi = (col & H) != 0;
+    double s0 = hi ? p2 : p0, s1 = hi ? p3 : p1;   // kept
+    double t0 = hi ? p0 : p2, t1 = hi ? p1 : p3;   // sent
+    s0 += __shfl_xor(t0, H, WAVE);
+    s1 += __shfl_xor(t1, H, WAVE);
+    const bool qb = (col & Q) != 0;
+    double r = qb ? s1 : s0;
+    r += __shfl_xor(qb ? s0 : s1, Q, WAVE);
+    for (int w = Q / 2; w >= 1; w >>= 1) r += __shfl_xor(r, w, WAVE);
+    return r;
+}
+
+template <class T, int VW, int KMAX, int RW, bool OFF32>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int64_t* __restrict__ slice_ptr,
                                                          const int32_t* __restrict__ slice_width) {
     constexpr int RPW = WAVE / VW;
-    constexpr int RW = 16;  // a workgroup tile is one 64-row slice
     constexpr int U = RW / RPW;
-    static_assert(VW >= 4, "bad block configuration");
+    constexpr int TR = RW * WAVES_PER_BLOCK;
+    constexpr int TPS = WAVE / TR;
+    constexpr int Q = VW / 4;   // lanes per reduced value in reduce4
+    static_assert(VW >= 4 && KMAX % 4 == 0 && RW % RPW == 0 && WAVE % TR == 0, "bad block configuration");
+    __shared__ int32_t s_idx[KMAX * TR];
+    __shared__ T s_val[KMAX * TR];
     const int lane = threadIdx.x % WAVE;
     const int wave = threadIdx.x / WAVE;
     const int rsub = lane / VW;
     const int col = lane % VW;
+    const bool glane = (col % Q) == 0;   // this lane stores one reduced gradient per 4 entries:
+    const int gk = (col / Q) & 3;         // entry k4 + gk of its row
+    const T* __restrict__ lam1 = a.lam1;
+    const T* __restrict__ vj = a.v_j;
+    const int32_t* __restrict__ sidx = a.indices;
+    const T* __restrict__ sval = a.vals;
     const TileWalk tw = tile_walk(a.num_tiles);
-    for (int64_t slice = tw.first; slice < tw.end; slice += tw.stride) {
-        const int64_t base = slice_ptr[slice];
-        const int width = slice_width[slice];
+    // software pipeline of the tile's indices and values, as in k_step_sellb
+    constexpr int E = (KMAX * TR + BLOCK - 1) / BLOCK;
+    int32_t p_idx[E];
+    T p_val[E];
+    auto prefetch = [&](int64_t tile, int64_t& pbase, int& pwidth) {
+        const int64_t slc = tile / TPS;
+        const int rr0 = (int)(tile % TPS) * TR;
+        pbase = slice_ptr[slc];
+        pwidth = slice_width[slc];
+        const int kk = pwidth < KMAX ? pwidth : KMAX;
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * BLOCK;
+            const int k = e / TR, r = e % TR;
+            if (e < KMAX * TR && k < kk) {
+                p_idx[q] = sidx[pbase + (int64_t)k * WAVE + rr0 + r];
+                p_val[q] = sval[pbase + (int64_t)k * WAVE + rr0 + r];
+            } else {
+                const int64_t own = slc * WAVE + rr0 + r;
+                p_idx[q] = (int32_t)(own < a.n ? own : a.n - 1);
+                p_val[q] = zero(T{});
+            }
+        }
+    };
+    auto gather = [&](const T* __restrict__ v, int32_t c) -> T {
+        if constexpr (OFF32) {
+            // the whole block is below 4 GiB: a 32-bit byte offset from the uniform base
+            const uint32_t off = ((uint32_t)c * VW + col) * (uint32_t)sizeof(T);
+            return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(v) + off);
+        } else {
+            return v[(int64_t)c * VW + col];
+        }
+    };
+    int64_t next_base = 0;
+    int next_width = 0;
+    if (tw.first < tw.end) prefetch(tw.first, next_base, next_width);
+    for (int64_t tile = tw.first; tile < tw.end; tile += tw.stride) {
+        const int64_t slice = tile / TPS;
+        const int r0 = (int)(tile % TPS) * TR;
+        const int64_t base = next_base;
+        const int width = next_width;
+        const int kst = width < KMAX ? width : KMAX;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * BLOCK;
+            if (e < KMAX * TR) {
+                s_idx[e] = p_idx[q];
+                s_val[e] = p_val[q];
+            }
+        }
+        __syncthreads();
+        int rin[U];
+        int64_t row[U];
+        bool ok[U];
+        T l1[U], l2[U], vo[U];
+        double gold[U][KMAX / 4];
+        T lg[U][KMAX], vg[U][KMAX];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int rin = wave * RW + u * RPW + rsub;
-            const int64_t row = slice * WAVE + rin;
-            const bool ok = row < a.n;
-            const int64_t e = (ok ? row : a.n - 1) * VW + col;
-            const T l1 = a.lam1[e];
-            T acc = zero(T{});
-            for (int k = 0; k < width; ++k) {
-                const int64_t off = base + (int64_t)k * WAVE + rin;
-                const int64_t c = a.indices[off];
-                if (a.update) acc = fmac(a.vals[off], a.lam1[c * VW + col], acc);
-                // gradient of entry off: sum over the VW columns of lam1[row] * v_j[c]
-                double g = (double)(l1 * a.v_j[c * VW + col]);
-                for (int w = VW / 2; w >= 1; w >>= 1) g += __shfl_xor(g, w, WAVE);
-                if (col == 0 && ok) a.grad[off] += a.gscale * g;
+            rin[u] = wave * RW + u * RPW + rsub;
+            row[u] = slice * WAVE + r0 + rin[u];
+            ok[u] = row[u] < a.n;
+            const int64_t e = (ok[u] ? row[u] : a.n - 1) * VW + col;
+            l1[u] = lam1[e];
+            vo[u] = vj[e];
+            l2[u] = a.update ? nt_load(&a.lam2[e]) : zero(T{});
+            // the gradient slots this lane updates, loaded early
+#pragma unroll
+            for (int k4 = 0; k4 < KMAX / 4; ++k4) {
+                const int k = k4 * 4 + gk;
+                gold[u][k4] = (glane && ok[u] && k < kst) ? a.grad[base + (int64_t)k * WAVE + r0 + rin[u]] : 0.0;
             }
-            if (a.update && ok) {
-                T lnew = sub(acc, a.lam2[e]);
-                lnew = fmac((T)a.ca, a.v_j[e], lnew);
-                if (a.cb != 0.0) lnew = fmac((T)a.cb, a.v_next[e], lnew);
-                if (a.cc != 0.0) lnew = fmac((T)a.cc, a.v_prev[e], lnew);
-                a.lam_out[e] = lnew;
+        }
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int32_t c = s_idx[k * TR + rin[u]];
+                lg[u][k] = gather(lam1, c);
+                vg[u][k] = gather(vj, c);
+            }
+        if (tile + tw.stride < tw.end) prefetch(tile + tw.stride, next_base, next_width);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t e0 = base + r0 + rin[u];
+            // gradient of the staged entries of this row: (lam1 + cg v_j)[row] * v_j[col]
+            const T rf = fmac((T)a.cg, vo[u], l1[u]);
+#pragma unroll
+            for (int k4 = 0; k4 < KMAX; k4 += 4) {
+                const double r = reduce4<VW>((double)(rf * vg[u][k4]), (double)(rf * vg[u][k4 + 1]),
+                                             (double)(rf * vg[u][k4 + 2]), (double)(rf * vg[u][k4 + 3]), col);
+                const int k = k4 + gk;
+                if (glane && ok[u] && k < kst) a.grad[e0 + (int64_t)k * WAVE] = gold[u][k4 / 4] + a.gscale * r;
+            }
+            T acc_l = zero(T{}), acc_v = zero(T{});
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                const T av = s_val[k * TR + rin[u]];
+                acc_l = fmac(av, lg[u][k], acc_l);
+                acc_v = fmac(av, vg[u][k], acc_v);
+            }
+            // entries beyond KMAX (wide slices): straight from global memory
+            for (int k = KMAX; k < width; ++k) {
+                const int64_t off = e0 + (int64_t)k * WAVE;
+                const int32_t c = sidx[off];
+                const T av = sval[off];
+                const T xl = gather(lam1, c), xv = gather(vj, c);
+                acc_l = fmac(av, xl, acc_l);
+                acc_v = fmac(av, xv, acc_v);
+                double g = (double)(rf * xv);
+                for (int w = VW / 2; w >= 1; w >>= 1) g += __shfl_xor(g, w, WAVE);
+                if (col == 0 && ok[u]) a.grad[off] += a.gscale * g;
+            }
+            if (a.update && ok[u]) {
+                T lnew = sub(acc_l, l2[u]);
+                lnew = fmac((T)a.ca, vo[u], lnew);
+                lnew = fmac((T)a.chv, acc_v, lnew);
+                nt_store(lnew, &a.lam_out[row[u] * VW + col]);
             }
         }
     }
@@ -947,22 +1067,30 @@ static void adjoint_t(const AdjointLaunch& A, hipStream_t stream) {
     a.lam1 = static_cast<const T*>(A.lam1);
     a.lam2 = static_cast<const T*>(A.lam2);
     a.lam_out = static_cast<T*>(A.lam_out);
-    a.v_prev = static_cast<const T*>(A.v_prev);
     a.v_j = static_cast<const T*>(A.v_j);
-    a.v_next = static_cast<const T*>(A.v_next);
     a.ca = A.ca;
-    a.cb = A.cb;
-    a.cc = A.cc;
+    a.chv = A.chv;
+    a.cg = A.cg;
     a.gscale = A.gscale;
     a.update = A.update;
     a.grad = A.grad;
     a.n = A.n;
-    a.num_tiles = (A.n + WAVE - 1) / WAVE;
-    int64_t g = a.num_tiles < (int64_t)A.num_cu * 8 ? a.num_tiles : (int64_t)A.num_cu * 8;
-    if (g >= 8) g -= g % 8;
-    if (g < 1) g = 1;
-    hipLaunchKernelGGL((k_adjoint_sellb<T, VW, 8>), dim3((unsigned)g), dim3(BLOCK), 0, stream, a, A.slice_ptr,
-                       A.slice_width);
+    // rows per wave per tile: 8 (two or more row groups in flight) up to 16 vectors, 4 above
+    constexpr int RW = (WAVE / VW) > 8 ? (WAVE / VW) : (VW <= 16 ? 8 : 4);
+    constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
+    a.num_tiles = (A.n + tr - 1) / tr;
+    const bool off32 = (uint64_t)A.n * VW * sizeof(T) < (1ull << 32);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(resident_grid(kernel, A.num_cu, a.num_tiles)), dim3(BLOCK), 0, stream, a,
+                           A.slice_ptr, A.slice_width);
+    };
+    if (A.max_width <= 4) {
+        if (off32) go(k_adjoint_sellb<T, VW, 4, RW, true>);
+        else go(k_adjoint_sellb<T, VW, 4, RW, false>);
+    } else {
+        if (off32) go(k_adjoint_sellb<T, VW, 8, RW, true>);
+        else go(k_adjoint_sellb<T, VW, 8, RW, false>);
+    }
 }
 
 template <class T>
