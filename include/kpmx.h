@@ -52,6 +52,7 @@ extern "C" {
 
 #define KPMX_KUBO_FP64 0        /* mu_nm contraction in FP64 (reference arithmetic)             */
 #define KPMX_KUBO_FP32_SPLIT 1  /* FP32 MFMA with hi/lo split-precision compensation            */
+#define KPMX_KUBO_BF16X3 2      /* bf16 MFMA on three-piece operand splits (fp32-accurate)      */
 
 typedef struct kpmx_ctx kpmx_ctx;
 typedef struct kpmx_mat kpmx_mat;
@@ -129,7 +130,7 @@ int kpmx_moments_dense(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_mom
  * cppcore/src/kpm/Moments.cpp:92-101):
  *   mu_nm = (1/R) sum_r sum_i L_r[n,i] conj(R_r[m,i]),
  *   L_r rows = T_n(H~) (vl r),  R_r rows = vr T_m(H~) r,  r = the R vectors of `r0`.
- * mode: KPMX_KUBO_FP64 or KPMX_KUBO_FP32_SPLIT.  mu_nm: host, M x M, scalar type of h2. */
+ * mode: KPMX_KUBO_FP64, KPMX_KUBO_FP32_SPLIT or KPMX_KUBO_BF16X3.  mu_nm: host, M x M, scalar type of h2. */
 int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr, const kpmx_block* r0,
                       int64_t num_moments, int mode, void* mu_nm);
 

@@ -254,7 +254,7 @@ def kpm(model, energy_range=None, kernel="default", num_threads="auto", silent=F
         Don't show any progress messages.
     **kwargs
         matrix_format ("ELL" or "CSR"), optimal_size, interleaved, lanczos_precision,
-        device (GPU index), kubo_precision ("fp64" or "fp32_split"),
+        device (GPU index), kubo_precision ("fp64", "fp32_split" or "bf16x3"),
         site_order ("auto": a Hilbert-curve order of the model's site positions for whole-system
         calculations, "bfs": the reference's BFS reorder only, or an explicit permutation).
     """

@@ -173,7 +173,9 @@ PYBIND11_MODULE(_kpmx, m) {
             }
             int kubo_mode = KPMX_KUBO_FP64;
             if (kubo_precision == "fp32_split") kubo_mode = KPMX_KUBO_FP32_SPLIT;
-            else if (kubo_precision != "fp64") throw std::invalid_argument("kubo_precision must be 'fp64' or 'fp32_split'");
+            else if (kubo_precision == "bf16x3") kubo_mode = KPMX_KUBO_BF16X3;
+            else if (kubo_precision != "fp64")
+                throw std::invalid_argument("kubo_precision must be 'fp64', 'fp32_split' or 'bf16x3'");
             auto dt = py::dtype::from_args(data.attr("dtype"));
             const char kind = dt.kind();
             const auto itemsize = dt.itemsize();

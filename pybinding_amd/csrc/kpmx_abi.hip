@@ -875,7 +875,8 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
     return guarded([&] {
         check_pair(h2, r0);
         KPMX_REQUIRE(vl && vr && mu_nm, "null argument");
-        KPMX_REQUIRE(mode == KPMX_KUBO_FP64 || mode == KPMX_KUBO_FP32_SPLIT, "invalid contraction mode");
+        KPMX_REQUIRE(mode == KPMX_KUBO_FP64 || mode == KPMX_KUBO_FP32_SPLIT || mode == KPMX_KUBO_BF16X3,
+                     "invalid contraction mode");
         for (const kpmx_mat* v : {vl, vr})
             KPMX_REQUIRE(v->ctx == h2->ctx && v->scalar == h2->scalar && v->rows == h2->rows,
                          "velocity operator does not match the Hamiltonian");
@@ -964,11 +965,11 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
                         launch(O, ctx->stream);
                     });
                 // mu_nm += sum_c L_c R_c^H
-                const int64_t wd = gemm_workspace_doubles(M, n, sc);
+                const int64_t wd = gemm_workspace_doubles(M, n, sc, mode, ctx->num_cu);
                 double* work = static_cast<double*>(ctx->partials.get((size_t)wd * sizeof(double)));
                 for (int64_t c = 0; c < cols; ++c) {
                     if (launch_gemm_nc(sc, mode, Lpan + (size_t)c * M * n * sb, n, Rpan + (size_t)c * M * n * sb, n, M,
-                                       n, C, work, ctx->stream) != 0)
+                                       n, C, work, ctx->num_cu, ctx->stream) != 0)
                         throw Error{KPMX_EINVAL, "unsupported contraction"};
                     KPMX_HIP(hipGetLastError());
                 }

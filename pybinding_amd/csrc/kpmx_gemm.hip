@@ -10,8 +10,12 @@
 //                  formed with v_mfma_f32_32x32x2_f32; the float accumulators are flushed into
 //                  double every 32-deep k panel, so the rounding of the float sums stays bounded
 //                  by one panel's depth.
-// Both split k into S slices written to a [S][M][M] workspace and summed in slice order, so the
+//   mode 2 (bf16x3 split MFMA): three bf16 pieces per operand value, six piece products per real
+//                  product on v_mfma_f32_32x32x16_bf16 (fp32-accurate), see below.
+// All split k into S slices written to a [S][M][M] workspace and summed in slice order, so the
 // result is deterministic.
+#include <algorithm>
+
 #include "kpmx_device.hpp"
 #include "kpmx_internal.hpp"
 
@@ -207,6 +211,202 @@ __global__ __launch_bounds__(256) void k_gemm_nc_split(const T* __restrict__ A, 
     }
 }
 
+// ------------------------------------------------------------------- bf16x3 split-precision MFMA
+// Every operand value x (fp64 or fp32) is split into three bf16 pieces x = x0 + x1 + x2 + O(2^-24 x)
+// (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1), the remainders exact in the input
+// precision), and a product x y is formed from the six piece products with i + j <= 2,
+// accumulated by v_mfma_f32_32x32x16_bf16 in fp32 and flushed into fp64 every 64 k, giving an
+// fp32-accurate contraction on the bf16 matrix cores.  Complex products use four real products
+// (Cr += Lr Rr + Li Ri, Ci += Li Rr - Lr Ri), i.e. 24 MFMAs per 32x32x16 complex block.
+//
+// Work decomposition (persistent, XCD-aware): a workgroup of 8 waves computes one 128x64 output
+// tile over one K range; each wave owns a 32x32 block and keeps four independent fp32
+// accumulator chains (Lr Rr, Li Ri, Li Rr, -Lr Ri) plus the fp64 sums.  The T output tiles of one K
+// range run on the same XCD (workgroup b sits on XCD b % 8), so the K range's L and R rows are
+// read from HBM once and served to the T workgroups from that XCD's L2.  Partial tiles go to
+// [KG][M][M] and are summed in K-range order (deterministic).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int XTN = 128;         // output tile rows (n)
+constexpr int XTM = 64;          // output tile columns (m)
+constexpr int XK = 32;           // k per LDS stage (two MFMA k16 steps)
+constexpr int XS = XK + 8;       // LDS row stride in bf16 (pads the 16-byte lane reads off bank conflicts)
+constexpr int XTHREADS = 512;    // 8 waves: 4 x 2 blocks of 32 x 32
+
+__device__ __forceinline__ uint32_t bf16_bits_rn(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;   // round to nearest even (finite inputs)
+}
+__device__ __forceinline__ float bf16_value(uint32_t h) { return __uint_as_float(h << 16); }
+
+// three bf16 pieces of v, as 16-bit patterns
+template <class R>
+__device__ __forceinline__ void split3(R v, uint16_t& p0, uint16_t& p1, uint16_t& p2) {
+    const uint32_t h0 = bf16_bits_rn((float)v);
+    const R r1 = v - (R)bf16_value(h0);
+    const uint32_t h1 = bf16_bits_rn((float)r1);
+    const R r2 = r1 - (R)bf16_value(h1);
+    const uint32_t h2 = bf16_bits_rn((float)r2);
+    p0 = (uint16_t)h0;
+    p1 = (uint16_t)h1;
+    p2 = (uint16_t)h2;
+}
+
+template <class T> struct parts_of { using R = T; static constexpr int N = 1; };
+template <class R_> struct parts_of<cplx<R_>> { using R = R_; static constexpr int N = 2; };
+template <class R> __device__ __forceinline__ R part(R v, int) { return v; }
+template <class R> __device__ __forceinline__ R part(cplx<R> v, int p) { return p == 0 ? v.re : v.im; }
+
+__device__ __forceinline__ bf16x8 negate(bf16x8 v) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+template <class T>
+__global__ __launch_bounds__(XTHREADS) void k_gemm_nc_bf16x3(const T* __restrict__ A, int64_t lda,
+                                                             const T* __restrict__ B, int64_t ldb, int64_t M,
+                                                             int64_t kdim, int64_t kslice, int groups_per_xcd,
+                                                             int64_t tiles_per_wg, double* __restrict__ P) {
+    constexpr int NP = parts_of<T>::N;   // real parts per value (1 real, 2 complex)
+    using R = typename parts_of<T>::R;
+    __shared__ __align__(16) uint16_t As[NP][3][XTN][XS];
+    __shared__ __align__(16) uint16_t Bs[NP][3][XTM][XS];
+    const int t = threadIdx.x;
+    const int lane = t % 64, wave = t / 64;
+    const int br = wave / 2, bc = wave % 2;      // this wave's 32x32 block inside the tile
+    const int64_t tiles_n = (M + XTN - 1) / XTN, tiles_m = (M + XTM - 1) / XTM;
+    const int64_t T2 = tiles_n * tiles_m;
+    const int xcd = blockIdx.x % 8;
+    const int slot = blockIdx.x / 8;
+    const int slots = gridDim.x / 8;
+    int64_t kg, tile0, tile_step;
+    if (T2 <= slots) {
+        if (slot >= groups_per_xcd * T2) return;   // idle workgroup (the whole block exits before any barrier)
+        kg = (int64_t)xcd * groups_per_xcd + slot / T2;
+        tile0 = slot % T2;
+        tile_step = T2;
+    } else {
+        kg = xcd;
+        tile0 = slot;
+        tile_step = slots;
+    }
+    const int64_t kbeg = kg * kslice;
+    const int64_t kend = (kbeg + kslice < kdim) ? kbeg + kslice : kdim;
+    // rows x 32 k of src into dst[NP][3][rows][XS] as bf16 pieces, two consecutive k per item
+    auto stage_rows = [&](const T* __restrict__ src, int64_t ld, int64_t r0, int rows, int64_t k0, uint16_t* dst) {
+        for (int e = t; e < rows * (XK / 2); e += XTHREADS) {
+            const int r = e / (XK / 2), kk = 2 * (e % (XK / 2));
+            const int64_t k = k0 + kk;
+            T v[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) v[h] = (k + h < kend && r0 + r < M) ? src[(r0 + r) * ld + k + h] : zero(T{});
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                uint16_t x[3][2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) split3<R>(part(v[h], p), x[0][h], x[1][h], x[2][h]);
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    *reinterpret_cast<uint32_t*>(dst + ((p * 3 + j) * rows + r) * XS + kk) =
+                        (uint32_t)x[j][0] | ((uint32_t)x[j][1] << 16);
+            }
+        }
+    };
+    for (int64_t tile = tile0, it = 0; tile < T2 && it < tiles_per_wg; tile += tile_step, ++it) {
+        const int64_t n0 = (tile / tiles_m) * XTN, m0 = (tile % tiles_m) * XTM;
+        double dre[16], dim_[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) dre[i] = dim_[i] = 0.0;
+        f32x16 c0, c1, c2, c3;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) c0[i] = c1[i] = c2[i] = c3[i] = 0.f;
+        int stage = 0;
+        for (int64_t k0 = kbeg; k0 < kend; k0 += XK, ++stage) {
+            __syncthreads();   // the previous stage's LDS operands are no longer read
+            stage_rows(A, lda, n0, XTN, k0, &As[0][0][0][0]);
+            stage_rows(B, ldb, m0, XTM, k0, &Bs[0][0][0][0]);
+            __syncthreads();
+            const int ar = br * 32 + (lane % 32);
+            const int bcl = bc * 32 + (lane % 32);
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int ko = s * 16 + (lane / 32) * 8;
+                bf16x8 a[NP][3], b[NP][3];
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        a[p][j] = *reinterpret_cast<const bf16x8*>(&As[p][j][ar][ko]);
+                        b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[p][j][bcl][ko]);
+                    }
+                // piece pairs (i, j) with i + j <= 2, smallest first
+#define KPMX_PAIRS(acc, X, Y)                                                        \
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[2], Y[0], acc, 0, 0, 0);         \
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[1], Y[1], acc, 0, 0, 0);         \
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[0], Y[2], acc, 0, 0, 0);         \
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[1], Y[0], acc, 0, 0, 0);         \
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[0], Y[1], acc, 0, 0, 0);         \
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[0], Y[0], acc, 0, 0, 0);
+                KPMX_PAIRS(c0, a[0], b[0]);
+                if constexpr (NP == 2) {
+                    bf16x8 nbi[3] = {negate(b[1][0]), negate(b[1][1]), negate(b[1][2])};
+                    KPMX_PAIRS(c1, a[1], b[1]);
+                    KPMX_PAIRS(c2, a[1], b[0]);
+                    KPMX_PAIRS(c3, a[0], nbi);
+                }
+#undef KPMX_PAIRS
+            }
+            if ((stage & 1) == 1 || k0 + XK >= kend) {   // flush the fp32 sums every 64 k
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    dre[i] += (double)c0[i] + (double)c1[i];
+                    dim_[i] += (double)c2[i] + (double)c3[i];
+                    c0[i] = c1[i] = c2[i] = c3[i] = 0.f;
+                }
+            }
+        }
+        double* out = P + kg * M * M * NP;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int64_t n = n0 + br * 32 + 8 * (i / 4) + 4 * (lane / 32) + (i % 4);
+            const int64_t m = m0 + bc * 32 + (lane % 32);
+            if (n < M && m < M) {
+                if (NP == 2) {
+                    out[(n * M + m) * 2] = dre[i];
+                    out[(n * M + m) * 2 + 1] = dim_[i];
+                } else {
+                    out[n * M + m] = dre[i];
+                }
+            }
+        }
+    }
+}
+
+struct X3Plan {
+    int64_t kg_total, kslice, tiles_per_wg;
+    int groups_per_xcd, grid;
+};
+
+X3Plan x3_plan(int64_t M, int64_t kdim, int num_cu) {
+    X3Plan p;
+    const int64_t T2 = ((M + XTN - 1) / XTN) * ((M + XTM - 1) / XTM);
+    p.grid = std::max(8, (num_cu / 8) * 8);
+    const int slots = p.grid / 8;
+    if (T2 <= slots) {
+        p.groups_per_xcd = (int)(slots / T2);
+        p.kg_total = 8 * (int64_t)p.groups_per_xcd;
+        p.tiles_per_wg = 1;
+    } else {
+        p.groups_per_xcd = 1;
+        p.kg_total = 8;
+        p.tiles_per_wg = (T2 + slots - 1) / slots;
+    }
+    int64_t ks = (kdim + p.kg_total - 1) / p.kg_total;
+    p.kslice = std::max<int64_t>(XK, (ks + XK - 1) / XK * XK);
+    return p;
+}
+
 __global__ void k_gemm_reduce(const double* P, int64_t slices, int64_t count, double* C) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
         double s = C[e];
@@ -227,39 +427,49 @@ int64_t num_slices(int64_t M, int64_t kdim) {
 
 }  // namespace
 
-int64_t gemm_workspace_doubles(int64_t M, int64_t kdim, int scalar) {
-    return num_slices(M, kdim) * M * M * (scalar >= 2 ? 2 : 1);
+int64_t gemm_workspace_doubles(int64_t M, int64_t kdim, int scalar, int mode, int num_cu) {
+    const int64_t S = mode == 2 ? x3_plan(M, kdim, num_cu).kg_total : num_slices(M, kdim);
+    return S * M * M * (scalar >= 2 ? 2 : 1);
 }
 
 template <class T>
 static void gemm_t(int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M, int64_t kdim, double* C,
-                   double* work, hipStream_t s) {
-    const int64_t S = num_slices(M, kdim);
-    int64_t kslice = (kdim + S - 1) / S;
-    const int64_t align = mode == 1 ? KP : TK;
-    kslice = (kslice + align - 1) / align * align;
-    const int64_t tiles = (M + TM - 1) / TM;
-    const dim3 grid((unsigned)(tiles * tiles * S)), block(256);
-    if (mode == 1)
-        hipLaunchKernelGGL(k_gemm_nc_split<T>, grid, block, 0, s, static_cast<const T*>(A), lda, static_cast<const T*>(B),
-                           ldb, M, kdim, kslice, work);
-    else
-        hipLaunchKernelGGL(k_gemm_nc_fp64<T>, grid, block, 0, s, static_cast<const T*>(A), lda, static_cast<const T*>(B),
-                           ldb, M, kdim, kslice, work);
+                   double* work, int num_cu, hipStream_t s) {
     const int64_t count = M * M * (traits<T>::is_complex ? 2 : 1);
+    int64_t S;
+    if (mode == 2) {
+        const X3Plan p = x3_plan(M, kdim, num_cu);
+        S = p.kg_total;
+        hipLaunchKernelGGL(k_gemm_nc_bf16x3<T>, dim3((unsigned)p.grid), dim3(XTHREADS), 0, s, static_cast<const T*>(A),
+                           lda, static_cast<const T*>(B), ldb, M, kdim, p.kslice, p.groups_per_xcd, p.tiles_per_wg,
+                           work);
+    } else {
+        S = num_slices(M, kdim);
+        int64_t kslice = (kdim + S - 1) / S;
+        const int64_t align = mode == 1 ? KP : TK;
+        kslice = (kslice + align - 1) / align * align;
+        const int64_t tiles = (M + TM - 1) / TM;
+        const dim3 grid((unsigned)(tiles * tiles * S)), block(256);
+        if (mode == 1)
+            hipLaunchKernelGGL(k_gemm_nc_split<T>, grid, block, 0, s, static_cast<const T*>(A), lda,
+                               static_cast<const T*>(B), ldb, M, kdim, kslice, work);
+        else
+            hipLaunchKernelGGL(k_gemm_nc_fp64<T>, grid, block, 0, s, static_cast<const T*>(A), lda,
+                               static_cast<const T*>(B), ldb, M, kdim, kslice, work);
+    }
     int64_t g = (count + 255) / 256;
     if (g > 65536) g = 65536;
     hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)g), dim3(256), 0, s, work, S, count, C);
 }
 
 int launch_gemm_nc(int scalar, int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M, int64_t kdim,
-                   double* C, double* work, hipStream_t s) {
-    if (mode != 0 && mode != 1) return -1;
+                   double* C, double* work, int num_cu, hipStream_t s) {
+    if (mode < 0 || mode > 2) return -1;
     switch (scalar) {
-        case 0: gemm_t<float>(mode, A, lda, B, ldb, M, kdim, C, work, s); break;
-        case 1: gemm_t<double>(mode, A, lda, B, ldb, M, kdim, C, work, s); break;
-        case 2: gemm_t<cf32>(mode, A, lda, B, ldb, M, kdim, C, work, s); break;
-        case 3: gemm_t<cf64>(mode, A, lda, B, ldb, M, kdim, C, work, s); break;
+        case 0: gemm_t<float>(mode, A, lda, B, ldb, M, kdim, C, work, num_cu, s); break;
+        case 1: gemm_t<double>(mode, A, lda, B, ldb, M, kdim, C, work, num_cu, s); break;
+        case 2: gemm_t<cf32>(mode, A, lda, B, ldb, M, kdim, C, work, num_cu, s); break;
+        case 3: gemm_t<cf64>(mode, A, lda, B, ldb, M, kdim, C, work, num_cu, s); break;
         default: return -1;
     }
     return 0;

@@ -90,10 +90,10 @@ void launch_scatter_scaled(int scalar, const void* src, int64_t n, int vw, int64
                            int64_t col_stride, hipStream_t s);
 // C (M x M, {re, im} doubles, or doubles for real scalars) += sum_k A[n][k] conj(B[m][k]) over
 // k < kdim, A and B row-major with leading dimensions lda / ldb (elements).  Deterministic
-// split-K: `work` must hold gemm_workspace_doubles(M, kdim, scalar) doubles.
-int64_t gemm_workspace_doubles(int64_t M, int64_t kdim, int scalar);
+// split-K: `work` must hold gemm_workspace_doubles(M, kdim, scalar, mode, num_cu) doubles.
+int64_t gemm_workspace_doubles(int64_t M, int64_t kdim, int scalar, int mode, int num_cu);
 int launch_gemm_nc(int scalar, int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M,
-                   int64_t kdim, double* C, double* work, hipStream_t s);
+                   int64_t kdim, double* C, double* work, int num_cu, hipStream_t s);
 void launch_zero(int scalar, void* dst, int64_t count, hipStream_t s);
 void launch_zero_f64(double* dst, int64_t count, hipStream_t s);
 void launch_csr_matvec(int scalar, const int32_t* indptr, const int32_t* indices, const void* vals, const void* x,

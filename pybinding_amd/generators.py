@@ -181,10 +181,11 @@ def peierls_honeycomb_supercell(n1, n2, field, disorder=0.0, seed=5, t=T_GRAPHEN
     return _assemble(2 * ncell, rows, cols, vals, _anderson(2 * ncell, disorder, seed), np.complex128), x, y
 
 
-def strained_graphene_flake(width, height, bump_height=1.0, bump_sigma=5.0):
+def strained_graphene_flake(width, height, bump_height=1.0, bump_sigma=5.0, return_positions=False):
     """Graphene flake with a Gaussian bump z = h exp(-r^2 / sigma^2) and strained hoppings
     t = -2.8 exp(-3.37 (d / a_cc - 1)) (pybinding/repository/graphene/modifiers.py:76-82,
-    118-137).  Returns (num_sites, bonds (E, 2), hoppings (E,), onsite zeros (N,))."""
+    118-137).  Returns (num_sites, bonds (E, 2), hoppings (E,), onsite zeros (N,)) and, with
+    `return_positions`, the (N, 2) in-plane positions."""
     h, pos = honeycomb_flake(width, height, return_positions=True)
     coo = sp.triu(h, k=1).tocoo()
     bonds = np.stack([coo.row, coo.col], axis=1)
@@ -192,4 +193,6 @@ def strained_graphene_flake(width, height, bump_height=1.0, bump_sigma=5.0):
     d = np.sqrt((pos[bonds[:, 0], 0] - pos[bonds[:, 1], 0]) ** 2 + (pos[bonds[:, 0], 1] - pos[bonds[:, 1], 1]) ** 2
                 + (z[bonds[:, 0]] - z[bonds[:, 1]]) ** 2)
     hop = T_GRAPHENE * np.exp(-3.37 * (d / A_CC - 1))
+    if return_positions:
+        return h.shape[0], bonds, hop, np.zeros(h.shape[0]), pos[:, :2]
     return h.shape[0], bonds, hop, np.zeros(h.shape[0])

@@ -99,3 +99,30 @@ def test_checkpointed_equals_stored(oracle):
     blk.close()
     mat.close()
     ctx.close()
+
+
+def test_site_order_does_not_change_gradient():
+    """TightBinding(positions=...) runs in a Hilbert order of the sites; the moments and the
+    gradients per original site and bond are the same as in the given order."""
+    torch = pytest.importorskip("torch")
+    from pybinding_amd.autograd import TightBinding, kpm_moments
+    h, pos = generators.honeycomb_flake(5, 4, return_positions=True)
+    coo = sp.triu(h, k=1).tocoo()
+    bonds = np.stack([coo.row, coo.col], axis=1)
+    rng = np.random.default_rng(8)
+    hop = -2.8 * np.exp(-3.37 * rng.uniform(-0.05, 0.05, len(bonds)))
+    onsite = rng.uniform(-0.3, 0.3, h.shape[0])
+    M, R = 42, 4
+    w = torch.from_numpy(np.random.default_rng(9).standard_normal(M))
+    out = []
+    for positions in (None, (pos[:, 0], pos[:, 1])):
+        tb = TightBinding(h.shape[0], bonds, (-9.0, 9.0), M, R, checkpoint=8, positions=positions)
+        on_t = torch.tensor(onsite, requires_grad=True)
+        hop_t = torch.tensor(hop, requires_grad=True)
+        mu = kpm_moments(tb, on_t, hop_t)
+        (mu * w).sum().backward()
+        out.append((mu.detach().numpy(), on_t.grad.numpy().copy(), hop_t.grad.numpy().copy()))
+        tb.close()
+    np.testing.assert_allclose(out[1][0], out[0][0], rtol=0, atol=1e-12 * abs(out[0][0][0]))
+    for k in (1, 2):
+        np.testing.assert_allclose(out[1][k], out[0][k], rtol=1e-10, atol=1e-12 * np.abs(out[0][k]).max())

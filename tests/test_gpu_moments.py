@@ -155,12 +155,14 @@ def test_dense_vs_oracle(ctx, oracle, dtype, with_op):
         opm.close()
 
 
-@pytest.mark.parametrize("dtype", [np.complex128, np.float64])
-def test_kubo_contraction_vs_fp64_gemm(ctx, oracle, dtype):
-    """mu_nm = (1/R) sum_r L_r R_r^H against an FP64 numpy GEMM of oracle panels; the FP32
-    split-precision MFMA contraction is reported with its error against the same oracle."""
+@pytest.mark.parametrize("M", [64, 70, 200])
+@pytest.mark.parametrize("dtype", [np.complex128, np.float64, np.complex64, np.float32])
+def test_kubo_contraction_vs_fp64_gemm(ctx, oracle, dtype, M):
+    """mu_nm = (1/R) sum_r L_r R_r^H against an FP64 numpy GEMM of oracle panels; the FP32 split
+    and bf16x3 split MFMA contractions are checked against the same oracle at fp32 accuracy.
+    M = 70 and 200 leave partial output tiles; n = 900 leaves partial k panels."""
     dtype = np.dtype(dtype)
-    n, M, R = 900, 64, 3
+    n, R = 900, 3
     h2 = scaled_random(n, dtype, seed=9)
     x = np.random.default_rng(10).uniform(-5, 5, n).astype(np.float32)
     y = np.random.default_rng(11).uniform(-5, 5, n).astype(np.float32)
@@ -175,16 +177,16 @@ def test_kubo_contraction_vs_fp64_gemm(ctx, oracle, dtype):
     ref /= R
     mat, ml, mr = ctx.mat(h2), ctx.mat(vl), ctx.mat(vr)
     blk = ctx.block(r0)
-    mu64 = abi.kubo_moments(mat, ml, mr, blk, M, mode=0)
-    mu32 = abi.kubo_moments(mat, ml, mr, blk, M, mode=1)
+    mus = {mode: abi.kubo_moments(mat, ml, mr, blk, M, mode=mode) for mode in (0, 1, 2)}
     for h in (blk, mat, ml, mr):
         h.close()
     scale = np.abs(ref).max()
-    err64 = np.abs(mu64 - ref).max() / scale
-    err32 = np.abs(mu32 - ref).max() / scale
-    print("kubo fp64 err %.3e  fp32-split err %.3e" % (err64, err32))
-    assert err64 <= 1e-11
-    assert err32 <= 1e-5
+    err = {mode: np.abs(mu - ref).max() / scale for mode, mu in mus.items()}
+    print("kubo M=%d %s: fp64 err %.3e  fp32-split err %.3e  bf16x3 err %.3e" % (M, dtype, err[0], err[1], err[2]))
+    double = dtype in (np.float64, np.complex128)
+    assert err[0] <= (1e-11 if double else 1e-5)
+    assert err[1] <= 1e-5
+    assert err[2] <= 1e-5
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
