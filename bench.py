@@ -83,11 +83,11 @@ def cpu_baseline(cells, dtype, nvec, target_seconds=15.0):
     threads = max(1, min(16, os.cpu_count() or 1))
     ell = oracle.to_ell(h2)
     n = h2.shape[0]
-    # calibrate on 10 and 22 moments (interleaved needs (M - 2) % 4 == 0): the difference is the
+    # calibrate on 10 and 106 moments (interleaved needs (M - 2) % 4 == 0): the difference is the
     # per-moment cost, the rest the fixed cost of drawing the starters
     _, t10 = oracle.cpu_baseline_dos(ell, n, 10, nvec, threads, perm=rmap, borders=borders, interleaved=True)
-    _, t22 = oracle.cpu_baseline_dos(ell, n, 22, nvec, threads, perm=rmap, borders=borders, interleaved=True)
-    per_moment = max((t22 - t10) / 12, 1e-9)
+    _, t106 = oracle.cpu_baseline_dos(ell, n, 106, nvec, threads, perm=rmap, borders=borders, interleaved=True)
+    per_moment = max((t106 - t10) / 96, t106 / 106 / 4, 1e-9)
     fixed = max(t10 - 10 * per_moment, 0.0)
     m = int((target_seconds - fixed) / per_moment)
     m = max(10, min(4098, m - (m - 2) % 4))
@@ -233,6 +233,86 @@ def kubo_contraction(n1=724, n2=724, M=512, R=16):
     return out
 
 
+def plain_dos_flake(M=2050, R=8, reps=5):
+    """The reference's plain DOS case: graphene 40 x 40 nm flake (the model builder, reference site
+    order), fp64, energy range (-8.6, 8.6), 2050 moments (2048 requested, rounded as the
+    reference does), 8 reference random vectors, BFS order.  Device time of the moments through
+    the C-ABI (inputs resident), the restated reference CPU compute (interleaved, all threads) on the
+    same H2 and vectors, and the largest difference of the two mean moment sets."""
+    import oracle
+    from pybinding_amd import abi, lattice as L
+    from pybinding_amd._native import load_module
+    from pybinding_amd.model import Model
+    from pybinding_amd.repository import graphene
+    import scipy.sparse as sp
+    native = load_module()
+    m = Model(graphene.monolayer(), L.rectangle(40, 40), L.force_double_precision())
+    h = m.hamiltonian
+    ip, ix, dv, rmap, borders, _, _, _ = native.optimize(h.shape, h.indptr, h.indices, h.data, (-8.6, 8.6), True,
+                                                         [0], [0])
+    h2 = sp.csr_matrix((dv, ix, ip), shape=h.shape)
+    n = h.shape[0]
+    r0 = native.random_starters(n, R, "float64", rmap)
+    ctx = abi.Ctx(0)
+    mat = ctx.mat(h2, fmt=1)
+    blk = ctx.block(r0)
+    abi.moments_diagonal_run(mat, blk, M)
+    ctx.synchronize()
+    ctx.timer_start()
+    for _ in range(reps):
+        abi.moments_diagonal_run(mat, blk, M)
+    gpu_ms = ctx.timer_stop() / reps
+    mu_gpu = abi.moments_diagonal_fetch(mat, blk, M).mean(axis=0)
+    blk.close()
+    mat.close()
+    ctx.close()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    mu_cpu, t_cpu = oracle.cpu_baseline_dos(oracle.to_ell(h2), n, M, R, threads, perm=rmap, borders=borders,
+                                            interleaved=True)
+    err = float(np.abs(mu_gpu - mu_cpu).max() / abs(mu_cpu[0]))
+    out = {"sites": n, "nnz": int(h2.nnz), "moments": M, "vectors": R,
+           "gpu_ms": gpu_ms, "gpu_value": h2.nnz * M * R / (gpu_ms / 1000) / 1e9, "gpu_us_per_step": gpu_ms * 1000 / (M // 2),
+           "cpu_ms": t_cpu * 1000, "cpu_value": h2.nnz * M * R / t_cpu / 1e9, "cpu_threads": threads,
+           "max_rel_diff_vs_cpu": err, "unit": "GNNZ*moments/s",
+           "workload": "graphene 40x40 nm flake (%d sites), fp64, 2050 moments, 8 vectors" % n}
+    log("plain DOS flake: GPU %.2f ms (%.1f us/step), CPU %.1f ms on %d threads, max diff %.2e"
+        % (gpu_ms, out["gpu_us_per_step"], t_cpu * 1000, threads, err))
+    return out
+
+
+def fp32_variant(cells, M, R, fmt, order):
+    """The same block-SpMM workload in fp32 (the reference's default precision for models without
+    force_double_precision): one timed moment computation after autotuning."""
+    from pybinding_amd import abi
+    dtype = np.dtype(np.float32)
+    h2, rmap, borders, native = build_workload(cells[0], cells[1], 0.5, dtype, order)
+    n, nnz = h2.shape[0], h2.nnz
+    r0 = starters(native, n, R, dtype, rmap)
+    ctx = abi.Ctx(0)
+    mat = ctx.mat(h2, fmt=1 if fmt == "SELL" else 0)
+    blk = ctx.block(r0)
+    ctx.synchronize()
+    variant = abi.autotune_diagonal(mat, blk)
+    abi.moments_diagonal_run(mat, blk, M)
+    ctx.synchronize()
+    ctx.timer_start()
+    abi.moments_diagonal_run(mat, blk, M)
+    ms = ctx.timer_stop()
+    mu = abi.moments_diagonal_fetch(mat, blk, M)
+    assert np.all(np.isfinite(mu)), "non-finite moments"
+    blk.close()
+    mat.close()
+    ctx.close()
+    step_ms = ms / (M // 2)
+    vw = min(64, 1 << (max(R, 1) - 1).bit_length())
+    out = {"value": nnz * M * R / (ms / 1000) / 1e9, "unit": "GNNZ*moments/s", "ms_per_computation": ms,
+           "step_ms": step_ms, "achieved_GBps": step_bytes(n, nnz, vw, 4) / (step_ms / 1000) / 1e9,
+           "kernel_variant": variant}
+    log("fp32 variant: %.1f ms per moment computation, %.4f ms per step, %.0f GB/s algorithmic"
+        % (ms, step_ms, out["achieved_GBps"]))
+    return out
+
+
 def counted_traffic(key):
     """HBM bytes per step launch measured by rocprofv3 counters (FETCH_SIZE and WRITE_SIZE, each
     scaled by its copy-kernel calibration factor) for this exact workload, read from
@@ -260,6 +340,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the single-vector (plain) comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-realizations run")
+    ap.add_argument("--no-flake", action="store_true", help="skip the plain DOS run on the 40 nm graphene flake")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the fp32 variant of the block-SpMM workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient cost run")
     ap.add_argument("--no-kubo", action="store_true", help="skip the Kubo contraction run")
     ap.add_argument("--copy-probe", action="store_true",
@@ -350,6 +432,10 @@ def main():
         blk1.close()
         mat1.close()
 
+    if not args.no_fp32 and dtype == np.float64:
+        extra["fp32_variant"] = fp32_variant(args.cells, M, R, args.format, args.order)
+    if not args.no_flake:
+        extra["plain_dos_flake"] = plain_dos_flake()
     if not args.no_batched:
         extra["batched_realizations"] = batched_realizations()
     if not args.no_adjoint:
