@@ -355,7 +355,7 @@ constexpr int sellb_rw(int tune) {
 // system in one stretch; the diagonal-moment sums are flushed to a new partial slot whenever the
 // system changes (slot numbering precomputed on the host: slot_begin[b] is the first slot of
 // workgroup b), and k_reduce_segments sums the slots of every system in order.
-template <class T, int VW, int KMAX>
+template <class T, int VW, int KMAX, bool OFF32>
 __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
                                                         const int32_t* __restrict__ slice_width,
                                                         const int32_t* __restrict__ slice_sys,
@@ -380,6 +380,39 @@ __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int
     int64_t slot = slot_begin[blockIdx.x];
     int cur = slice_sys[t0];
     double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
+    // software pipeline: the next slice's indices and values are loaded into registers while the
+    // current slice's gathers are in flight (as in k_step_sellb)
+    constexpr int E = (KMAX * TR + BLOCK - 1) / BLOCK;
+    int32_t p_idx[E];
+    T p_val[E];
+    auto prefetch = [&](int64_t slc, int64_t& pbase, int& pwidth) {
+        pbase = slice_ptr[slc];
+        pwidth = slice_width[slc];
+        const int kk = pwidth < KMAX ? pwidth : KMAX;
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * BLOCK;
+            const int k = e / TR, r = e % TR;
+            if (e < KMAX * TR && k < kk) {
+                p_idx[q] = sidx[pbase + (int64_t)k * WAVE + r];
+                p_val[q] = sval[pbase + (int64_t)k * WAVE + r];
+            } else {
+                p_idx[q] = (int32_t)(slc * WAVE + r);
+                p_val[q] = zero(T{});
+            }
+        }
+    };
+    auto gather = [&](int32_t c) -> T {
+        if constexpr (OFF32) {
+            const uint32_t off = ((uint32_t)c * VW + col) * (uint32_t)sizeof(T);
+            return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(x) + off);
+        } else {
+            return x[(int64_t)c * VW + col];
+        }
+    };
+    int64_t next_base = 0;
+    int next_width = 0;
+    prefetch(t0, next_base, next_width);
     for (int64_t slice = t0; slice < t1; ++slice) {
         const int sys = slice_sys[slice];
         if (sys != cur) {
@@ -388,18 +421,15 @@ __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int
             cur = sys;
             a_re = a_im = b_re = b_im = 0;
         }
-        const int64_t base = slice_ptr[slice];
-        const int width = slice_width[slice];
-        const int kst = width < KMAX ? width : KMAX;
+        const int64_t base = next_base;
+        const int width = next_width;
         __syncthreads();
-        for (int e = threadIdx.x; e < KMAX * TR; e += BLOCK) {
-            const int k = e / TR, r = e % TR;
-            if (k < kst) {
-                s_idx[e] = sidx[base + (int64_t)k * WAVE + r];
-                s_val[e] = sval[base + (int64_t)k * WAVE + r];
-            } else {
-                s_idx[e] = (int32_t)(slice * WAVE + r);
-                s_val[e] = zero(T{});
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int e = threadIdx.x + q * BLOCK;
+            if (e < KMAX * TR) {
+                s_idx[e] = p_idx[q];
+                s_val[e] = p_val[q];
             }
         }
         __syncthreads();
@@ -417,7 +447,8 @@ __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int
 #pragma unroll
         for (int k = 0; k < KMAX; ++k)
 #pragma unroll
-            for (int u = 0; u < U; ++u) xg[u][k] = x[(int64_t)s_idx[k * TR + rin[u]] * VW + col];
+            for (int u = 0; u < U; ++u) xg[u][k] = gather(s_idx[k * TR + rin[u]]);
+        if (slice + 1 < t1) prefetch(slice + 1, next_base, next_width);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             T acc = zero(T{});
@@ -1021,12 +1052,18 @@ static void batched_t(const BatchedLaunch& B, hipStream_t stream) {
     a.n = B.n;
     a.hscale = static_cast<typename traits<T>::real>(B.hscale);
     a.sub_y = B.sub_y;
-    if (B.max_width <= 4)
-        hipLaunchKernelGGL((k_step_batched<T, VW, 4>), dim3(B.num_wg), dim3(BLOCK), 0, stream, a, B.slice_ptr,
-                           B.slice_width, B.slice_sys, B.tile_begin, B.slot_begin);
-    else
-        hipLaunchKernelGGL((k_step_batched<T, VW, 8>), dim3(B.num_wg), dim3(BLOCK), 0, stream, a, B.slice_ptr,
-                           B.slice_width, B.slice_sys, B.tile_begin, B.slot_begin);
+    const bool off32 = (uint64_t)B.n * VW * sizeof(T) < ((uint64_t)1 << 32);
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(B.num_wg), dim3(BLOCK), 0, stream, a, B.slice_ptr, B.slice_width, B.slice_sys,
+                           B.tile_begin, B.slot_begin);
+    };
+    if (B.max_width <= 4) {
+        if (off32) go(k_step_batched<T, VW, 4, true>);
+        else go(k_step_batched<T, VW, 4, false>);
+    } else {
+        if (off32) go(k_step_batched<T, VW, 8, true>);
+        else go(k_step_batched<T, VW, 8, false>);
+    }
 }
 
 template <class T>
