@@ -212,7 +212,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellb(StepArgs<T> a, const int64
     constexpr int U = RW / RPW;                    // row groups per wave per tile
     constexpr int TR = RW * WAVES_PER_BLOCK;       // rows per tile (divides 64)
     constexpr int TPS = WAVE / TR;                 // tiles per slice
-    static_assert(VW >= 4 && KMAX % 4 == 0 && RW % RPW == 0 && WAVE % TR == 0, "bad block configuration");
+    static_assert(VW >= 4 && RW % RPW == 0 && WAVE % TR == 0, "bad block configuration");
     __shared__ int32_t s_idx[KMAX * TR];
     __shared__ T s_val[KMAX * TR];
     const int lane = threadIdx.x % WAVE;
@@ -846,6 +846,8 @@ static int launch_sellb_k(StepArgs<T> a, const StepLaunch& L, hipStream_t stream
 
 template <class T, int VW>
 static int launch_sellb(StepArgs<T> a, const StepLaunch& L, bool narrow, hipStream_t stream) {
+    // rows of exactly 5 entries (square lattices with onsite terms): the diagonal path stages 5
+    if (!narrow && L.max_width == 5 && L.mode == MODE_DIAG) return launch_sellb_k<T, VW, MODE_DIAG, 5>(a, L, stream);
     if (narrow) {
         switch (L.mode) {
             case MODE_DIAG: return launch_sellb_k<T, VW, MODE_DIAG, 4>(a, L, stream);
@@ -1060,6 +1062,9 @@ static void batched_t(const BatchedLaunch& B, hipStream_t stream) {
     if (B.max_width <= 4) {
         if (off32) go(k_step_batched<T, VW, 4, true>);
         else go(k_step_batched<T, VW, 4, false>);
+    } else if (B.max_width == 5) {   // square lattices with onsite terms
+        if (off32) go(k_step_batched<T, VW, 5, true>);
+        else go(k_step_batched<T, VW, 5, false>);
     } else {
         if (off32) go(k_step_batched<T, VW, 8, true>);
         else go(k_step_batched<T, VW, 8, false>);

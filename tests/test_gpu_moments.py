@@ -52,6 +52,25 @@ def normwise(mu, ref):
     return np.max(np.abs(mu - ref) / np.abs(ref[:, :1]))
 
 
+@pytest.mark.parametrize("nvec", [4, 16, 64])
+def test_diagonal_five_entry_rows(ctx, oracle, nvec):
+    """Square lattice with onsite disorder: every interior row has exactly 5 entries (the SELL
+    block kernel's 5-entry staging)."""
+    h = generators.square_flake(40, 37, t=-1.0, disorder=1.5, seed=6)
+    a, b = oracle.scale_factors(-4.8, 4.8)
+    h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
+    h2.sort_indices()
+    assert np.diff(h2.indptr).max() == 5
+    r0 = oracle.random_starter(h2.shape[0], nvec, np.float64)
+    mat = ctx.mat(h2, fmt=1)
+    blk = ctx.block(r0)
+    mu = abi.moments_diagonal(mat, blk, 130)
+    ref = oracle.diag_moments(h2, r0, 130)
+    blk.close()
+    mat.close()
+    assert normwise(mu, ref) <= 1e-10
+
+
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize("nvec", [1, 2, 3, 8, 16, 32, 64, 70])
 @pytest.mark.parametrize("fmt", [0, 1], ids=["csr", "sell"])
