@@ -361,8 +361,13 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / 5
+        from pybinding_amd import abi
+        pctx = abi.Ctx(0)
+        cg, rg = pctx.stream_probe(1 << 31)
+        pctx.close()
         print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8,
-                          "copy_ms": ms, "copy_GBps": 2 * a.numel() * 8 / (ms / 1000) / 1e9}))
+                          "copy_ms": ms, "copy_GBps": 2 * a.numel() * 8 / (ms / 1000) / 1e9,
+                          "stream_probe_copy_GBps": cg, "stream_probe_read_GBps": rg}))
         return
 
     if args.gpus != 1:
@@ -413,6 +418,11 @@ def main():
 
     traffic_key = "%dx%d %s v%d %s %s" % (args.cells[0], args.cells[1], dtype.name, R, args.format, args.order)
     extra = {}
+    cg, rg = ctx.stream_probe(1 << 31)
+    extra["stream_probe"] = {"copy_GBps": cg, "read_GBps": rg,
+                             "step_traffic_over_copy": (counted_traffic(traffic_key) or float("nan")) /
+                             (step_ms / 1000) / 1e9 / cg}
+    log("stream probe: copy %.0f GB/s, read %.0f GB/s" % (cg, rg))
     if not args.no_plain:
         # the plain single-vector path (one vector, SELL-64 kernel) on the same matrix, timed over
         # 512 moments and reported in the same metric

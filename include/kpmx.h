@@ -100,9 +100,10 @@ int kpmx_block_destroy(kpmx_block* b);
 int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu);
 
 /* Autotuner (no counterpart in the reference, whose CPU kernels have one fixed shape): time the
- * tile-shape variants of the fused diagonal step kernel on this matrix and starter block and keep
- * the fastest for every later recurrence of that scalar type and vector width on the context.
- * variant (optional): the chosen variant index. */
+ * variants of the fused diagonal step kernel (three tile shapes, each on the full resident grid,
+ * half of it or a quarter) on this matrix and starter block and keep the fastest for every later
+ * recurrence of that scalar type and vector width on the context.
+ * variant (optional): the chosen variant index, tile shape + 3 * grid fraction (0: full). */
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant);
 
 /* Same recurrence, results kept as raw per-step sums on the device; for timing loops.
@@ -178,6 +179,12 @@ int kpmx_batch_destroy(kpmx_batch* b);
  * the small tridiagonal eigenproblem on the host. */
 int kpmx_lanczos_bounds(const kpmx_mat* h, double precision_percent, double* min_energy, double* max_energy,
                         int* loops);
+
+/* Streaming probe (no counterpart in the reference; the practical bandwidth reference for the
+ * roofline reports): best GB/s over a few grid sizes of a non-temporal 16-byte copy of `bytes`
+ * (bytes read + bytes written) and of a 16-byte read-only sweep, on buffers the call allocates
+ * and frees. */
+int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* read_gbps);
 
 #ifdef __cplusplus
 }

@@ -75,6 +75,12 @@ class Ctx:
         _check(lib().kpmx_ctx_timer_stop(self.h, ctypes.byref(ms)))
         return ms.value
 
+    def stream_probe(self, nbytes=1 << 31):
+        """(copy GB/s, read GB/s) of the device's streaming probe (kpmx_stream_probe)."""
+        c, r = ctypes.c_double(), ctypes.c_double()
+        _check(lib().kpmx_stream_probe(self.h, ctypes.c_int64(nbytes), ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
 
 class Mat:
     def __init__(self, ctx, csr, fmt=0):

@@ -1592,4 +1592,41 @@ int kpmx_adjoint_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, c
     return rc;
 }
 
+int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* read_gbps) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx && copy_gbps && read_gbps, "null argument");
+        KPMX_REQUIRE(bytes >= (1 << 20) && bytes % 16 == 0, "bytes must be a multiple of 16, at least 1 MiB");
+        KPMX_HIP(hipSetDevice(ctx->device));
+        void* a = dev_alloc((size_t)bytes);
+        void* b = dev_alloc((size_t)bytes);
+        struct Free {
+            void *a, *b;
+            ~Free() {
+                dev_free(a);
+                dev_free(b);
+            }
+        } guard{a, b};
+        KPMX_HIP(hipMemsetAsync(a, 0, (size_t)bytes, ctx->stream));
+        KPMX_HIP(hipMemsetAsync(b, 0, (size_t)bytes, ctx->stream));
+        if (!ctx->t0) KPMX_HIP(hipEventCreate(&ctx->t0));
+        if (!ctx->t1) KPMX_HIP(hipEventCreate(&ctx->t1));
+        double best[2] = {0, 0};
+        for (int kind = 0; kind < 2; ++kind)
+            for (int per_cu : {2, 4, 8, 16})
+                for (int rep = 0; rep < 3; ++rep) {
+                    KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
+                    launch_probe(kind, a, b, bytes, per_cu * ctx->num_cu, ctx->stream);
+                    KPMX_HIP(hipGetLastError());
+                    KPMX_HIP(hipEventRecord(ctx->t1, ctx->stream));
+                    KPMX_HIP(hipEventSynchronize(ctx->t1));
+                    float ms = 0;
+                    KPMX_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+                    const double moved = (kind == 0 ? 2.0 : 1.0) * (double)bytes;
+                    if (rep > 0 && ms > 0) best[kind] = std::max(best[kind], moved / (ms * 1e-3) / 1e9);
+                }
+        *copy_gbps = best[0];
+        *read_gbps = best[1];
+    });
+}
+
 }  // extern "C"

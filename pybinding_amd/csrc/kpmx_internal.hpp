@@ -78,7 +78,8 @@ int launch_adjoint_step(const AdjointLaunch& A, hipStream_t stream);
 
 int launch_step(const StepLaunch& L, hipStream_t stream);
 int step_grid(const StepLaunch& L);
-constexpr int NUM_TUNE_VARIANTS = 3;
+constexpr int NUM_TILE_SHAPES = 3;     // rows per wave per tile (k_step_sellb)
+constexpr int NUM_TUNE_VARIANTS = 9;   // tile shape x grid (resident, 1/2, 1/4)
 int64_t step_rows_per_tile(int format, int vw);
 void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
                             hipStream_t stream);
@@ -94,6 +95,8 @@ void launch_scatter_scaled(int scalar, const void* src, int64_t n, int vw, int64
 int64_t gemm_workspace_doubles(int64_t M, int64_t kdim, int scalar, int mode, int num_cu);
 int launch_gemm_nc(int scalar, int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M,
                    int64_t kdim, double* C, double* work, int num_cu, hipStream_t s);
+// streaming probes: kind 0 copies `bytes` from a to b, kind 1 reads a
+void launch_probe(int kind, const void* a, void* b, int64_t bytes, int grid, hipStream_t stream);
 void launch_zero(int scalar, void* dst, int64_t count, hipStream_t s);
 void launch_zero_f64(double* dst, int64_t count, hipStream_t s);
 void launch_csr_matvec(int scalar, const int32_t* indptr, const int32_t* indices, const void* vals, const void* x,

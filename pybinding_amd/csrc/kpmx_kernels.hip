@@ -820,8 +820,15 @@ static int launch_sellb_off(StepArgs<T> a, const StepLaunch& L, hipStream_t stre
     auto kernel = k_step_sellb<T, VW, MODE, KMAX, RW, OFF32>;
     constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
     a.num_tiles = (L.n + tr - 1) / tr;
-    if (L.query_grid) return resident_grid(kernel, L.num_cu, a.num_tiles);
-    const int grid = L.num_wg > 0 ? L.num_wg : resident_grid(kernel, L.num_cu, a.num_tiles);
+    // tuned variants 3..8 run 1/2 or 1/4 of the resident grid (fewer waves in flight per CU)
+    const int shift = (MODE == MODE_DIAG) ? L.tune / NUM_TILE_SHAPES : 0;
+    auto grid_of = [&] {
+        int g = resident_grid(kernel, L.num_cu, a.num_tiles) >> shift;
+        if (g >= 8) g -= g % 8;
+        return g < 1 ? 1 : g;
+    };
+    if (L.query_grid) return grid_of();
+    const int grid = L.num_wg > 0 ? L.num_wg : grid_of();
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, L.slice_ptr, L.slice_width);
     return 0;
 }
@@ -836,7 +843,7 @@ static int launch_sellb_rw(StepArgs<T> a, const StepLaunch& L, hipStream_t strea
 template <class T, int VW, int MODE, int KMAX>
 static int launch_sellb_k(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
     // the diagonal (hot) path carries the tuned variants, the other modes use the default shape
-    const int tune = (MODE == MODE_DIAG) ? L.tune : 0;
+    const int tune = (MODE == MODE_DIAG) ? L.tune % NUM_TILE_SHAPES : 0;
     switch (tune) {
         case 1: return launch_sellb_rw<T, VW, MODE, KMAX, sellb_rw<VW>(1)>(a, L, stream);
         case 2: return launch_sellb_rw<T, VW, MODE, KMAX, sellb_rw<VW>(2)>(a, L, stream);
@@ -1153,6 +1160,32 @@ int launch_adjoint_step(const AdjointLaunch& A, hipStream_t stream) {
         case 1: return adjoint_vw<double>(A, stream);
         default: return -1;  // real Hamiltonians only
     }
+}
+
+// Streaming probes (device bandwidth reference for the roofline): a non-temporal 16-byte copy and
+// a 16-byte read-only sweep, grid-stride loops over n elements.
+typedef double probe_v2 __attribute__((ext_vector_type(2)));
+__global__ void k_probe_copy(const probe_v2* __restrict__ a, probe_v2* __restrict__ b, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(&a[i]), &b[i]);
+}
+__global__ void k_probe_read(const probe_v2* __restrict__ a, double* out, int64_t n) {
+    double s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const probe_v2 v = a[i];
+        s += v.x + v.y;
+    }
+    if (s == 1.25e300) out[0] = s;   // never true for the zero-filled input; keeps the loads
+}
+
+void launch_probe(int kind, const void* a, void* b, int64_t bytes, int grid, hipStream_t stream) {
+    const int64_t n = bytes / (int64_t)sizeof(probe_v2);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_probe_copy, dim3(grid), dim3(256), 0, stream, static_cast<const probe_v2*>(a),
+                           static_cast<probe_v2*>(b), n);
+    else
+        hipLaunchKernelGGL(k_probe_read, dim3(grid), dim3(256), 0, stream, static_cast<const probe_v2*>(a),
+                           static_cast<double*>(b), n);
 }
 
 }  // namespace kpmx
