@@ -893,7 +893,12 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
         size_t free_b = 0, total_b = 0;
         KPMX_HIP(hipMemGetInfo(&free_b, &total_b));
         const double budget = 0.45 * (double)free_b;
-        const size_t panel1 = (size_t)M * n * sb;
+        // bf16x3: panels are written by the sweeps as bf16 pieces (3 per real part), rows padded to
+        // a multiple of 8 elements with a zero tail
+        const bool split = mode == KPMX_KUBO_BF16X3;
+        const int64_t ldp = split ? (n + 7) / 8 * 8 : n;
+        const size_t plane = (size_t)M * ldp;
+        const size_t panel1 = split ? (size_t)cw * 3 * plane * sizeof(uint16_t) : (size_t)M * n * sb;
         KPMX_REQUIRE(2.0 * panel1 <= budget, "Kubo moment panels do not fit in device memory");
 
         double* C = static_cast<double*>(ctx->sums.get((size_t)M * M * cw * sizeof(double)));
@@ -918,6 +923,12 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
                 }
                 char* Lpan = static_cast<char*>(ctx->panel_l.get((size_t)g * panel1));
                 char* Rpan = static_cast<char*>(ctx->panel_r.get((size_t)g * panel1));
+                uint16_t* Lsp = reinterpret_cast<uint16_t*>(Lpan);
+                uint16_t* Rsp = reinterpret_cast<uint16_t*>(Rpan);
+                if (split && ldp > n) {
+                    launch_zero_tails(Lsp, (int64_t)g * cw * 3 * M, ldp, n, ctx->stream);
+                    launch_zero_tails(Rsp, (int64_t)g * cw * 3 * M, ldp, n, ctx->stream);
+                }
                 void* L0 = ctx->aux2.get((size_t)n * g * sb);
 
                 // L0 = vl r
@@ -933,13 +944,21 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
                 tl.vw = g;
                 tl.cols = cols;
                 tl.data = L0;
-                launch_scatter_scaled(sc, L0, n, g, g, 0.5, Lpan, M * n, ctx->stream);
+                if (split)
+                    launch_scatter_split(sc, L0, n, g, g, 0.5, Lsp, (int64_t)plane, ctx->stream);
+                else
+                    launch_scatter_scaled(sc, L0, n, g, g, 0.5, Lpan, M * n, ctx->stream);
                 KPMX_HIP(hipGetLastError());
                 run_recurrence(
                     h2, tl, MODE_NONE, M - 1, nullptr, nullptr,
                     [&](StepLaunch& L, int64_t k) {
-                        L.copy_out = Lpan + (size_t)(k + 1) * n * sb;
-                        L.copy_col_stride = M * n;
+                        if (split) {
+                            L.copy_split = Lsp + (size_t)(k + 1) * ldp;
+                            L.copy_col_stride = (int64_t)plane;
+                        } else {
+                            L.copy_out = Lpan + (size_t)(k + 1) * n * sb;
+                            L.copy_col_stride = M * n;
+                        }
                     },
                     [](StepLaunch const&, int64_t) {});
                 // R panel rows: vr T_m(H~) r (row 0 halved)
@@ -947,10 +966,13 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
                 O.mode = MODE_NONE;
                 O.sub_y = 0;
                 O.yout = nullptr;
-                O.copy_col_stride = M * n;
+                O.copy_col_stride = split ? (int64_t)plane : M * n;
                 O.x = Pr;
                 O.hscale = 0.5;
-                O.copy_out = Rpan;
+                if (split)
+                    O.copy_split = Rsp;
+                else
+                    O.copy_out = Rpan;
                 launch(O, ctx->stream);
                 Tile tr;
                 tr.vw = g;
@@ -961,16 +983,23 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
                     [&](StepLaunch const& L, int64_t k) {
                         O.x = L.yout;
                         O.hscale = 1.0;
-                        O.copy_out = Rpan + (size_t)(k + 1) * n * sb;
+                        if (split)
+                            O.copy_split = Rsp + (size_t)(k + 1) * ldp;
+                        else
+                            O.copy_out = Rpan + (size_t)(k + 1) * n * sb;
                         launch(O, ctx->stream);
                     });
                 // mu_nm += sum_c L_c R_c^H
                 const int64_t wd = gemm_workspace_doubles(M, n, sc, mode, ctx->num_cu);
                 double* work = static_cast<double*>(ctx->partials.get((size_t)wd * sizeof(double)));
                 for (int64_t c = 0; c < cols; ++c) {
-                    if (launch_gemm_nc(sc, mode, Lpan + (size_t)c * M * n * sb, n, Rpan + (size_t)c * M * n * sb, n, M,
-                                       n, C, work, ctx->num_cu, ctx->stream) != 0)
-                        throw Error{KPMX_EINVAL, "unsupported contraction"};
+                    const int rc = split ? launch_gemm_nc_planes(cplx, Lsp + (size_t)c * cw * 3 * plane,
+                                                                 Rsp + (size_t)c * cw * 3 * plane, ldp, (int64_t)plane,
+                                                                 M, n, C, work, ctx->num_cu, ctx->stream)
+                                         : launch_gemm_nc(sc, mode, Lpan + (size_t)c * M * n * sb, n,
+                                                          Rpan + (size_t)c * M * n * sb, n, M, n, C, work, ctx->num_cu,
+                                                          ctx->stream);
+                    if (rc != 0) throw Error{KPMX_EINVAL, "unsupported contraction"};
                     KPMX_HIP(hipGetLastError());
                 }
             }

@@ -74,4 +74,49 @@ template <class R> __device__ __forceinline__ void nt_store(cplx<R> v, cplx<R>* 
     __builtin_nontemporal_store(v.im, q + 1);
 }
 
+
+// ---- three-piece bf16 split (fp32-accurate operands for bf16 MFMA) ----
+// x = x0 + x1 + x2 + O(2^-24 x): x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1), the
+// remainders exact in the input precision; pieces returned as 16-bit patterns.
+__device__ __forceinline__ uint32_t bf16_bits_rn(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;   // round to nearest even (finite inputs)
+}
+__device__ __forceinline__ float bf16_value(uint32_t h) { return __uint_as_float(h << 16); }
+template <class R>
+__device__ __forceinline__ void split3(R v, uint16_t& p0, uint16_t& p1, uint16_t& p2) {
+    const uint32_t h0 = bf16_bits_rn((float)v);
+    const R r1 = v - (R)bf16_value(h0);
+    const uint32_t h1 = bf16_bits_rn((float)r1);
+    const R r2 = r1 - (R)bf16_value(h1);
+    const uint32_t h2 = bf16_bits_rn((float)r2);
+    p0 = (uint16_t)h0;
+    p1 = (uint16_t)h1;
+    p2 = (uint16_t)h2;
+}
+// Write the pieces of v (column col, row row) into split planes: plane ((col * NP + p) * 3 + j)
+// of `base` (stride `plane` elements) holds piece j of part p (NP = 1 real, 2 complex: re, im).
+template <class R>
+__device__ __forceinline__ void store_split(uint16_t* base, int64_t plane, int col, int64_t row, R v) {
+    uint16_t a, b, c;
+    split3<R>(v, a, b, c);
+    uint16_t* q = base + (int64_t)col * 3 * plane + row;
+    q[0] = a;
+    q[plane] = b;
+    q[2 * plane] = c;
+}
+template <class R>
+__device__ __forceinline__ void store_split(uint16_t* base, int64_t plane, int col, int64_t row, cplx<R> v) {
+    uint16_t a, b, c;
+    uint16_t* q = base + (int64_t)col * 6 * plane + row;
+    split3<R>(v.re, a, b, c);
+    q[0] = a;
+    q[plane] = b;
+    q[2 * plane] = c;
+    split3<R>(v.im, a, b, c);
+    q[3 * plane] = a;
+    q[4 * plane] = b;
+    q[5 * plane] = c;
+}
+
 }  // namespace kpmx

@@ -1,7 +1,7 @@
 // Kubo-Bastin moment contraction  mu_nm += sum_k L[n][k] conj(R[m][k])
 // (the reference's MomentMultiplication::matrix_mul_add, cppcore/src/kpm/Moments.cpp:92-101).
 //
-// Two implementations of the same M x M contraction over a long k (= sites x vectors):
+// Three implementations of the same M x M contraction over a long k (= sites x vectors):
 //   mode 0 (FP64): 64x64 output tiles, 16-deep k panels staged in LDS, every product and sum in
 //                  double on the vector ALUs (FP64 vector peak equals the FP64 matrix peak on
 //                  gfx950), accumulation exact to double rounding.
@@ -10,8 +10,9 @@
 //                  formed with v_mfma_f32_32x32x2_f32; the float accumulators are flushed into
 //                  double every 32-deep k panel, so the rounding of the float sums stays bounded
 //                  by one panel's depth.
-//   mode 2 (bf16x3 split MFMA): three bf16 pieces per operand value, six piece products per real
-//                  product on v_mfma_f32_32x32x16_bf16 (fp32-accurate), see below.
+//   mode 2 (bf16x3 split MFMA): three bf16 pieces per operand value (written by the sweeps that
+//                  produce L and R), six piece products per real product on
+//                  v_mfma_f32_32x32x16_bf16 (fp32-accurate), see below.
 // All split k into S slices written to a [S][M][M] workspace and summed in slice order, so the
 // result is deterministic.
 #include <algorithm>
@@ -219,42 +220,20 @@ __global__ __launch_bounds__(256) void k_gemm_nc_split(const T* __restrict__ A, 
 // fp32-accurate contraction on the bf16 matrix cores.  Complex products use four real products
 // (Cr += Lr Rr + Li Ri, Ci += Li Rr - Lr Ri), i.e. 24 MFMAs per 32x32x16 complex block.
 //
+// The pieces are produced once, by the sweeps that write the L and R panels (store_split), so
+// the contraction only copies 16-byte chunks into LDS.
 // Work decomposition (persistent, XCD-aware): a workgroup of 8 waves computes one 128x64 output
-// tile over one K range; each wave owns a 32x32 block and keeps four independent fp32
-// accumulator chains (Lr Rr, Li Ri, Li Rr, -Lr Ri) plus the fp64 sums.  The T output tiles of one K
-// range run on the same XCD (workgroup b sits on XCD b % 8), so the K range's L and R rows are
-// read from HBM once and served to the T workgroups from that XCD's L2.  Partial tiles go to
-// [KG][M][M] and are summed in K-range order (deterministic).
+// tile over one K range; each wave owns a 32x32 block with two fp32 accumulator chains (real,
+// imaginary) plus the fp64 sums.  The T output tiles of one K range run on the same XCD
+// (workgroup b sits on XCD b % 8), so the K range's L and R rows are read from HBM once and
+// served to the T workgroups from that XCD's L2.  Partial tiles go to [KG][M][M] and are summed
+// in K-range order (deterministic).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr int XTN = 128;         // output tile rows (n)
 constexpr int XTM = 64;          // output tile columns (m)
 constexpr int XK = 32;           // k per LDS stage (two MFMA k16 steps)
 constexpr int XS = XK + 8;       // LDS row stride in bf16 (pads the 16-byte lane reads off bank conflicts)
 constexpr int XTHREADS = 512;    // 8 waves: 4 x 2 blocks of 32 x 32
-
-__device__ __forceinline__ uint32_t bf16_bits_rn(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;   // round to nearest even (finite inputs)
-}
-__device__ __forceinline__ float bf16_value(uint32_t h) { return __uint_as_float(h << 16); }
-
-// three bf16 pieces of v, as 16-bit patterns
-template <class R>
-__device__ __forceinline__ void split3(R v, uint16_t& p0, uint16_t& p1, uint16_t& p2) {
-    const uint32_t h0 = bf16_bits_rn((float)v);
-    const R r1 = v - (R)bf16_value(h0);
-    const uint32_t h1 = bf16_bits_rn((float)r1);
-    const R r2 = r1 - (R)bf16_value(h1);
-    const uint32_t h2 = bf16_bits_rn((float)r2);
-    p0 = (uint16_t)h0;
-    p1 = (uint16_t)h1;
-    p2 = (uint16_t)h2;
-}
-
-template <class T> struct parts_of { using R = T; static constexpr int N = 1; };
-template <class R_> struct parts_of<cplx<R_>> { using R = R_; static constexpr int N = 2; };
-template <class R> __device__ __forceinline__ R part(R v, int) { return v; }
-template <class R> __device__ __forceinline__ R part(cplx<R> v, int p) { return p == 0 ? v.re : v.im; }
 
 __device__ __forceinline__ bf16x8 negate(bf16x8 v) {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -263,18 +242,25 @@ __device__ __forceinline__ bf16x8 negate(bf16x8 v) {
     return __builtin_bit_cast(bf16x8, u);
 }
 
-template <class T>
-__global__ __launch_bounds__(XTHREADS) void k_gemm_nc_bf16x3(const T* __restrict__ A, int64_t lda,
-                                                             const T* __restrict__ B, int64_t ldb, int64_t M,
-                                                             int64_t kdim, int64_t kslice, int groups_per_xcd,
-                                                             int64_t tiles_per_wg, double* __restrict__ P) {
-    constexpr int NP = parts_of<T>::N;   // real parts per value (1 real, 2 complex)
-    using R = typename parts_of<T>::R;
+// The bf16x3 contraction on operands already split into bf16 planes by the producing sweeps
+// (store_split): A planes pa(p, j) = A + (p * 3 + j) * plane, rows of `ld` elements (ld a
+// multiple of 8, the row tail [kdim, ld) zero).  Staging is a plain 16-byte copy, software
+// pipelined: the next k panel is loaded into registers while the current one is multiplied.
+template <int NP>
+__global__ __launch_bounds__(XTHREADS) void k_gemm_nc_planes(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B,
+                                                             int64_t ld, int64_t plane, int64_t M, int64_t kdim,
+                                                             int64_t kslice, int groups_per_xcd, int64_t tiles_per_wg,
+                                                             double* __restrict__ P) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     __shared__ __align__(16) uint16_t As[NP][3][XTN][XS];
     __shared__ __align__(16) uint16_t Bs[NP][3][XTM][XS];
+    constexpr int CA = NP * 3 * XTN * (XK / 8);   // 16-byte chunks per stage
+    constexpr int CB = NP * 3 * XTM * (XK / 8);
+    constexpr int QA = (CA + XTHREADS - 1) / XTHREADS;
+    constexpr int QB = (CB + XTHREADS - 1) / XTHREADS;
     const int t = threadIdx.x;
     const int lane = t % 64, wave = t / 64;
-    const int br = wave / 2, bc = wave % 2;      // this wave's 32x32 block inside the tile
+    const int br = wave / 2, bc = wave % 2;
     const int64_t tiles_n = (M + XTN - 1) / XTN, tiles_m = (M + XTM - 1) / XTM;
     const int64_t T2 = tiles_n * tiles_m;
     const int xcd = blockIdx.x % 8;
@@ -293,40 +279,59 @@ __global__ __launch_bounds__(XTHREADS) void k_gemm_nc_bf16x3(const T* __restrict
     }
     const int64_t kbeg = kg * kslice;
     const int64_t kend = (kbeg + kslice < kdim) ? kbeg + kslice : kdim;
-    // rows x 32 k of src into dst[NP][3][rows][XS] as bf16 pieces, two consecutive k per item
-    auto stage_rows = [&](const T* __restrict__ src, int64_t ld, int64_t r0, int rows, int64_t k0, uint16_t* dst) {
-        for (int e = t; e < rows * (XK / 2); e += XTHREADS) {
-            const int r = e / (XK / 2), kk = 2 * (e % (XK / 2));
-            const int64_t k = k0 + kk;
-            T v[2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) v[h] = (k + h < kend && r0 + r < M) ? src[(r0 + r) * ld + k + h] : zero(T{});
-#pragma unroll
-            for (int p = 0; p < NP; ++p) {
-                uint16_t x[3][2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h) split3<R>(part(v[h], p), x[0][h], x[1][h], x[2][h]);
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    *reinterpret_cast<uint32_t*>(dst + ((p * 3 + j) * rows + r) * XS + kk) =
-                        (uint32_t)x[j][0] | ((uint32_t)x[j][1] << 16);
-            }
-        }
+    u32x4 ra[QA], rb[QB];
+    // chunk e of a stage: plane pj, row r, 8-element chunk q (rows beyond M, chunks at or beyond
+    // kend: zero)
+    auto load = [&](const uint16_t* __restrict__ src, int64_t r0, int rows, int64_t k0, int e) -> u32x4 {
+        const int q = e % (XK / 8), r = (e / (XK / 8)) % rows, pj = e / (rows * (XK / 8));
+        const int64_t k = k0 + q * 8;
+        if (r0 + r >= M || k >= kend) return u32x4{0, 0, 0, 0};
+        return *reinterpret_cast<const u32x4*>(src + pj * plane + (r0 + r) * ld + k);
     };
     for (int64_t tile = tile0, it = 0; tile < T2 && it < tiles_per_wg; tile += tile_step, ++it) {
         const int64_t n0 = (tile / tiles_m) * XTN, m0 = (tile % tiles_m) * XTM;
         double dre[16], dim_[16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) dre[i] = dim_[i] = 0.0;
-        f32x16 c0, c1, c2, c3;
+        f32x16 cre, cim;   // two accumulator chains (registers: the prefetched panel shares the budget)
 #pragma unroll
-        for (int i = 0; i < 16; ++i) c0[i] = c1[i] = c2[i] = c3[i] = 0.f;
+        for (int i = 0; i < 16; ++i) cre[i] = cim[i] = 0.f;
+        if (kbeg < kend) {
+#pragma unroll
+            for (int q = 0; q < QA; ++q)
+                if (t + q * XTHREADS < CA) ra[q] = load(A, n0, XTN, kbeg, t + q * XTHREADS);
+#pragma unroll
+            for (int q = 0; q < QB; ++q)
+                if (t + q * XTHREADS < CB) rb[q] = load(B, m0, XTM, kbeg, t + q * XTHREADS);
+        }
         int stage = 0;
         for (int64_t k0 = kbeg; k0 < kend; k0 += XK, ++stage) {
             __syncthreads();   // the previous stage's LDS operands are no longer read
-            stage_rows(A, lda, n0, XTN, k0, &As[0][0][0][0]);
-            stage_rows(B, ldb, m0, XTM, k0, &Bs[0][0][0][0]);
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                const int e = t + q * XTHREADS;
+                if (e < CA) {
+                    const int qq = e % (XK / 8), r = (e / (XK / 8)) % XTN, pj = e / (XTN * (XK / 8));
+                    *reinterpret_cast<u32x4*>(&As[pj / 3][pj % 3][r][qq * 8]) = ra[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                const int e = t + q * XTHREADS;
+                if (e < CB) {
+                    const int qq = e % (XK / 8), r = (e / (XK / 8)) % XTM, pj = e / (XTM * (XK / 8));
+                    *reinterpret_cast<u32x4*>(&Bs[pj / 3][pj % 3][r][qq * 8]) = rb[q];
+                }
+            }
             __syncthreads();
+            if (k0 + XK < kend) {   // the next panel is in flight during the multiplications
+#pragma unroll
+                for (int q = 0; q < QA; ++q)
+                    if (t + q * XTHREADS < CA) ra[q] = load(A, n0, XTN, k0 + XK, t + q * XTHREADS);
+#pragma unroll
+                for (int q = 0; q < QB; ++q)
+                    if (t + q * XTHREADS < CB) rb[q] = load(B, m0, XTM, k0 + XK, t + q * XTHREADS);
+            }
             const int ar = br * 32 + (lane % 32);
             const int bcl = bc * 32 + (lane % 32);
 #pragma unroll
@@ -340,7 +345,6 @@ __global__ __launch_bounds__(XTHREADS) void k_gemm_nc_bf16x3(const T* __restrict
                         a[p][j] = *reinterpret_cast<const bf16x8*>(&As[p][j][ar][ko]);
                         b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[p][j][bcl][ko]);
                     }
-                // piece pairs (i, j) with i + j <= 2, smallest first
 #define KPMX_PAIRS(acc, X, Y)                                                        \
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[2], Y[0], acc, 0, 0, 0);         \
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[1], Y[1], acc, 0, 0, 0);         \
@@ -348,21 +352,23 @@ __global__ __launch_bounds__(XTHREADS) void k_gemm_nc_bf16x3(const T* __restrict
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[1], Y[0], acc, 0, 0, 0);         \
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[0], Y[1], acc, 0, 0, 0);         \
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(X[0], Y[0], acc, 0, 0, 0);
-                KPMX_PAIRS(c0, a[0], b[0]);
+                KPMX_PAIRS(cre, a[0], b[0]);
                 if constexpr (NP == 2) {
-                    bf16x8 nbi[3] = {negate(b[1][0]), negate(b[1][1]), negate(b[1][2])};
-                    KPMX_PAIRS(c1, a[1], b[1]);
-                    KPMX_PAIRS(c2, a[1], b[0]);
-                    KPMX_PAIRS(c3, a[0], nbi);
+                    KPMX_PAIRS(cim, a[1], b[0]);
+                    KPMX_PAIRS(cre, a[1], b[1]);
+                    b[1][0] = negate(b[1][0]);
+                    b[1][1] = negate(b[1][1]);
+                    b[1][2] = negate(b[1][2]);
+                    KPMX_PAIRS(cim, a[0], b[1]);
                 }
 #undef KPMX_PAIRS
             }
             if ((stage & 1) == 1 || k0 + XK >= kend) {   // flush the fp32 sums every 64 k
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    dre[i] += (double)c0[i] + (double)c1[i];
-                    dim_[i] += (double)c2[i] + (double)c3[i];
-                    c0[i] = c1[i] = c2[i] = c3[i] = 0.f;
+                    dre[i] += (double)cre[i];
+                    dim_[i] += (double)cim[i];
+                    cre[i] = cim[i] = 0.f;
                 }
             }
         }
@@ -436,15 +442,8 @@ template <class T>
 static void gemm_t(int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M, int64_t kdim, double* C,
                    double* work, int num_cu, hipStream_t s) {
     const int64_t count = M * M * (traits<T>::is_complex ? 2 : 1);
-    int64_t S;
-    if (mode == 2) {
-        const X3Plan p = x3_plan(M, kdim, num_cu);
-        S = p.kg_total;
-        hipLaunchKernelGGL(k_gemm_nc_bf16x3<T>, dim3((unsigned)p.grid), dim3(XTHREADS), 0, s, static_cast<const T*>(A),
-                           lda, static_cast<const T*>(B), ldb, M, kdim, p.kslice, p.groups_per_xcd, p.tiles_per_wg,
-                           work);
-    } else {
-        S = num_slices(M, kdim);
+    const int64_t S = num_slices(M, kdim);
+    {
         int64_t kslice = (kdim + S - 1) / S;
         const int64_t align = mode == 1 ? KP : TK;
         kslice = (kslice + align - 1) / align * align;
@@ -462,9 +461,26 @@ static void gemm_t(int mode, const void* A, int64_t lda, const void* B, int64_t 
     hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)g), dim3(256), 0, s, work, S, count, C);
 }
 
+int launch_gemm_nc_planes(bool cplx, const uint16_t* A, const uint16_t* B, int64_t ld, int64_t plane, int64_t M,
+                          int64_t kdim, double* C, double* work, int num_cu, hipStream_t s) {
+    if (ld % 8 != 0) return -1;
+    const X3Plan p = x3_plan(M, kdim, num_cu);
+    if (cplx)
+        hipLaunchKernelGGL(k_gemm_nc_planes<2>, dim3((unsigned)p.grid), dim3(XTHREADS), 0, s, A, B, ld, plane, M, kdim,
+                           p.kslice, p.groups_per_xcd, p.tiles_per_wg, work);
+    else
+        hipLaunchKernelGGL(k_gemm_nc_planes<1>, dim3((unsigned)p.grid), dim3(XTHREADS), 0, s, A, B, ld, plane, M, kdim,
+                           p.kslice, p.groups_per_xcd, p.tiles_per_wg, work);
+    const int64_t count = M * M * (cplx ? 2 : 1);
+    int64_t g = (count + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)g), dim3(256), 0, s, work, p.kg_total, count, C);
+    return 0;
+}
+
 int launch_gemm_nc(int scalar, int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M, int64_t kdim,
                    double* C, double* work, int num_cu, hipStream_t s) {
-    if (mode < 0 || mode > 2) return -1;
+    if (mode != 0 && mode != 1) return -1;   // mode 2 runs on pre-split planes (launch_gemm_nc_planes)
     switch (scalar) {
         case 0: gemm_t<float>(mode, A, lda, B, ldb, M, kdim, C, work, num_cu, s); break;
         case 1: gemm_t<double>(mode, A, lda, B, ldb, M, kdim, C, work, num_cu, s); break;

@@ -27,6 +27,7 @@ struct StepLaunch {
     const void* w = nullptr;
     void* copy_out = nullptr;
     int64_t copy_col_stride = 0;
+    uint16_t* copy_split = nullptr;   // bf16 pieces of y_new (see store_split), plane stride copy_col_stride
     const int32_t* mark = nullptr;
     void* mark_out = nullptr;
     int64_t mark_stride = 0;
@@ -87,12 +88,20 @@ void launch_gather_block(int scalar, const void* src, int64_t n, int64_t c0, int
                          hipStream_t s);
 void launch_scatter_block(int scalar, const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst,
                           hipStream_t s);
+void launch_zero_tails(uint16_t* base, int64_t rows, int64_t ld, int64_t n, hipStream_t s);
+// the bf16 pieces of scale * src (columns 0..cols-1) into split planes (stride `plane` elements)
+void launch_scatter_split(int scalar, const void* src, int64_t n, int vw, int64_t cols, double scale, uint16_t* dst,
+                          int64_t plane, hipStream_t s);
 void launch_scatter_scaled(int scalar, const void* src, int64_t n, int vw, int64_t cols, double scale, void* dst,
                            int64_t col_stride, hipStream_t s);
 // C (M x M, {re, im} doubles, or doubles for real scalars) += sum_k A[n][k] conj(B[m][k]) over
 // k < kdim, A and B row-major with leading dimensions lda / ldb (elements).  Deterministic
 // split-K: `work` must hold gemm_workspace_doubles(M, kdim, scalar, mode, num_cu) doubles.
 int64_t gemm_workspace_doubles(int64_t M, int64_t kdim, int scalar, int mode, int num_cu);
+// The bf16x3 contraction on pre-split operands (store_split planes: row stride ld, a multiple of 8
+// with a zero tail beyond kdim; plane stride `plane`); workspace as for mode 2.
+int launch_gemm_nc_planes(bool cplx, const uint16_t* A, const uint16_t* B, int64_t ld, int64_t plane, int64_t M,
+                          int64_t kdim, double* C, double* work, int num_cu, hipStream_t s);
 int launch_gemm_nc(int scalar, int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M,
                    int64_t kdim, double* C, double* work, int num_cu, hipStream_t s);
 // streaming probes: kind 0 copies `bytes` from a to b, kind 1 reads a

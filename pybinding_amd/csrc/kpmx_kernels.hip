@@ -43,6 +43,7 @@ struct StepArgs {
     const T* w;          // MODE_DOT weights, n x VW
     T* copy_out;         // optional: y_new also written to copy_out[col * copy_col_stride + row]
     int64_t copy_col_stride;
+    uint16_t* copy_split;  // optional: y_new written as bf16 pieces (store_split, plane stride copy_col_stride)
     const int32_t* mark; // optional: per-row slot (or -1); y_new written to mark_out[slot * mark_stride]
     T* mark_out;
     int64_t mark_stride;
@@ -126,6 +127,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_csr(StepArgs<T> a) {
         if (a.sub_y) ynew = sub(ynew, a.yin[e]);
         if (a.yout) a.yout[e] = ynew;
         if (a.copy_out) a.copy_out[col * a.copy_col_stride + row] = ynew;
+        if (a.copy_split) store_split(a.copy_split, a.copy_col_stride, col, row, ynew);
         if (a.mark) {
             const int32_t slot = a.mark[row];
             if (slot >= 0) a.mark_out[(int64_t)slot * a.mark_stride + col] = ynew;
@@ -173,6 +175,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sell(StepArgs<T> a, const int64_
         if (a.sub_y) ynew = sub(ynew, a.yin[row]);
         if (a.yout) a.yout[row] = ynew;
         if (a.copy_out) a.copy_out[row] = ynew;
+        if (a.copy_split) store_split(a.copy_split, a.copy_col_stride, 0, row, ynew);
         if (a.mark) {
             const int32_t slot = a.mark[row];
             if (slot >= 0) a.mark_out[(int64_t)slot * a.mark_stride] = ynew;
@@ -316,7 +319,10 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellb(StepArgs<T> a, const int64
             T ynew = scal(acc[u], a.hscale);
             if (a.sub_y) ynew = sub(ynew, yv[u]);
             if (a.yout) nt_store(ynew, &a.yout[e]);
-            if (a.copy_out) a.copy_out[col * a.copy_col_stride + row[u]] = ynew;
+            if constexpr (MODE != MODE_DIAG) {
+                if (a.copy_out) a.copy_out[col * a.copy_col_stride + row[u]] = ynew;
+                if (a.copy_split) store_split(a.copy_split, a.copy_col_stride, col, row[u], ynew);
+            }
             if (a.mark) {
                 const int32_t slot = a.mark[row[u]];
                 if (slot >= 0) a.mark_out[(int64_t)slot * a.mark_stride + col] = ynew;
@@ -728,6 +734,25 @@ __global__ void k_scatter_scaled(const T* src, int64_t n, int vw, int64_t cols, 
 }
 
 template <class T>
+__global__ void k_scatter_split(const T* src, int64_t n, int vw, int64_t cols, double scale, uint16_t* dst,
+                                int64_t plane) {
+    using real = typename traits<T>::real;
+    const int64_t total = n * cols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n;
+        const int64_t row = e % n;
+        store_split(dst, plane, (int)c, row, scal(src[row * vw + c], (real)scale));
+    }
+}
+
+// zero elements [n, ld) of `rows` rows of ld 16-bit elements
+__global__ void k_zero_tails(uint16_t* base, int64_t rows, int64_t ld, int64_t n) {
+    const int64_t w = ld - n, total = rows * w;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+        base[(e / w) * ld + n + e % w] = 0;
+}
+
+template <class T>
 __global__ void k_fill(T* dst, int64_t count, T value) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x)
         dst[e] = value;
@@ -881,6 +906,7 @@ static int launch_step_t(const StepLaunch& L, hipStream_t stream) {
     a.w = static_cast<const T*>(L.w);
     a.copy_out = static_cast<T*>(L.copy_out);
     a.copy_col_stride = L.copy_col_stride;
+    a.copy_split = L.copy_split;
     a.mark = L.mark;
     a.mark_out = static_cast<T*>(L.mark_out);
     a.mark_stride = L.mark_stride;
@@ -1013,6 +1039,28 @@ void launch_gather_block(int scalar, const void* src, int64_t n, int64_t c0, int
 void launch_scatter_block(int scalar, const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst, hipStream_t s) {
     KPMX_DISPATCH(scalar, scatter_t, src, n, vw, c0, cols, dst, s)
 }
+void launch_zero_tails(uint16_t* base, int64_t rows, int64_t ld, int64_t n, hipStream_t s) {
+    if (ld <= n || rows <= 0) return;
+    hipLaunchKernelGGL(k_zero_tails, grid_for(rows * (ld - n)), dim3(256), 0, s, base, rows, ld, n);
+}
+
+template <class T>
+static void scatter_split_t(const void* src, int64_t n, int vw, int64_t cols, double scale, uint16_t* dst, int64_t plane,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_split<T>, grid_for(n * cols), dim3(256), 0, s, static_cast<const T*>(src), n, vw, cols,
+                       scale, dst, plane);
+}
+void launch_scatter_split(int scalar, const void* src, int64_t n, int vw, int64_t cols, double scale, uint16_t* dst,
+                          int64_t plane, hipStream_t s) {
+    switch (scalar) {
+        case 0: scatter_split_t<float>(src, n, vw, cols, scale, dst, plane, s); break;
+        case 1: scatter_split_t<double>(src, n, vw, cols, scale, dst, plane, s); break;
+        case 2: scatter_split_t<cf32>(src, n, vw, cols, scale, dst, plane, s); break;
+        case 3: scatter_split_t<cf64>(src, n, vw, cols, scale, dst, plane, s); break;
+        default: break;
+    }
+}
+
 void launch_scatter_scaled(int scalar, const void* src, int64_t n, int vw, int64_t cols, double scale, void* dst,
                            int64_t col_stride, hipStream_t s) {
     KPMX_DISPATCH(scalar, scatter_scaled_t, src, n, vw, cols, scale, dst, col_stride, s)
