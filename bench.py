@@ -208,7 +208,7 @@ def kubo_contraction(n1=724, n2=724, M=512, R=16):
     r0 = native.random_starters(n, R, "complex128", rmap)
     ctx = abi.Ctx(0)
     mat = ctx.mat(h2, fmt=1)
-    vm = ctx.mat(v)
+    vm = ctx.mat(v, fmt=1)
     blk = ctx.block(r0)
     log("kubo: %d sites (%.1fs setup)" % (n, time.time() - t0))
     out = {}

@@ -284,8 +284,8 @@ public:
                         idx_t num_moments, idx_t num_random, int mode) {
         kpmx_mat* h2 = matrix(oh, format);
         const idx_t n = oh.size();
-        auto vlr = upload(ctx(), oh.reorder(vl), KPMX_FMT_CSR);
-        auto vrr = upload(ctx(), oh.reorder(vr), KPMX_FMT_CSR);
+        auto vlr = upload(ctx(), oh.reorder(vl), format);
+        auto vrr = upload(ctx(), oh.reorder(vr), format);
         // Both starters are default-seeded random starters, so they draw the same vectors
         // (cppcore/src/kpm/Core.cpp:132-137); vl is applied on the device.
         auto st = random_starter(oh);

@@ -932,7 +932,7 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
                 void* L0 = ctx->aux2.get((size_t)n * g * sb);
 
                 // L0 = vl r
-                StepLaunch V = base_launch(vl, g, false);
+                StepLaunch V = base_launch(vl, g, true);   // SELL when the operator was uploaded so
                 V.mode = MODE_NONE;
                 V.sub_y = 0;
                 V.hscale = 1.0;
@@ -962,7 +962,7 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
                     },
                     [](StepLaunch const&, int64_t) {});
                 // R panel rows: vr T_m(H~) r (row 0 halved)
-                StepLaunch O = base_launch(vr, g, false);
+                StepLaunch O = base_launch(vr, g, true);
                 O.mode = MODE_NONE;
                 O.sub_y = 0;
                 O.yout = nullptr;
