@@ -176,12 +176,14 @@ def adjoint_cost(steps=2):
         grad_all, _ = abi.adjoint_diagonal(mat, tb.block, M, w, checkpoint=0)
     adj_all = tb.ctx.timer_stop() / steps
     assert np.all(np.isfinite(grad))
+    diff = float(np.abs(grad - grad_all).max() / np.abs(grad_all).max())
     mat.close()
     tb.close()
-    log("adjoint: forward %.1f ms, gradient %.1f ms (checkpoint 64) = %.2fx, %.1f ms storing all = %.2fx"
-        % (fwd, adj, adj / fwd, adj_all, adj_all / fwd))
+    log("adjoint: forward %.1f ms, gradient %.1f ms (checkpoint 64) = %.2fx, %.1f ms storing all = %.2fx, "
+        "max |g_ck - g_all| / max |g_all| = %.1e" % (fwd, adj, adj / fwd, adj_all, adj_all / fwd, diff))
     return {"forward_ms": fwd, "gradient_ms_checkpoint64": adj, "ratio_checkpoint64": adj / fwd,
             "gradient_ms_store_all": adj_all, "ratio_store_all": adj_all / fwd,
+            "checkpoint64_vs_store_all_max_rel_diff": diff,
             "workload": "strained graphene flake, %d sites, 1024 moments, 16 vectors, fp64" % n}
 
 
@@ -256,12 +258,17 @@ def plain_dos_flake(M=2050, R=8, reps=5):
     ctx = abi.Ctx(0)
     mat = ctx.mat(h2, fmt=1)
     blk = ctx.block(r0)
-    abi.moments_diagonal_run(mat, blk, M)
-    ctx.synchronize()
-    ctx.timer_start()
-    for _ in range(reps):
+    timings = {}
+    for policy, name in ((0, "eager"), (1, "graph")):
+        ctx.set_graph_policy(policy)
+        abi.moments_diagonal_run(mat, blk, M)  # the automatic policy captures on the second run
         abi.moments_diagonal_run(mat, blk, M)
-    gpu_ms = ctx.timer_stop() / reps
+        ctx.synchronize()
+        ctx.timer_start()
+        for _ in range(reps):
+            abi.moments_diagonal_run(mat, blk, M)
+        timings[name] = ctx.timer_stop() / reps
+    gpu_ms = timings["graph"]
     mu_gpu = abi.moments_diagonal_fetch(mat, blk, M).mean(axis=0)
     blk.close()
     mat.close()
@@ -272,6 +279,7 @@ def plain_dos_flake(M=2050, R=8, reps=5):
     err = float(np.abs(mu_gpu - mu_cpu).max() / abs(mu_cpu[0]))
     out = {"sites": n, "nnz": int(h2.nnz), "moments": M, "vectors": R,
            "gpu_ms": gpu_ms, "gpu_value": h2.nnz * M * R / (gpu_ms / 1000) / 1e9, "gpu_us_per_step": gpu_ms * 1000 / (M // 2),
+           "gpu_ms_step_launches": timings["eager"], "gpu_ms_graph_replay": timings["graph"],
            "cpu_ms": t_cpu * 1000, "cpu_value": h2.nnz * M * R / t_cpu / 1e9, "cpu_threads": threads,
            "max_rel_diff_vs_cpu": err, "unit": "GNNZ*moments/s",
            "workload": "graphene 40x40 nm flake (%d sites), fp64, 2050 moments, 8 vectors" % n}

@@ -48,6 +48,11 @@ extern "C" {
 
 #define KPMX_FMT_CSR 0  /* CSR, one wavefront per row group                                  */
 #define KPMX_FMT_SELL 1 /* sliced ELL with 64-row slices (single-vector recurrences)          */
+
+/* Graph policy of a context for the diagonal recurrence (kpmx_ctx_set_graph_policy). */
+#define KPMX_GRAPH_NEVER 0  /* launch every step from the host                                     */
+#define KPMX_GRAPH_AUTO 1   /* starter blocks <= 64 MiB: capture on the second identical run (default) */
+#define KPMX_GRAPH_ALWAYS 2 /* capture every recurrence                                             */
                         /* replaces kpm::MatrixFormat {CSR, ELL} (cppcore/include/kpm/Config.hpp:7) */
 
 #define KPMX_KUBO_FP64 0        /* mu_nm contraction in FP64 (reference arithmetic)             */
@@ -106,6 +111,13 @@ int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_
  * variant (optional): the chosen variant index, tile shape + 3 * grid fraction (0: full). */
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant);
 
+/* Graph policy (no counterpart in the reference, whose CPU loop has no launch cost): with
+ * KPMX_GRAPH_ALWAYS the diagonal recurrence of a starter block is captured into a HIP graph on its
+ * first run (KPMX_GRAPH_AUTO: on its second run) and replayed by later runs with the same matrix,
+ * number of moments, tuned variant and workspaces; the results are the same as with
+ * KPMX_GRAPH_NEVER. */
+int kpmx_ctx_set_graph_policy(kpmx_ctx* ctx, int policy);
+
 /* Same recurrence, results kept as raw per-step sums on the device; for timing loops.
  * Leaves the sums readable through kpmx_moments_diagonal_fetch. */
 int kpmx_moments_diagonal_run(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments);
@@ -141,7 +153,9 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
  * CSR order of the upload), by the reverse (adjoint) Chebyshev recurrence
  *   lambda_j = g_j + H2 lambda_{j+1} - lambda_{j+2},   dL/dH2 += lambda_{j+1} v_j^T,
  * fused with the gradient accumulation over the sparsity pattern.  checkpoint: steps between stored
- * vector pairs, the vectors in between recomputed; <= 0 keeps every vector (no recomputation).
+ * vector pairs, the vectors in between recomputed by running the recurrence backwards
+ * (v_{j-1} = H2 v_j - v_{j+1}) inside the adjoint steps, restarting from every stored pair; <= 0
+ * keeps every vector (no recomputation).
  * Real H2 (KPMX_F32 / KPMX_F64) uploaded with KPMX_FMT_SELL; at most 64 vectors.
  * mu (optional): host, num_vectors x num_moments, the forward moments. */
 int kpmx_adjoint_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, const double* w,

@@ -75,6 +75,10 @@ class Ctx:
         _check(lib().kpmx_ctx_timer_stop(self.h, ctypes.byref(ms)))
         return ms.value
 
+    def set_graph_policy(self, policy):
+        """Graph policy of the diagonal recurrence: 0 never, 1 auto (small starter blocks), 2 always."""
+        _check(lib().kpmx_ctx_set_graph_policy(self.h, int(policy)))
+
     def stream_probe(self, nbytes=1 << 31):
         """(copy GB/s, read GB/s) of the device's streaming probe (kpmx_stream_probe)."""
         c, r = ctypes.c_double(), ctypes.c_double()

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <limits>
 #include <random>
@@ -100,10 +101,12 @@ struct kpmx_ctx {
     Workspace partials, sums, work0, work1, aux0, aux1, aux2, panel_l, panel_r;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int tune[4][7] = {};  // tile-shape variant per (scalar, log2 vw), chosen by kpmx_autotune_diagonal
+    int graph_policy = KPMX_GRAPH_AUTO;
 };
 
 struct kpmx_mat {
     kpmx_ctx* ctx = nullptr;
+    uint64_t serial = 0;  // unique per upload: keys captured graphs (addresses can be reused)
     int scalar = 1;
     int format = KPMX_FMT_CSR;
     int64_t rows = 0, nnz = 0;
@@ -144,7 +147,19 @@ struct kpmx_block {
     // raw sums of the last kpmx_moments_diagonal_run, per tile: (M/2) x (4 vw) doubles
     std::vector<void*> sums;
     int64_t sums_moments = 0;
+    // captured diagonal recurrence and the launch state it was captured with
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key, last_key;
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_key.clear();
+    }
     ~kpmx_block() {
+        drop_graph();
         for (auto& t : tiles) dev_free(t.data);
         for (auto* s : sums) dev_free(s);
     }
@@ -268,6 +283,12 @@ int log2i(int v) {
     return l;
 }
 
+// Steps whose per-workgroup partial sums are buffered before one reduction launch (256 MiB cap).
+int64_t partials_chunk(size_t per_step_bytes, int64_t steps) {
+    const size_t budget = (size_t)256 << 20;
+    return std::max<int64_t>(1, std::min<int64_t>(steps, (int64_t)(budget / per_step_bytes)));
+}
+
 template <class PerStep, class PostStep>
 void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, const void* w, double* sums,
                     PerStep per_step, PostStep post_step, int tune = -1) {
@@ -287,8 +308,7 @@ void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, 
     double* partials = nullptr;
     if (mode != MODE_NONE) {
         const size_t per_step_bytes = (size_t)L.num_wg * width * sizeof(double);
-        const size_t budget = (size_t)256 << 20;
-        chunk = std::max<int64_t>(1, std::min<int64_t>(steps, (int64_t)(budget / per_step_bytes)));
+        chunk = partials_chunk(per_step_bytes, steps);
         partials = static_cast<double*>(ctx->partials.get(per_step_bytes * chunk));
     }
     int64_t chunk_start = 0;
@@ -359,22 +379,81 @@ void check_pair(const kpmx_mat* h2, const kpmx_block* r0) {
     KPMX_REQUIRE(h2->rows == r0->n, "Size mismatch between the matrix and the starter vectors");
 }
 
+void diag_enqueue(const kpmx_mat* h2, const kpmx_block* r0, int64_t steps) {
+    for (size_t i = 0; i < r0->tiles.size(); ++i) {
+        run_recurrence(h2, r0->tiles[i], MODE_DIAG, steps, nullptr, static_cast<double*>(r0->sums[i]),
+                       [](StepLaunch&, int64_t) {}, [](StepLaunch const&, int64_t) {});
+    }
+}
+
+// Small systems are launch-bound (a step of a 60k-site flake is a few microseconds of device
+// work), so their whole recurrence is captured once and replayed as one graph.
+bool diag_graph_candidate(const kpmx_mat* h2, const kpmx_block* r0) {
+    const int policy = h2->ctx->graph_policy;
+    if (policy == KPMX_GRAPH_NEVER) return false;
+    if (policy == KPMX_GRAPH_ALWAYS) return true;
+    return (size_t)r0->n * r0->nvec * scalar_bytes(h2->scalar) <= ((size_t)64 << 20);
+}
+
 void diag_run(const kpmx_mat* h2, const kpmx_block* r0c, int64_t M) {
     auto* r0 = const_cast<kpmx_block*>(r0c);
     check_pair(h2, r0);
     KPMX_REQUIRE(M >= 2 && M % 2 == 0, "num_moments must be even and >= 2");
+    kpmx_ctx* ctx = h2->ctx;
     const int64_t steps = M / 2;
     if (r0->sums_moments != M) {
-        if (!r0->sums.empty()) KPMX_HIP(hipStreamSynchronize(h2->ctx->stream));
+        if (!r0->sums.empty()) KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        r0->drop_graph();
         for (auto* s : r0->sums) dev_free(s);
         r0->sums.clear();
         for (auto& t : r0->tiles) r0->sums.push_back(dev_alloc((size_t)steps * 4 * t.vw * sizeof(double)));
         r0->sums_moments = M;
     }
-    for (size_t i = 0; i < r0->tiles.size(); ++i) {
-        run_recurrence(h2, r0->tiles[i], MODE_DIAG, steps, nullptr, static_cast<double*>(r0->sums[i]),
-                       [](StepLaunch&, int64_t) {}, [](StepLaunch const&, int64_t) {});
+    if (!diag_graph_candidate(h2, r0)) {
+        diag_enqueue(h2, r0, steps);
+        return;
     }
+    // Size the shared workspaces for every tile before capturing (nothing is allocated while the
+    // stream captures) and key the graph on everything its launches read.
+    const size_t sb = scalar_bytes(h2->scalar);
+    size_t work = 0, part = 0;
+    std::vector<uint64_t> key = {h2->serial, (uint64_t)M};
+    for (const Tile& t : r0->tiles) {
+        work = std::max(work, (size_t)h2->rows * t.vw * sb);
+        StepLaunch L = base_launch(h2, t.vw, true);
+        L.mode = MODE_DIAG;
+        L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
+        L.num_wg = step_grid(L);
+        const size_t per_step_bytes = (size_t)L.num_wg * 4 * t.vw * sizeof(double);
+        part = std::max(part, per_step_bytes * partials_chunk(per_step_bytes, steps));
+        key.push_back((uint64_t)L.tune);
+    }
+    key.push_back((uint64_t)(uintptr_t)ctx->work0.get(work));
+    key.push_back((uint64_t)(uintptr_t)ctx->work1.get(work));
+    key.push_back((uint64_t)(uintptr_t)ctx->partials.get(part));
+    if (ctx->graph_policy == KPMX_GRAPH_AUTO && r0->graph_key != key && r0->last_key != key) {
+        // automatic mode captures on the second run of the same recurrence: a one-off run
+        // does not pay for capture and instantiation
+        r0->last_key = key;
+        diag_enqueue(h2, r0, steps);
+        return;
+    }
+    if (!r0->graph_exec || r0->graph_key != key) {
+        r0->drop_graph();
+        KPMX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            diag_enqueue(h2, r0, steps);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(ctx->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        KPMX_HIP(hipStreamEndCapture(ctx->stream, &r0->graph));
+        KPMX_HIP(hipGraphInstantiate(&r0->graph_exec, r0->graph, nullptr, nullptr, 0));
+        r0->graph_key = key;
+    }
+    KPMX_HIP(hipGraphLaunch(r0->graph_exec, ctx->stream));
 }
 
 void diag_fetch(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, void* mu) {
@@ -543,6 +622,8 @@ int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t
         const size_t sb = scalar_bytes(scalar);
         auto m = std::make_unique<kpmx_mat>();
         m->ctx = ctx;
+        static std::atomic<uint64_t> next_serial{1};
+        m->serial = next_serial++;
         m->scalar = scalar;
         m->format = format;
         m->rows = rows;
@@ -656,6 +737,15 @@ int kpmx_block_destroy(kpmx_block* b) {
         (void)hipSetDevice(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
         delete b;
+    });
+}
+
+int kpmx_ctx_set_graph_policy(kpmx_ctx* ctx, int policy) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_REQUIRE(policy == KPMX_GRAPH_NEVER || policy == KPMX_GRAPH_AUTO || policy == KPMX_GRAPH_ALWAYS,
+                     "invalid graph policy");
+        ctx->graph_policy = policy;
     });
 }
 
@@ -1121,24 +1211,29 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
         kpmx_ctx* ctx = h2->ctx;
         KPMX_HIP(hipSetDevice(ctx->device));
         const Tile& t = r0->tiles.front();
-        const int64_t steps = 9;
+        // About 1 GB of vector traffic per timed sample (at least 9 steps): small systems get
+        // enough steps that launch jitter does not decide the choice.
+        const double step_bytes = 3.0 * (double)h2->rows * t.vw * scalar_bytes(h2->scalar);
+        const int64_t steps = std::max<int64_t>(9, std::min<int64_t>(101, (int64_t)(1e9 / step_bytes)));
         double* sums = static_cast<double*>(ctx->sums.get((size_t)steps * 4 * t.vw * sizeof(double)));
         if (!ctx->t0) KPMX_HIP(hipEventCreate(&ctx->t0));
         if (!ctx->t1) KPMX_HIP(hipEventCreate(&ctx->t1));
         int best = 0;
         float best_ms = 0;
         for (int v = 0; v < NUM_TUNE_VARIANTS; ++v) {
-            for (int rep = 0; rep < 2; ++rep) {  // first repetition warms caches and code
+            float ms_v = 0;
+            for (int rep = 0; rep < 3; ++rep) {  // the first repetition warms caches and code
                 KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
                 run_recurrence(h2, t, MODE_DIAG, steps, nullptr, sums, [](StepLaunch&, int64_t) {},
                                [](StepLaunch const&, int64_t) {}, v);
                 KPMX_HIP(hipEventRecord(ctx->t1, ctx->stream));
                 KPMX_HIP(hipEventSynchronize(ctx->t1));
+                float ms = 0;
+                KPMX_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+                if (rep == 1 || (rep > 1 && ms < ms_v)) ms_v = ms;
             }
-            float ms = 0;
-            KPMX_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
-            if (v == 0 || ms < best_ms) {
-                best_ms = ms;
+            if (v == 0 || ms_v < best_ms) {
+                best_ms = ms_v;
                 best = v;
             }
         }
@@ -1448,7 +1543,8 @@ int kpmx_adjoint_forward(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, in
         tp->C = (checkpoint <= 0 || checkpoint >= S) ? S : std::max<int64_t>(2, checkpoint);
         const int64_t C = tp->C;
         tp->store_all = (C >= S);
-        tp->nslots = tp->store_all ? S + 1 : C + 2;
+        // checkpointed: 3 rotating slots for the vectors the backward pass rebuilds, then v_S-1, v_S
+        tp->nslots = tp->store_all ? S + 1 : 5;
         tp->slots = static_cast<char*>(dev_alloc((size_t)tp->nslots * blk));
         tp->nck = tp->store_all ? 0 : (S + C - 1) / C;
         if (tp->nck) tp->cks = static_cast<char*>(dev_alloc((size_t)tp->nck * 2 * blk));
@@ -1476,6 +1572,11 @@ int kpmx_adjoint_forward(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, in
                                if (j % C == 1 && j / C < T->nck)
                                    KPMX_HIP(hipMemcpyAsync(T->cks + ((size_t)(j / C) * 2 + 1) * blk, L.yout, blk,
                                                            hipMemcpyDeviceToDevice, ctx->stream));
+                               if (k == S - 1) {  // the last pair starts the backward pass
+                                   KPMX_HIP(hipMemcpyAsync(T->slot(3), L.x, blk, hipMemcpyDeviceToDevice, ctx->stream));
+                                   KPMX_HIP(hipMemcpyAsync(T->slot(4), L.yout, blk, hipMemcpyDeviceToDevice,
+                                                           ctx->stream));
+                               }
                            });
         }
         std::vector<double> hs((size_t)S * 4 * vw);
@@ -1532,9 +1633,6 @@ int kpmx_adjoint_backward(kpmx_tape* T, const double* w, double* grad) {
         A.grad = dgrad;
         A.n = n;
         A.max_width = h2->max_width;
-        StepLaunch R = base_launch(h2, vw, true);
-        R.mode = MODE_NONE;
-        R.num_wg = step_grid(R);
         // With C_j = <v_j|H2|v_j>, B_0 = C_0 / 2 and B_j = C_j - B_j-1 (j >= 1, from
         // v_j+1 = H2 v_j - v_j-1), so L = sum_j alpha_j A_j + sum_{j>=1} gamma_j C_j + gamma_0 C_0 / 2
         // with gamma_j = sum_{i>=j} (-1)^(i-j) beta_i.  v_S no longer enters, and
@@ -1543,50 +1641,50 @@ int kpmx_adjoint_backward(kpmx_tape* T, const double* w, double* grad) {
         std::vector<double> gamma((size_t)S + 1, 0.0);
         for (int64_t j = S - 1; j >= 0; --j) gamma[(size_t)j] = beta[(size_t)j] - gamma[(size_t)j + 1];
 
-        // slots 0.. hold v_{s0}, v_{s0+1}, ... of the current segment (recomputed from its checkpoint)
-        auto recompute = [&](int64_t s0, int64_t s1) {
-            const int64_t c = s0 / C;
-            KPMX_HIP(hipMemcpyAsync(T->slot(0), T->cks + (size_t)c * 2 * blk, blk, hipMemcpyDeviceToDevice, ctx->stream));
-            KPMX_HIP(hipMemcpyAsync(T->slot(1), T->cks + ((size_t)c * 2 + 1) * blk, blk, hipMemcpyDeviceToDevice,
-                                    ctx->stream));
-            for (int64_t j = s0 + 1; j < s1; ++j) {  // v_{j+1} = H2 v_j - v_{j-1}, up to v_{s1}
-                R.x = T->slot(j - s0);
-                R.yin = T->slot(j - s0 - 1);
-                R.yout = T->slot(j - s0 + 1);
-                R.hscale = 1.0;
-                R.sub_y = 1;
-                launch(R, ctx->stream);
-            }
+        // Checkpointed tapes rebuild v_j backwards inside the adjoint steps: step j forms H2 v_j
+        // anyway, and v_j-1 = H2 v_j - v_j+1 holds for every j >= 1 (v_2 = H2 v_1 - v_0 included).
+        // The rebuilt vectors restart from the stored ones wherever the forward pass kept v_j
+        // (the last pair and the checkpoint pairs), so rounding drift spans at most one segment.
+        auto stored = [&](int64_t j) -> char* {
+            if (T->store_all) return T->slot(j);
+            if (j == S) return T->slot(4);
+            if (j == S - 1) return T->slot(3);
+            if (j / C < T->nck && (j % C == 0 || j % C == 1))
+                return T->cks + ((size_t)(j / C) * 2 + (size_t)(j % C)) * blk;
+            return nullptr;
+        };
+        auto vec = [&](int64_t j) -> char* {  // v_j: stored, or rebuilt in rotating slot j mod 3
+            char* p = stored(j);
+            return p ? p : T->slot(j % 3);
         };
         // adjoint steps j = S-1 .. 1 (lambda_S = lambda_S+1 = 0)
-        int64_t s1 = S - 1;
-        for (;;) {
-            const int64_t s0 = T->store_all ? 0 : ((s1 - 1) / C) * C;
-            if (!T->store_all) recompute(s0, s1);
-            const int64_t base_j = T->store_all ? 0 : s0;
-            for (int64_t j = s1; j > s0; --j) {
-                A.lam1 = lamB;
-                A.lam2 = lamA;
-                A.lam_out = lamA;
-                A.v_j = T->slot(j - base_j);
-                A.ca = 2.0 * alpha[(size_t)j];
-                A.chv = 2.0 * gamma[(size_t)j];
-                A.cg = gamma[(size_t)j];
-                A.gscale = 1.0;
-                A.update = 1;
-                if (launch_adjoint_step(A, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported adjoint configuration"};
-                KPMX_HIP(hipGetLastError());
-                std::swap(lamA, lamB);  // lamB <- lambda_j, lamA <- lambda_{j+1}
+        for (int64_t j = S - 1; j >= 1; --j) {
+            A.lam1 = lamB;
+            A.lam2 = lamA;
+            A.lam_out = lamA;
+            A.v_j = vec(j);
+            A.v_next = nullptr;
+            A.v_prev = nullptr;
+            if (!stored(j - 1)) {
+                A.v_next = vec(j + 1);
+                A.v_prev = T->slot((j - 1) % 3);
             }
-            if (s0 == 0) break;
-            s1 = s0;
+            A.ca = 2.0 * alpha[(size_t)j];
+            A.chv = 2.0 * gamma[(size_t)j];
+            A.cg = gamma[(size_t)j];
+            A.gscale = 1.0;
+            A.update = 1;
+            if (launch_adjoint_step(A, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported adjoint configuration"};
+            KPMX_HIP(hipGetLastError());
+            std::swap(lamA, lamB);  // lamB <- lambda_j, lamA <- lambda_{j+1}
         }
         // v_1 = 0.5 H2 v_0 and C_0 / 2:  grad += 0.5 (lambda_1 + gamma_0 v_0) v_0^T   (lamB holds lambda_1)
-        if (!T->store_all) recompute(0, 0);
+        A.v_next = nullptr;
+        A.v_prev = nullptr;
         A.lam1 = lamB;
         A.lam2 = lamA;
         A.lam_out = lamA;
-        A.v_j = T->slot(0);
+        A.v_j = vec(0);
         A.ca = A.chv = 0.0;
         A.cg = gamma[0];
         A.gscale = 0.5;

@@ -70,6 +70,9 @@ struct AdjointLaunch {
     const void* lam2 = nullptr;
     void* lam_out = nullptr;
     const void* v_j = nullptr;
+    // reverse recurrence (optional): v_prev = H2 v_j - v_next, from the product the step forms anyway
+    const void* v_next = nullptr;
+    void* v_prev = nullptr;
     double ca = 0, chv = 0, cg = 0, gscale = 1;
     int update = 1, max_width = 0;
     double* grad = nullptr;

@@ -508,6 +508,8 @@ struct AdjArgs {
     const T* lam2;
     T* lam_out;
     const T* v_j;
+    const T* v_next;  // with v_prev: v_prev = H2 v_j - v_next (the recurrence run backwards)
+    T* v_prev;
     double ca, chv, cg, gscale;
     int update;
     double* grad;
@@ -607,7 +609,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
         int rin[U];
         int64_t row[U];
         bool ok[U];
-        T l1[U], l2[U], vo[U];
+        T l1[U], l2[U], vo[U], vn[U];
         double gold[U][KMAX / 4];
         T lg[U][KMAX], vg[U][KMAX];
 #pragma unroll
@@ -619,6 +621,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
             l1[u] = lam1[e];
             vo[u] = vj[e];
             l2[u] = a.update ? nt_load(&a.lam2[e]) : zero(T{});
+            vn[u] = a.v_prev ? nt_load(&a.v_next[e]) : zero(T{});
             // the gradient slots this lane updates, loaded early
 #pragma unroll
             for (int k4 = 0; k4 < KMAX / 4; ++k4) {
@@ -672,6 +675,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
                 lnew = fmac((T)a.chv, acc_v, lnew);
                 nt_store(lnew, &a.lam_out[row[u] * VW + col]);
             }
+            if (a.v_prev && ok[u]) nt_store(sub(acc_v, vn[u]), &a.v_prev[row[u] * VW + col]);
         }
     }
 }
@@ -1165,6 +1169,8 @@ static void adjoint_t(const AdjointLaunch& A, hipStream_t stream) {
     a.lam2 = static_cast<const T*>(A.lam2);
     a.lam_out = static_cast<T*>(A.lam_out);
     a.v_j = static_cast<const T*>(A.v_j);
+    a.v_next = static_cast<const T*>(A.v_next);
+    a.v_prev = static_cast<T*>(A.v_prev);
     a.ca = A.ca;
     a.chv = A.chv;
     a.cg = A.cg;

@@ -90,6 +90,51 @@ def test_diagonal_block_vs_oracle(ctx, oracle, dtype, nvec, fmt):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
+def test_diagonal_graph_replays_match_eager(ctx, oracle, dtype):
+    """The captured recurrence against step-by-step launches, bit for bit, on every replay, with
+    eager work on the same stream and synchronisations between the replays; then a new number of
+    moments, a re-tuned variant and a new matrix each force a fresh capture."""
+    dtype = np.dtype(dtype)
+    h2 = graphene_h2(24, 20, dtype)
+    r0 = oracle.random_starter(h2.shape[0], 8, dtype)
+    mat = ctx.mat(h2, fmt=1)
+    blk = ctx.block(r0)
+    other = ctx.block(r0[:3])
+    try:
+        ctx.set_graph_policy(0)
+        eager = abi.moments_diagonal(mat, blk, 258)
+        eager_short = abi.moments_diagonal(mat, blk, 130)
+        eager_other = abi.moments_diagonal(mat, other, 130)
+        assert normwise(eager, oracle.diag_moments(h2, r0, 258)) <= TOL[dtype]
+        for policy in (2, 1):
+            ctx.set_graph_policy(policy)
+            for rep in range(4):
+                abi.moments_diagonal_run(mat, blk, 258)
+                abi.moments_diagonal_run(mat, other, 130)  # other work queued behind the replay
+                ctx.synchronize()
+                assert np.array_equal(abi.moments_diagonal_fetch(mat, blk, 258), eager), (policy, rep)
+                assert np.array_equal(abi.moments_diagonal_fetch(mat, other, 130), eager_other), (policy, rep)
+            assert np.array_equal(abi.moments_diagonal(mat, blk, 130), eager_short)
+            assert np.array_equal(abi.moments_diagonal(mat, blk, 258), eager)
+        abi.autotune_diagonal(mat, blk)
+        ctx.set_graph_policy(2)
+        tuned = abi.moments_diagonal(mat, blk, 258)
+        assert normwise(tuned, eager) <= TOL[dtype]
+        mat2 = ctx.mat(h2 * dtype.type(0.5), fmt=1)
+        try:
+            half = abi.moments_diagonal(mat2, blk, 130)
+            ctx.set_graph_policy(0)
+            assert np.array_equal(half, abi.moments_diagonal(mat2, blk, 130))
+        finally:
+            mat2.close()
+    finally:
+        ctx.set_graph_policy(1)
+        other.close()
+        blk.close()
+        mat.close()
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
 def test_diagonal_sell_single_vector(ctx, oracle, dtype):
     dtype = np.dtype(dtype)
     h2 = graphene_h2(40, 30, dtype)
