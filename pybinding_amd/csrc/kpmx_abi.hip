@@ -122,7 +122,9 @@ struct kpmx_mat {
     void* sell_vals = nullptr;
     int64_t sell_total = 0;
     std::vector<int64_t> sell_pos;  // SELL storage position of every CSR entry
+    mutable int64_t* d_sell_pos = nullptr;  // device copy, uploaded by the first adjoint pass
     ~kpmx_mat() {
+        dev_free(d_sell_pos);
         dev_free(indptr);
         dev_free(indices);
         dev_free(vals);
@@ -1691,10 +1693,17 @@ int kpmx_adjoint_backward(kpmx_tape* T, const double* w, double* grad) {
         A.update = 0;
         if (launch_adjoint_step(A, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported adjoint configuration"};
         KPMX_HIP(hipGetLastError());
-        std::vector<double> hg((size_t)gsize);
-        KPMX_HIP(hipMemcpyAsync(hg.data(), dgrad, hg.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        // SELL order -> CSR order on the device, then one download
+        if (!h2->d_sell_pos && h2->nnz) {
+            h2->d_sell_pos = static_cast<int64_t*>(dev_alloc((size_t)h2->nnz * sizeof(int64_t)));
+            KPMX_HIP(hipMemcpyAsync(h2->d_sell_pos, h2->sell_pos.data(), (size_t)h2->nnz * sizeof(int64_t),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        }
+        double* csr_grad = static_cast<double*>(ctx->aux1.get((size_t)std::max<int64_t>(h2->nnz, 1) * sizeof(double)));
+        launch_permute_f64(dgrad, h2->d_sell_pos, h2->nnz, csr_grad, ctx->stream);
+        KPMX_HIP(hipGetLastError());
+        KPMX_HIP(hipMemcpyAsync(grad, csr_grad, (size_t)h2->nnz * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         KPMX_HIP(hipStreamSynchronize(ctx->stream));
-        for (int64_t k = 0; k < h2->nnz; ++k) grad[k] = hg[(size_t)h2->sell_pos[(size_t)k]];
     });
 }
 

@@ -77,6 +77,7 @@ struct AdjointLaunch {
     int update = 1, max_width = 0;
     double* grad = nullptr;
     int64_t n = 0;
+    int tune = 0;  // tile shape + NUM_TILE_SHAPES * grid fraction, as for the step kernel
 };
 int launch_adjoint_step(const AdjointLaunch& A, hipStream_t stream);
 
@@ -111,6 +112,7 @@ int launch_gemm_nc(int scalar, int mode, const void* A, int64_t lda, const void*
 void launch_probe(int kind, const void* a, void* b, int64_t bytes, int grid, hipStream_t stream);
 void launch_zero(int scalar, void* dst, int64_t count, hipStream_t s);
 void launch_zero_f64(double* dst, int64_t count, hipStream_t s);
+void launch_permute_f64(const double* src, const int64_t* pos, int64_t count, double* dst, hipStream_t s);
 void launch_csr_matvec(int scalar, const int32_t* indptr, const int32_t* indices, const void* vals, const void* x,
                        int64_t n, void* y, hipStream_t s);
 

@@ -762,6 +762,12 @@ __global__ void k_fill(T* dst, int64_t count, T value) {
         dst[e] = value;
 }
 
+// dst[k] = src[pos[k]]: a permutation gather (SELL storage order -> CSR order)
+__global__ void k_permute_f64(const double* src, const int64_t* pos, int64_t count, double* dst) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = src[pos[e]];
+}
+
 // y = A x (plain CSR product, one vector), used for op * r and op^H beta preparation.
 template <class T>
 __global__ void k_csr_matvec(const int32_t* indptr, const int32_t* indices, const T* vals, const T* x, int64_t n, T* y) {
@@ -1071,6 +1077,10 @@ void launch_scatter_scaled(int scalar, const void* src, int64_t n, int vw, int64
 }
 void launch_zero(int scalar, void* dst, int64_t count, hipStream_t s) { KPMX_DISPATCH(scalar, zero_t, dst, count, s) }
 void launch_zero_f64(double* dst, int64_t count, hipStream_t s) { zero_t<double>(dst, count, s); }
+void launch_permute_f64(const double* src, const int64_t* pos, int64_t count, double* dst, hipStream_t s) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_permute_f64, grid_for(count), dim3(256), 0, s, src, pos, count, dst);
+}
 void launch_csr_matvec(int scalar, const int32_t* indptr, const int32_t* indices, const void* vals, const void* x, int64_t n,
                        void* y, hipStream_t s) {
     KPMX_DISPATCH(scalar, matvec_t, indptr, indices, vals, x, n, y, s)
@@ -1160,6 +1170,35 @@ void launch_reduce_segments(const double* partials, const int64_t* slot_first, i
     hipLaunchKernelGGL(k_reduce_segments, dim3((unsigned)g), dim3(256), 0, stream, partials, slot_first, num_sys, width, sums);
 }
 
+// Rows per wave per adjoint tile: 8 (two or more row groups in flight) up to 16 vectors, 4 above;
+// variants 1 and 2 halve and double it within [64 / VW, 16].
+template <int VW>
+constexpr int adjoint_rw(int tune) {
+    constexpr int RPW = WAVE / VW;
+    constexpr int base = RPW > 8 ? RPW : (VW <= 16 ? 8 : 4);
+    return tune == 1 ? (base / 2 >= RPW ? base / 2 : base) : tune == 2 ? (base * 2 <= 16 ? base * 2 : base) : base;
+}
+
+template <class T, int VW, int RW>
+static void adjoint_rw_launch(AdjArgs<T> a, const AdjointLaunch& A, hipStream_t stream) {
+    constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
+    a.num_tiles = (A.n + tr - 1) / tr;
+    const bool off32 = (uint64_t)A.n * VW * sizeof(T) < (1ull << 32);
+    const int shift = A.tune / NUM_TILE_SHAPES;
+    auto go = [&](auto kernel) {
+        int g = resident_grid(kernel, A.num_cu, a.num_tiles) >> shift;
+        if (g >= 8) g -= g % 8;
+        hipLaunchKernelGGL(kernel, dim3(g < 1 ? 1 : g), dim3(BLOCK), 0, stream, a, A.slice_ptr, A.slice_width);
+    };
+    if (A.max_width <= 4) {
+        if (off32) go(k_adjoint_sellb<T, VW, 4, RW, true>);
+        else go(k_adjoint_sellb<T, VW, 4, RW, false>);
+    } else {
+        if (off32) go(k_adjoint_sellb<T, VW, 8, RW, true>);
+        else go(k_adjoint_sellb<T, VW, 8, RW, false>);
+    }
+}
+
 template <class T, int VW>
 static void adjoint_t(const AdjointLaunch& A, hipStream_t stream) {
     AdjArgs<T> a{};
@@ -1178,21 +1217,10 @@ static void adjoint_t(const AdjointLaunch& A, hipStream_t stream) {
     a.update = A.update;
     a.grad = A.grad;
     a.n = A.n;
-    // rows per wave per tile: 8 (two or more row groups in flight) up to 16 vectors, 4 above
-    constexpr int RW = (WAVE / VW) > 8 ? (WAVE / VW) : (VW <= 16 ? 8 : 4);
-    constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
-    a.num_tiles = (A.n + tr - 1) / tr;
-    const bool off32 = (uint64_t)A.n * VW * sizeof(T) < (1ull << 32);
-    auto go = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, dim3(resident_grid(kernel, A.num_cu, a.num_tiles)), dim3(BLOCK), 0, stream, a,
-                           A.slice_ptr, A.slice_width);
-    };
-    if (A.max_width <= 4) {
-        if (off32) go(k_adjoint_sellb<T, VW, 4, RW, true>);
-        else go(k_adjoint_sellb<T, VW, 4, RW, false>);
-    } else {
-        if (off32) go(k_adjoint_sellb<T, VW, 8, RW, true>);
-        else go(k_adjoint_sellb<T, VW, 8, RW, false>);
+    switch (A.tune % NUM_TILE_SHAPES) {
+        case 1: adjoint_rw_launch<T, VW, adjoint_rw<VW>(1)>(a, A, stream); break;
+        case 2: adjoint_rw_launch<T, VW, adjoint_rw<VW>(2)>(a, A, stream); break;
+        default: adjoint_rw_launch<T, VW, adjoint_rw<VW>(0)>(a, A, stream); break;
     }
 }
 
