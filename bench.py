@@ -196,6 +196,37 @@ def adjoint_cost(steps=2):
             "workload": "strained graphene flake, %d sites, 1024 moments, 16 vectors, fp64" % n}
 
 
+def reconstruction(reps=5):
+    """Wall time of the reconstructions through the host core (host prep, device sums, transfers):
+    the DOS of the 40 nm flake's 2050 moments at 2000 energies, the LDOS of 64 columns there, and
+    KuboBastin of 512 x 512 moments at 1000 energy samples."""
+    from pybinding_amd._native import load_module
+    native = load_module()
+    rng = np.random.default_rng(1)
+    a, b = 8.686000303268429, 0.0
+    energy = list(np.linspace(-8.5, 8.5, 2000))
+    out = {}
+    for name, cols in (("dos_2050x2000_ms", 1), ("ldos_64cols_2050x2000_ms", 64)):
+        mu = rng.standard_normal((cols, 2050)) / (1.0 + np.arange(2050))
+        native.spectral_density(mu, energy, (a, b))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            native.spectral_density(mu, energy, (a, b))
+        out[name] = (time.perf_counter() - t0) / reps * 1e3
+    M = 512
+    mu = (rng.standard_normal((M, M)) + 1j * rng.standard_normal((M, M))) / np.outer(np.arange(1, M + 1), np.arange(1, M + 1))
+    samples = list(np.linspace(-8.5, 8.5, 1000))
+    chem = list(np.linspace(-1, 1, 50))
+    native.kubo_bastin(mu, chem, samples, 0.0, (a, b))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        native.kubo_bastin(mu, chem, samples, 0.0, (a, b))
+    out["kubo_bastin_512x512_1000_samples_ms"] = (time.perf_counter() - t0) / reps * 1e3
+    log("reconstruction: DOS %.2f ms, LDOS (64 columns) %.2f ms, KuboBastin %.2f ms"
+        % (out["dos_2050x2000_ms"], out["ldos_64cols_2050x2000_ms"], out["kubo_bastin_512x512_1000_samples_ms"]))
+    return out
+
+
 def kubo_contraction(n1=724, n2=724, M=512, R=16):
     """Kubo-Bastin mu_nm of a Peierls graphene supercell (B = 20 T, W = 1 eV, complex128),
     512 x 512 moments, 16 vectors: device time of the FP64 and FP32 split-precision MFMA paths,
@@ -469,6 +500,7 @@ def main():
         extra["adjoint_gradient"] = adjoint_cost()
     if not args.no_kubo:
         extra["kubo_contraction"] = kubo_contraction()
+        extra["reconstruction"] = reconstruction()
 
     cpu = None
     if not args.no_cpu_baseline:

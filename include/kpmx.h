@@ -194,6 +194,32 @@ int kpmx_batch_destroy(kpmx_batch* b);
 int kpmx_lanczos_bounds(const kpmx_mat* h, double precision_percent, double* min_energy, double* max_energy,
                         int* loops);
 
+/* Reconstruction sums (cppcore/include/kpm/reconstruct.hpp:16-143).  The caller forms the
+ * per-energy quantities in the moments' real type as the reference does (E~ = (E - b) / a,
+ * theta = acos E~ and the prefactor); the device evaluates the M-term sums of every energy and
+ * column.  real: KPMX_F32 or KPMX_F64, the type the sums run in.  Host buffers.
+ *
+ * SpectralDensity (reconstruct.hpp:16-54):
+ *   out[e * cols + c] = pref[e] * sum_n mu[c * M + n] cos(n theta[e])
+ * mu: cols x M reals; theta, pref: num_energy reals; out: num_energy x cols doubles. */
+int kpmx_chebyshev_cos_series(kpmx_ctx* ctx, int real, int64_t num_moments, int64_t cols, const void* mu,
+                              int64_t num_energy, const void* theta, const void* pref, double* out);
+/* GreensFunction (reconstruct.hpp:59-85):
+ *   out[e * cols + c] = pref[e] * sum_n mu[c * M + n] exp(-i n theta[e])
+ * mu: cols x M complex ({re, im} reals); pref: num_energy complex; out: num_energy x cols complex
+ * doubles. */
+int kpmx_chebyshev_exp_series(kpmx_ctx* ctx, int real, int64_t num_moments, int64_t cols, const void* mu,
+                              int64_t num_energy, const void* theta, const void* pref, double* out);
+/* KuboBastin energy-sample sums (reconstruct.hpp:90-143) in double precision:
+ *   out[q] = sum_nm mu[n * M + m] Gamma_nm(E_q) / (1 - E_q^2)^2,
+ *   Gamma_nm = g(n, m) + conj(g(m, n)),  g(n, m) = (E - i m sqrt(1 - E^2)) e^{i m acos E} T_n(E),
+ * evaluated as sum_n T_n(E) (mu a)_n + conj(a_n) (mu t)_n with a_m = (E - i m sqrt(1 - E^2))
+ * e^{i m acos E} and t_m = T_m(E).  mu_nm: M x M complex doubles (damped moments); samples: the
+ * scaled energies E_q; out: num_samples complex doubles.  The Fermi-Dirac integration over the
+ * samples stays with the caller. */
+int kpmx_kubo_bastin_sums(kpmx_ctx* ctx, int64_t num_moments, const double* mu_nm, int64_t num_samples,
+                          const double* samples, double* out);
+
 /* Streaming probe (no counterpart in the reference; the practical bandwidth reference for the
  * roofline reports): best GB/s over a few grid sizes of a non-temporal 16-byte copy of `bytes`
  * (bytes read + bytes written) and of a 16-byte read-only sweep, on buffers the call allocates

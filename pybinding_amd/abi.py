@@ -254,6 +254,41 @@ class Tape:
             self.h = ctypes.c_void_p()
 
 
+def chebyshev_series(ctx, mu, theta, pref):
+    """out[e, c] = pref[e] * sum_n mu[c, n] * f(n theta[e]) with f = cos for real mu and pref
+    (kpmx_chebyshev_cos_series) or exp(-i .) for complex ones (kpmx_chebyshev_exp_series); the sums
+    run in mu's real precision."""
+    mu = np.atleast_2d(np.ascontiguousarray(mu))
+    cplx = np.iscomplexobj(mu) or np.iscomplexobj(pref)
+    real = np.float32 if mu.real.dtype == np.float32 else np.float64
+    code = 0 if real == np.float32 else 1
+    cols, M = mu.shape
+    theta = np.ascontiguousarray(theta, dtype=real)
+    if cplx:
+        mu = np.ascontiguousarray(mu, dtype=np.complex64 if code == 0 else np.complex128)
+        pref = np.ascontiguousarray(pref, dtype=mu.dtype)
+        out = np.empty((len(theta), cols), dtype=np.complex128)
+        fn = lib().kpmx_chebyshev_exp_series
+    else:
+        mu = np.ascontiguousarray(mu, dtype=real)
+        pref = np.ascontiguousarray(pref, dtype=real)
+        out = np.empty((len(theta), cols), dtype=np.float64)
+        fn = lib().kpmx_chebyshev_cos_series
+    _check(fn(ctx.h, code, ctypes.c_int64(M), ctypes.c_int64(cols), _p(mu), ctypes.c_int64(len(theta)), _p(theta),
+              _p(pref), _p(out)))
+    return out
+
+
+def kubo_bastin_sums(ctx, mu_nm, samples):
+    """sum_nm mu_nm Gamma_nm(E_q) / (1 - E_q^2)^2 for every scaled sample E_q (kpmx_kubo_bastin_sums)."""
+    mu = np.ascontiguousarray(mu_nm, dtype=np.complex128)
+    samples = np.ascontiguousarray(samples, dtype=np.float64)
+    out = np.empty(len(samples), dtype=np.complex128)
+    _check(lib().kpmx_kubo_bastin_sums(ctx.h, ctypes.c_int64(mu.shape[0]), _p(mu), ctypes.c_int64(len(samples)),
+                                       _p(samples), _p(out)))
+    return out
+
+
 def adjoint_diagonal(mat, block, num_moments, w, checkpoint=64):
     """(dL/dH2, moments) in one call (kpmx_adjoint_diagonal)."""
     w = np.ascontiguousarray(w, dtype=np.float64)

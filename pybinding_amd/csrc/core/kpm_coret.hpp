@@ -91,7 +91,7 @@ public:
         timed_compute(&m, starter, cfg_.algorithm);
         apply_damping(m.data, cfg_.kernel, M, K);
         *cols_out = K;
-        return spectral_density(m.data, M, K, energy, scale);
+        return spectral_density(compute_.ctx(), m.data, M, K, energy, scale);
     }
 
     std::vector<double> dos(std::vector<double> const& energy, double broadening, idx_t num_random) override {
@@ -105,7 +105,7 @@ public:
         BatchDiagonalMoments<T> m{M, num_random, true, {}};
         timed_compute(&m, starter, alg);
         apply_damping(m.data, cfg_.kernel, M);
-        return spectral_density(m.data, M, 1, energy, scale);
+        return spectral_density(compute_.ctx(), m.data, M, 1, energy, scale);
     }
 
     // Undamped mean diagonal moments of `num_random` reference random starters (the DOS moments).
@@ -133,14 +133,14 @@ public:
             DiagonalMoments<T> m{M, {}};
             timed_compute(&m, unit_starter(oh_), cfg_.algorithm);
             apply_damping(m.data, cfg_.kernel, M);
-            return {greens_function(m.data, energy, scale)};
+            return {greens_function(compute_.ctx(), m.data, energy, scale)};
         }
         MultiUnitMoments<T> m{M, &oh_.optimized_idx, {}};
         timed_compute(&m, unit_starter(oh_), cfg_.algorithm);
         std::vector<std::vector<cd>> out;
         for (auto& d : m.data) {
             apply_damping(d, cfg_.kernel, M);
-            out.push_back(greens_function(d, energy, scale));
+            out.push_back(greens_function(compute_.ctx(), d, energy, scale));
         }
         return out;
     }
@@ -166,7 +166,7 @@ public:
         std::vector<double> samples((size_t)num_points);
         for (idx_t i = 0; i < num_points; ++i)
             samples[(size_t)i] = (num_points == 1) ? max_ : min_ + (max_ - min_) * (double)i / (double)(num_points - 1);
-        return kubo_bastin(mu, M, chemical_potential, samples, temperature, scale);
+        return kubo_bastin(compute_.ctx(), mu, M, chemical_potential, samples, temperature, scale);
     }
 
     void set_locality_order(std::vector<int32_t> order) override {

@@ -314,47 +314,52 @@ private:
 };
 
 // ------------------------------------------------------------------------------------- reconstruct
-// SpectralDensity (cppcore/include/kpm/reconstruct.hpp:16-54): out[e * cols + c]
+// SpectralDensity (cppcore/include/kpm/reconstruct.hpp:16-54): out[e * cols + c].  The scaled
+// energies, their arccosines and the prefactors are formed here in the moments' real type, as the
+// reference does; the E x M x cols cosine sums run on the device (kpmx_chebyshev_cos_series).
+template <class R> constexpr int real_code() { return std::is_same<R, float>::value ? KPMX_F32 : KPMX_F64; }
+
 template <class T>
-std::vector<double> spectral_density(std::vector<T> const& moments, idx_t M, idx_t cols, std::vector<double> const& energy,
-                                     Scale<> s) {
+std::vector<double> spectral_density(kpmx_ctx* ctx, std::vector<T> const& moments, idx_t M, idx_t cols,
+                                     std::vector<double> const& energy, Scale<> s) {
     using R = real_t<T>;
     const Scale<R> scale(s);
     const R k = R(2 / PI_F) / scale.a;
-    std::vector<double> out(energy.size() * (size_t)cols);
-    std::vector<R> cosn((size_t)M);
+    std::vector<R> mu(moments.size()), th(energy.size()), pref(energy.size());
+    for (size_t i = 0; i < mu.size(); ++i) mu[i] = (R)re(moments[i]);
     for (size_t i = 0; i < energy.size(); ++i) {
         const R E = scale((R)energy[i]);
-        const R ac = std::acos(E);
-        for (idx_t n = 0; n < M; ++n) cosn[(size_t)n] = std::cos((R)n * ac);
-        const R pref = k / std::sqrt(1 - E * E);
-        for (idx_t c = 0; c < cols; ++c) {
-            R sum = 0;
-            for (idx_t n = 0; n < M; ++n) sum += (R)re(moments[(size_t)(c * M + n)]) * cosn[(size_t)n];
-            out[i * (size_t)cols + (size_t)c] = (double)(pref * sum);
-        }
+        th[i] = std::acos(E);
+        pref[i] = k / std::sqrt(1 - E * E);
     }
+    std::vector<double> out(energy.size() * (size_t)cols);
+    check(kpmx_chebyshev_cos_series(ctx, real_code<R>(), M, cols, mu.data(), (int64_t)energy.size(), th.data(),
+                                    pref.data(), out.data()));
     return out;
 }
 
-// GreensFunction (reconstruct.hpp:59-85)
+// GreensFunction (reconstruct.hpp:59-85); the exponential sums run on the device
+// (kpmx_chebyshev_exp_series).
 template <class T>
-std::vector<cd> greens_function(std::vector<T> const& moments, std::vector<double> const& energy, Scale<> s) {
+std::vector<cd> greens_function(kpmx_ctx* ctx, std::vector<T> const& moments, std::vector<double> const& energy,
+                                Scale<> s) {
     using R = real_t<T>;
     using C = std::complex<R>;
     const Scale<R> scale(s);
     const C i1(0, 1);
     const C k = -R(2) * i1 / scale.a;
     const idx_t M = (idx_t)moments.size();
-    std::vector<cd> out(energy.size());
+    std::vector<C> mu((size_t)M), pref(energy.size());
+    std::vector<R> th(energy.size());
+    for (idx_t n = 0; n < M; ++n) mu[(size_t)n] = C(moments[(size_t)n]);
     for (size_t i = 0; i < energy.size(); ++i) {
         const R E = scale((R)energy[i]);
-        const R ac = std::acos(E);
-        C sum(0, 0);
-        for (idx_t n = 0; n < M; ++n) sum += C(moments[(size_t)n]) * std::exp(-i1 * (R)n * ac);
-        const C g = k / std::sqrt(1 - E * E) * sum;
-        out[i] = cd(g.real(), g.imag());
+        th[i] = std::acos(E);
+        pref[i] = k / std::sqrt(1 - E * E);
     }
+    std::vector<cd> out(energy.size());
+    check(kpmx_chebyshev_exp_series(ctx, real_code<R>(), M, 1, mu.data(), (int64_t)energy.size(), th.data(),
+                                    pref.data(), reinterpret_cast<double*>(out.data())));
     return out;
 }
 
@@ -363,12 +368,14 @@ std::vector<cd> greens_function(std::vector<T> const& moments, std::vector<doubl
 // g(n,m) = (E - i m sqrt(1-E^2)) e^{i m acos E} T_n(E) for every energy sample, i.e. M^2 complex
 // exponentials per sample.  g factorises into T_n(E) a_m(E) with a_m = (E - i m sqrt(1-E^2))
 // e^{i m acos E}, so
-//     sum_nm mu_nm Gamma_nm = sum_n T_n (mu a)_n + sum_m T_m (mu^T conj(a))_m,
-// two matrix-vector products per sample and 2M transcendental calls instead of M^2.  The sums run
-// in double precision (the reference uses the model's real type); samples are spread over threads.
+//     sum_nm mu_nm Gamma_nm = sum_n T_n (mu a)_n + sum_n conj(a_n) (mu t)_n,
+// two matrix-vector products per sample, formed on the device for all samples at once
+// (kpmx_kubo_bastin_sums) in double precision (the reference uses the model's real type).  The
+// Fermi-Dirac trapezoid over the samples stays here.
 template <class T>
-std::vector<cd> kubo_bastin(std::vector<T> const& mu, idx_t M, std::vector<double> const& chemical_potential,
-                            std::vector<double> const& energy_samples, double temperature, Scale<> s) {
+std::vector<cd> kubo_bastin(kpmx_ctx* ctx, std::vector<T> const& mu, idx_t M,
+                            std::vector<double> const& chemical_potential, std::vector<double> const& energy_samples,
+                            double temperature, Scale<> s) {
     using R = real_t<T>;
     const Scale<R> scale(s);
     const double inv_kbt = (double)static_cast<R>(scale.a / (KB_F * temperature));
@@ -377,44 +384,9 @@ std::vector<cd> kubo_bastin(std::vector<T> const& mu, idx_t M, std::vector<doubl
     std::vector<cd> mud((size_t)(M * M));
     for (size_t i = 0; i < mud.size(); ++i) mud[i] = cd(mu[i]);
     std::vector<cd> sum_nm(es.size());
-    auto work = [&](size_t q0, size_t q1) {
-        std::vector<double> t((size_t)M);
-        std::vector<cd> a((size_t)M), ac((size_t)M), y((size_t)M);
-        for (size_t q = q0; q < q1; ++q) {
-            const double en = es[q];
-            const double th = std::acos(en);
-            const double sq = std::sqrt(1.0 - en * en);
-            for (idx_t m = 0; m < M; ++m) {
-                t[(size_t)m] = std::cos(th * (double)m);
-                a[(size_t)m] = cd(en, -(double)m * sq) * std::exp(cd(0.0, th * (double)m));
-                ac[(size_t)m] = std::conj(a[(size_t)m]);
-                y[(size_t)m] = cd(0, 0);
-            }
-            cd s1(0, 0);
-            for (idx_t n = 0; n < M; ++n) {
-                const cd* row = &mud[(size_t)(n * M)];
-                cd acc(0, 0);
-                const cd w = ac[(size_t)n];
-                const double tn = t[(size_t)n];
-                for (idx_t m = 0; m < M; ++m) {
-                    acc += row[m] * a[(size_t)m];
-                    y[(size_t)m] += row[m] * w;   // (mu^T conj(a))_m
-                }
-                s1 += tn * acc;
-            }
-            cd s2(0, 0);
-            for (idx_t m = 0; m < M; ++m) s2 += t[(size_t)m] * y[(size_t)m];
-            const double k = 1.0 / ((1.0 - en * en) * (1.0 - en * en));
-            sum_nm[q] = k * (s1 + s2);
-        }
-    };
+    check(kpmx_kubo_bastin_sums(ctx, M, reinterpret_cast<const double*>(mud.data()), (int64_t)es.size(), es.data(),
+                                reinterpret_cast<double*>(sum_nm.data())));
     const size_t P = es.size();
-    const size_t nt = std::max<size_t>(1, std::min<size_t>({(size_t)std::max(1u, std::thread::hardware_concurrency()),
-                                                            (size_t)16, P}));
-    std::vector<std::thread> pool;
-    for (size_t i = 0; i < nt; ++i) pool.emplace_back(work, P * i / nt, P * (i + 1) / nt);
-    for (auto& th : pool) th.join();
-
     double emin = *std::min_element(es.begin(), es.end()), emax = *std::max_element(es.begin(), es.end());
     const double coeff = (emax - emin) / static_cast<double>(2 * es.size());
     const double prefix = 4.0 / ((double)scale.a * (double)scale.a);

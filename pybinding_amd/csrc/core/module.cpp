@@ -266,12 +266,65 @@ PYBIND11_MODULE(_kpmx, m) {
         },
         py::arg("n"), py::arg("nvec"), py::arg("dtype"), py::arg("reorder_map") = std::vector<int32_t>{});
 
-    // The host reconstructions on their own (the KPM methods call them after the moments):
-    // KuboBastin of damped M x M moments, SpectralDensity and GreensFunction of damped moments.
+    // The reconstructions on their own (the KPM methods call them after the moments): KuboBastin
+    // of damped M x M moments, SpectralDensity and GreensFunction of damped moments; the sums run
+    // on `device`.
+    auto real_or_complex = [](py::array mu, auto run) -> py::array {
+        auto dt = py::dtype::from_args(mu.attr("dtype"));
+        if (dt.kind() == 'f' && dt.itemsize() == 4) return run(float{});
+        if (dt.kind() == 'f') return run(double{});
+        if (dt.itemsize() == 8) return run(std::complex<float>{});
+        return run(cd{});
+    };
+    m.def(
+        "spectral_density",
+        [real_or_complex](py::array mu, std::vector<double> energy, std::pair<double, double> ab, int device) {
+            return real_or_complex(mu, [&](auto tag) -> py::array {
+                using T = decltype(tag);
+                auto a = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(mu);
+                if (!a || a.ndim() != 2) throw std::invalid_argument("mu must be cols x M");
+                const idx_t cols = a.shape(0), M = a.shape(1);
+                std::vector<T> v(a.data(), a.data() + cols * M);
+                Scale<> sc;
+                sc.a = ab.first;
+                sc.b = ab.second;
+                auto dev = Device::instance(device);
+                std::vector<double> out;
+                {
+                    py::gil_scoped_release nogil;
+                    out = spectral_density(dev->get(), v, M, cols, energy, sc);
+                }
+                py::array_t<double> r({(py::ssize_t)energy.size(), (py::ssize_t)cols});
+                std::copy(out.begin(), out.end(), r.mutable_data());
+                return r;
+            });
+        },
+        py::arg("mu"), py::arg("energy"), py::arg("scale"), py::arg("device") = 0);
+    m.def(
+        "greens_function",
+        [real_or_complex](py::array mu, std::vector<double> energy, std::pair<double, double> ab, int device) {
+            return real_or_complex(mu, [&](auto tag) -> py::array {
+                using T = decltype(tag);
+                auto a = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(mu);
+                if (!a || a.ndim() != 1) throw std::invalid_argument("mu must be a vector");
+                std::vector<T> v(a.data(), a.data() + a.shape(0));
+                Scale<> sc;
+                sc.a = ab.first;
+                sc.b = ab.second;
+                auto dev = Device::instance(device);
+                std::vector<cd> out;
+                {
+                    py::gil_scoped_release nogil;
+                    out = greens_function(dev->get(), v, energy, sc);
+                }
+                return py::array_t<cd>((py::ssize_t)out.size(), out.data());
+            });
+        },
+        py::arg("mu"), py::arg("energy"), py::arg("scale"), py::arg("device") = 0);
     m.def(
         "kubo_bastin",
         [](py::array mu, std::vector<double> chemical_potential, std::vector<double> samples, double temperature,
-           std::pair<double, double> ab) -> py::array {
+           std::pair<double, double> ab, int device) -> py::array {
             auto run = [&](auto tag) -> py::array {
                 using T = decltype(tag);
                 auto a = py::array_t<T, py::array::c_style | py::array::forcecast>::ensure(mu);
@@ -281,10 +334,11 @@ PYBIND11_MODULE(_kpmx, m) {
                 Scale<> sc;
                 sc.a = ab.first;
                 sc.b = ab.second;
+                auto dev = Device::instance(device);
                 std::vector<cd> out;
                 {
                     py::gil_scoped_release nogil;
-                    out = kubo_bastin(v, M, chemical_potential, samples, temperature, sc);
+                    out = kubo_bastin(dev->get(), v, M, chemical_potential, samples, temperature, sc);
                 }
                 return py::array_t<cd>((py::ssize_t)out.size(), out.data());
             };
@@ -295,5 +349,5 @@ PYBIND11_MODULE(_kpmx, m) {
             return run(cd{});
         },
         py::arg("mu"), py::arg("chemical_potential"), py::arg("energy_samples"), py::arg("temperature"),
-        py::arg("scale"));
+        py::arg("scale"), py::arg("device") = 0);
 }

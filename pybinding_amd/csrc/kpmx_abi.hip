@@ -9,6 +9,7 @@
 #include <limits>
 #include <random>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <string>
 #include <vector>
@@ -546,6 +547,42 @@ void tridiagonal_minmax(const std::vector<double>& a, const std::vector<double>&
 }  // namespace
 
 // ==================================================================================== C-ABI
+namespace {
+// Upload pieces into one workspace, each at a 256-byte aligned offset; returns their device addresses.
+std::vector<char*> upload_pieces(kpmx_ctx* ctx, Workspace& ws, std::initializer_list<std::pair<const void*, size_t>> parts) {
+    size_t total = 0;
+    for (auto& p : parts) total += (p.second + 255) / 256 * 256;
+    char* base = static_cast<char*>(ws.get(std::max<size_t>(total, 256)));
+    std::vector<char*> out;
+    size_t off = 0;
+    for (auto& p : parts) {
+        out.push_back(base + off);
+        if (p.first && p.second)  // a null source reserves space only
+            KPMX_HIP(hipMemcpyAsync(base + off, p.first, p.second, hipMemcpyHostToDevice, ctx->stream));
+        off += (p.second + 255) / 256 * 256;
+    }
+    return out;
+}
+
+void cheb_series(kpmx_ctx* ctx, int real, bool exp_series, int64_t M, int64_t cols, const void* mu, int64_t E,
+                 const void* theta, const void* pref, double* out) {
+    KPMX_REQUIRE(ctx && mu && theta && pref && out, "null argument");
+    KPMX_REQUIRE(real == KPMX_F32 || real == KPMX_F64, "real must be KPMX_F32 or KPMX_F64");
+    KPMX_REQUIRE(M >= 1 && cols >= 1 && E >= 0, "empty series");
+    KPMX_HIP(hipSetDevice(ctx->device));
+    if (E == 0) return;
+    const size_t rb = real == KPMX_F32 ? 4 : 8, cw = exp_series ? 2 : 1;
+    auto d = upload_pieces(ctx, ctx->aux0,
+                           {{mu, (size_t)(cols * M) * cw * rb}, {theta, (size_t)E * rb}, {pref, (size_t)E * cw * rb}});
+    double* dout = static_cast<double*>(ctx->aux1.get((size_t)(E * cols) * cw * sizeof(double)));
+    if (launch_cheb_series(real, exp_series, d[0], M, cols, E, d[1], d[2], dout, ctx->stream) != 0)
+        throw Error{KPMX_EINVAL, "too many energies or columns for one call"};
+    KPMX_HIP(hipGetLastError());
+    KPMX_HIP(hipMemcpyAsync(out, dout, (size_t)(E * cols) * cw * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+}
+}  // namespace
+
 extern "C" {
 
 const char* kpmx_last_error(void) { return g_last_error.c_str(); }
@@ -1726,6 +1763,43 @@ int kpmx_adjoint_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, c
     kpmx_tape_destroy(T);
     if (rc != KPMX_OK) g_last_error = err;
     return rc;
+}
+
+
+int kpmx_chebyshev_cos_series(kpmx_ctx* ctx, int real, int64_t num_moments, int64_t cols, const void* mu,
+                              int64_t num_energy, const void* theta, const void* pref, double* out) {
+    return guarded([&] { cheb_series(ctx, real, false, num_moments, cols, mu, num_energy, theta, pref, out); });
+}
+
+int kpmx_chebyshev_exp_series(kpmx_ctx* ctx, int real, int64_t num_moments, int64_t cols, const void* mu,
+                              int64_t num_energy, const void* theta, const void* pref, double* out) {
+    return guarded([&] { cheb_series(ctx, real, true, num_moments, cols, mu, num_energy, theta, pref, out); });
+}
+
+int kpmx_kubo_bastin_sums(kpmx_ctx* ctx, int64_t num_moments, const double* mu_nm, int64_t num_samples,
+                          const double* samples, double* out) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx && mu_nm && samples && out, "null argument");
+        KPMX_REQUIRE(num_moments >= 1 && num_samples >= 0, "empty moments");
+        KPMX_REQUIRE(num_samples <= 0x7fffffff, "too many energy samples for one call");
+        KPMX_HIP(hipSetDevice(ctx->device));
+        if (num_samples == 0) return;
+        const int64_t M = num_moments, P = num_samples;
+        const int groups = kubo_sums_groups(M, P);
+        auto d = upload_pieces(ctx, ctx->aux0, {{mu_nm, (size_t)(M * M) * 2 * sizeof(double)},
+                                                {samples, (size_t)P * sizeof(double)}});
+        auto w = upload_pieces(ctx, ctx->aux1, {{nullptr, (size_t)(P * M) * sizeof(double)},
+                                                {nullptr, (size_t)(P * M) * 2 * sizeof(double)},
+                                                {nullptr, (size_t)(P * groups) * 2 * sizeof(double)},
+                                                {nullptr, (size_t)P * 2 * sizeof(double)}});
+        double* dout = reinterpret_cast<double*>(w[3]);
+        launch_kubo_sums(reinterpret_cast<const double*>(d[0]), M, reinterpret_cast<const double*>(d[1]), P,
+                         reinterpret_cast<double*>(w[0]), reinterpret_cast<double*>(w[1]),
+                         reinterpret_cast<double*>(w[2]), groups, dout, ctx->stream);
+        KPMX_HIP(hipGetLastError());
+        KPMX_HIP(hipMemcpyAsync(out, dout, (size_t)P * 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    });
 }
 
 int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* read_gbps) {

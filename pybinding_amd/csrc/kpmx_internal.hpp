@@ -108,6 +108,16 @@ int launch_gemm_nc_planes(bool cplx, const uint16_t* A, const uint16_t* B, int64
                           int64_t kdim, double* C, double* work, int num_cu, hipStream_t s);
 int launch_gemm_nc(int scalar, int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M,
                    int64_t kdim, double* C, double* work, int num_cu, hipStream_t s);
+// Reconstruction sums (kpmx_reconstruct.hip).  real: 0 float, 1 double.  Chebyshev series
+// out[e][c] = pref[e] sum_n mu[c][n] cos(n theta[e]) (exp_series: mu, pref complex and
+// exp(-i n theta[e]) in place of the cosine); returns -1 for an unsupported size or type.
+int launch_cheb_series(int real, bool exp_series, const void* mu, int64_t M, int64_t cols, int64_t num_energy,
+                       const void* theta, const void* pref, double* out, hipStream_t s);
+// KuboBastin sample sums out[q] (complex) from the M x M complex moments; t (P M doubles),
+// a (2 P M doubles) and partial (2 P groups doubles) are workspaces.
+int kubo_sums_groups(int64_t M, int64_t P);
+void launch_kubo_sums(const double* mu, int64_t M, const double* samples, int64_t P, double* t, double* a,
+                      double* partial, int groups, double* out, hipStream_t s);
 // streaming probes: kind 0 copies `bytes` from a to b, kind 1 reads a
 void launch_probe(int kind, const void* a, void* b, int64_t bytes, int grid, hipStream_t stream);
 void launch_zero(int scalar, void* dst, int64_t count, hipStream_t s);
