@@ -50,8 +50,8 @@ extern "C" {
 #define KPMX_FMT_SELL 1 /* sliced ELL with 64-row slices (single-vector recurrences)          */
 
 /* Graph policy of a context for the diagonal recurrence (kpmx_ctx_set_graph_policy). */
-#define KPMX_GRAPH_NEVER 0  /* launch every step from the host                                     */
-#define KPMX_GRAPH_AUTO 1   /* starter blocks <= 64 MiB: capture on the second identical run (default) */
+#define KPMX_GRAPH_NEVER 0  /* launch every step from the host (default)                           */
+#define KPMX_GRAPH_AUTO 1   /* starter blocks <= 64 MiB: capture on the second identical run          */
 #define KPMX_GRAPH_ALWAYS 2 /* capture every recurrence                                             */
                         /* replaces kpm::MatrixFormat {CSR, ELL} (cppcore/include/kpm/Config.hpp:7) */
 
@@ -115,7 +115,9 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
  * KPMX_GRAPH_ALWAYS the diagonal recurrence of a starter block is captured into a HIP graph on its
  * first run (KPMX_GRAPH_AUTO: on its second run) and replayed by later runs with the same matrix,
  * number of moments, tuned variant and workspaces; the results are the same as with
- * KPMX_GRAPH_NEVER. */
+ * KPMX_GRAPH_NEVER.  Capture and instantiation cost host time on the run that captures; on the
+ * engine's kernels replay has not been faster than per-step launches, so the default is
+ * KPMX_GRAPH_NEVER and the graph path is for callers whose host thread is busy. */
 int kpmx_ctx_set_graph_policy(kpmx_ctx* ctx, int policy);
 
 /* Same recurrence, results kept as raw per-step sums on the device; for timing loops.

@@ -76,7 +76,8 @@ class Ctx:
         return ms.value
 
     def set_graph_policy(self, policy):
-        """Graph policy of the diagonal recurrence: 0 never, 1 auto (small starter blocks), 2 always."""
+        """Graph policy of the diagonal recurrence: 0 never (default), 1 auto (small starter blocks,
+        captured on the second identical run), 2 always."""
         _check(lib().kpmx_ctx_set_graph_policy(self.h, int(policy)))
 
     def stream_probe(self, nbytes=1 << 31):

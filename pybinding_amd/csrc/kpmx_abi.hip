@@ -102,7 +102,7 @@ struct kpmx_ctx {
     Workspace partials, sums, work0, work1, aux0, aux1, aux2, panel_l, panel_r;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int tune[4][7] = {};  // tile-shape variant per (scalar, log2 vw), chosen by kpmx_autotune_diagonal
-    int graph_policy = KPMX_GRAPH_AUTO;
+    int graph_policy = KPMX_GRAPH_NEVER;
 };
 
 struct kpmx_mat {

@@ -128,7 +128,7 @@ def test_diagonal_graph_replays_match_eager(ctx, oracle, dtype):
         finally:
             mat2.close()
     finally:
-        ctx.set_graph_policy(1)
+        ctx.set_graph_policy(0)
         other.close()
         blk.close()
         mat.close()
