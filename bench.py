@@ -313,6 +313,18 @@ def plain_dos_flake(M=2050, R=8, reps=5):
     blk.close()
     mat.close()
     ctx.close()
+    # the drop-in end to end: pb.kpm(model).calc_dos at 2000 energies with the broadening that gives
+    # 2050 moments (Jackson), 8 random vectors -- BFS reorder, starters, moments, damping and
+    # reconstruction through the facade, the model already built
+    from pybinding_amd import chebyshev
+    kp = chebyshev.kpm(m, energy_range=(-8.6, 8.6), kernel=chebyshev.jackson_kernel(), silent=True)
+    broadening = 8.686000303268429 * np.pi / 2048.5
+    energy = np.linspace(-8.5, 8.5, 2000)
+    kp.calc_dos(energy, broadening, num_random=R)
+    t0 = time.perf_counter()
+    kp.calc_dos(energy, broadening, num_random=R)
+    facade_ms = (time.perf_counter() - t0) * 1e3
+    assert kp.stats.num_moments == M, kp.stats.num_moments
     threads = max(1, min(16, os.cpu_count() or 1))
     mu_cpu, t_cpu = oracle.cpu_baseline_dos(oracle.to_ell(h2), n, M, R, threads, perm=rmap, borders=borders,
                                             interleaved=True)
@@ -320,11 +332,12 @@ def plain_dos_flake(M=2050, R=8, reps=5):
     out = {"sites": n, "nnz": int(h2.nnz), "moments": M, "vectors": R,
            "gpu_ms": gpu_ms, "gpu_value": h2.nnz * M * R / (gpu_ms / 1000) / 1e9, "gpu_us_per_step": gpu_ms * 1000 / (M // 2),
            "gpu_ms_step_launches": timings["eager"], "gpu_ms_graph_replay": timings["graph"],
+           "facade_calc_dos_ms": facade_ms,
            "cpu_ms": t_cpu * 1000, "cpu_value": h2.nnz * M * R / t_cpu / 1e9, "cpu_threads": threads,
            "max_rel_diff_vs_cpu": err, "unit": "GNNZ*moments/s",
            "workload": "graphene 40x40 nm flake (%d sites), fp64, 2050 moments, 8 vectors" % n}
-    log("plain DOS flake: GPU %.2f ms (%.1f us/step), CPU %.1f ms on %d threads, max diff %.2e"
-        % (gpu_ms, out["gpu_us_per_step"], t_cpu * 1000, threads, err))
+    log("plain DOS flake: GPU %.2f ms (%.1f us/step), CPU %.1f ms on %d threads, max diff %.2e; "
+        "kpm(model).calc_dos end to end %.1f ms" % (gpu_ms, out["gpu_us_per_step"], t_cpu * 1000, threads, err, facade_ms))
     return out
 
 

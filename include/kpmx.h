@@ -74,7 +74,9 @@ int kpmx_device_count(int* count);
 
 /* A device context: one GPU, one stream, reusable workspaces.
  * Replaces the reference's DefaultCompute object (thread pool + SIMD batching),
- * cppcore/src/kpm/default/Compute.cpp:121-126. */
+ * cppcore/src/kpm/default/Compute.cpp:121-126.
+ * Matrices, starter blocks, tapes and batches belong to the context they were made on: destroy
+ * them before destroying the context. */
 int kpmx_ctx_create(int device, kpmx_ctx** out);
 int kpmx_ctx_destroy(kpmx_ctx* ctx);
 int kpmx_ctx_synchronize(kpmx_ctx* ctx);

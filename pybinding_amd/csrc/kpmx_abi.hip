@@ -842,7 +842,6 @@ int kpmx_moments_generic(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, co
         KPMX_HIP(hipMemcpyAsync(dw, wbuf.data(), wbuf.size(), hipMemcpyHostToDevice, ctx->stream));
         // one-vector tile view of the first starter
         const Tile& first = r0->tiles.front();
-        KPMX_REQUIRE(first.vw == 1 || r0->nvec == 1 || true, "");
         Tile t;
         t.vw = 1;
         t.cols = 1;
