@@ -87,6 +87,13 @@ struct Workspace {
         }
         return ptr;
     }
+    // give the memory back now (queued kernels that read it are drained first)
+    void release() {
+        if (ptr) (void)hipDeviceSynchronize();
+        dev_free(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
     ~Workspace() { dev_free(ptr); }
 };
 
@@ -1017,10 +1024,13 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
         const bool cplx = is_complex(sc);
         const int64_t cw = cplx ? 2 : 1;
 
-        // group width g: both panels (2 g M n scalars) within ~45% of the free device memory
+        // group width g: both panels (2 g M n scalars) within 80% of the device memory that is free
+        // or held by this context's panel workspaces (they are given back before growing), so that
+        // a 288 GB device advances as many vectors per sweep as the panels allow: every extra
+        // vector in a group shares the sweep's matrix reads
         size_t free_b = 0, total_b = 0;
         KPMX_HIP(hipMemGetInfo(&free_b, &total_b));
-        const double budget = 0.45 * (double)free_b;
+        const double budget = 0.8 * (double)(free_b + ctx->panel_l.bytes + ctx->panel_r.bytes);
         // bf16x3: panels are written by the sweeps as bf16 pieces (3 per real part), rows padded to
         // a multiple of 8 elements with a zero tail
         const bool split = mode == KPMX_KUBO_BF16X3;
@@ -1036,6 +1046,10 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
         for (const Tile& full : r0->tiles) {
             int g = full.vw;
             while (g > 1 && 2.0 * g * panel1 > budget) g >>= 1;
+            // a g-wide sweep writing split panels stores to 6 g bf16 planes (each M x n, about 1 GB
+            // apart at the 1 M-site workload) at once; measured at 512 moments on 1.05 M sites,
+            // 16 vectors: 0.70 s (g = 2), 0.56 s (4), 0.90 s (8), 3.18 s (16)
+            if (split) g = std::min(g, 4);
             for (int64_t c0 = 0; c0 < full.cols; c0 += g) {
                 const int64_t cols = std::min<int64_t>(g, full.cols - c0);
                 // this group's starters r (n x g), and vl r (n x g)
@@ -1048,6 +1062,10 @@ int kpmx_kubo_moments(const kpmx_mat* h2, const kpmx_mat* vl, const kpmx_mat* vr
                     launch_scatter_block(sc, full.data, n, full.vw, 0, full.vw, vm, ctx->stream);
                     launch_gather_block(sc, vm, n, c0, cols, g, Pr, ctx->stream);
                     KPMX_HIP(hipGetLastError());
+                }
+                if (ctx->panel_l.bytes < (size_t)g * panel1 || ctx->panel_r.bytes < (size_t)g * panel1) {
+                    ctx->panel_l.release();
+                    ctx->panel_r.release();
                 }
                 char* Lpan = static_cast<char*>(ctx->panel_l.get((size_t)g * panel1));
                 char* Rpan = static_cast<char*>(ctx->panel_r.get((size_t)g * panel1));

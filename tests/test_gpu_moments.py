@@ -253,6 +253,33 @@ def test_kubo_contraction_vs_fp64_gemm(ctx, oracle, dtype, M):
     assert err[2] <= 1e-5
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.float64])
+def test_kubo_contraction_sixteen_vectors(ctx, oracle, dtype):
+    """16 random vectors advanced as one group (the panel budget allows it at this size): the
+    16-wide sweeps, split-plane writes and per-vector contractions of every mode."""
+    dtype = np.dtype(dtype)
+    n, R, M = 700, 16, 66
+    h2 = scaled_random(n, dtype, seed=12)
+    x = np.random.default_rng(13).uniform(-5, 5, n).astype(np.float32)
+    vl = oracle.velocity(h2, x).astype(dtype)
+    r0 = oracle.random_starter(n, R, dtype)
+    ref = np.zeros((M, M), dtype=np.complex128)
+    for k in range(R):
+        L = oracle.offdiag_moments(h2, vl @ r0[k], M, "dense")
+        Rr = oracle.offdiag_moments(h2, r0[k], M, "dense", op=vl)
+        ref += L.astype(np.complex128) @ Rr.astype(np.complex128).conj().T
+    ref /= R
+    mat, ml = ctx.mat(h2, fmt=1), ctx.mat(vl, fmt=1)
+    blk = ctx.block(r0)
+    mus = {mode: abi.kubo_moments(mat, ml, ml, blk, M, mode=mode) for mode in (0, 1, 2)}
+    for h in (blk, mat, ml):
+        h.close()
+    scale = np.abs(ref).max()
+    err = {mode: np.abs(mu - ref).max() / scale for mode, mu in mus.items()}
+    print("kubo 16 vectors %s: %s" % (dtype, err))
+    assert err[0] <= 1e-11 and err[1] <= 1e-5 and err[2] <= 1e-5
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.float32])
 def test_lanczos_bounds(ctx, dtype):
     h = generators.honeycomb_supercell(12, 10, disorder=1.0, dtype=np.float64)
