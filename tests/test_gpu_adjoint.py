@@ -86,6 +86,20 @@ def test_checkpointed_equals_stored(oracle):
     ref = oracle.diag_moments(h2, r0, M)
     assert np.max(np.abs(mu - ref) / np.abs(ref[:, :1])) <= 1e-10
     np.testing.assert_allclose(g_ck, g_all, rtol=1e-11, atol=1e-13 * np.abs(g_all).max())
+    # a checkpointed tape run backward repeatedly (its rebuilt vectors reuse three slots), with
+    # another loss in between, gives the same gradient each time; the checkpoint interval that
+    # leaves the last segment one step long (S = 33 = 4 * 8 + 1) is covered too
+    for C in (5, 8, 2):
+        tape = abi.Tape(mat, blk, M, C)
+        w2 = np.random.default_rng(C).standard_normal(M)
+        g1 = tape.backward(w)
+        g_other = tape.backward(w2)
+        g2 = tape.backward(w)
+        tape.close()
+        assert np.array_equal(g1, g2), C
+        np.testing.assert_allclose(g1, g_all, rtol=1e-11, atol=1e-13 * np.abs(g_all).max())
+        g_other_ref, _ = abi.adjoint_diagonal(mat, blk, M, w2, checkpoint=0)
+        np.testing.assert_allclose(g_other, g_other_ref, rtol=1e-11, atol=1e-13 * np.abs(g_other_ref).max())
     # dL/dH2 entry-wise against finite differences of the oracle on a few stored entries
     eps = 1e-6
     for k in np.random.default_rng(5).choice(h2.nnz, 5, replace=False):
