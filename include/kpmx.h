@@ -188,6 +188,12 @@ int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int6
                       const int32_t* indptr, const int32_t* indices, const void* values, kpmx_batch** out);
 int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0);
 int kpmx_batch_run(kpmx_batch* b, int64_t num_moments, int use_graph);
+/* Autotuner of the batched step (no counterpart in the reference): times the step with the
+ * slices dealt as contiguous ranges to 1, 2, 4, ..., 64 workgroups per CU (shorter ranges, more
+ * per-system partial sums) on this batch and its starters, and keeps the fastest partition.
+ * num_workgroups (optional): the chosen workgroup count.  Results stay within rounding of any
+ * other partition (the per-system sums are grouped differently). */
+int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups);
 int kpmx_batch_fetch(kpmx_batch* b, int64_t num_moments, void* mu);
 int kpmx_batch_destroy(kpmx_batch* b);
 

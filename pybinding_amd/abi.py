@@ -220,6 +220,13 @@ class Batch:
     def run(self, num_moments, use_graph=True):
         _check(lib().kpmx_batch_run(self.h, ctypes.c_int64(num_moments), ctypes.c_int(1 if use_graph else 0)))
 
+    def autotune(self):
+        """Pick the fastest slice partition over workgroups (kpmx_batch_autotune); returns the
+        workgroup count."""
+        wg = ctypes.c_int()
+        _check(lib().kpmx_batch_autotune(self.h, ctypes.byref(wg)))
+        return wg.value
+
     def fetch(self, num_moments):
         out = np.empty((len(self.sizes), self.nvec, num_moments), dtype=self.dtype)
         _check(lib().kpmx_batch_fetch(self.h, ctypes.c_int64(num_moments), _p(out)))

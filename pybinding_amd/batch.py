@@ -43,6 +43,13 @@ class KPMBatch:
                             axis=1)
         self.batch.starters(r0)
 
+    def autotune(self, num_random=1):
+        """Time the batched step over several slice partitions and keep the fastest; returns the
+        chosen workgroup count."""
+        if self.batch.nvec != num_random:
+            self.set_random_starters(num_random)
+        return self.batch.autotune()
+
     def moments(self, num_moments, num_random=1, use_graph=True):
         """Undamped diagonal moments, shape (num_systems, num_random, num_moments)."""
         if self.batch.nvec != num_random:

@@ -183,6 +183,7 @@ struct kpmx_batch {
     int64_t num_sys = 0;
     std::vector<int64_t> row_off, pad_off;  // unpadded / padded first row of every system (S + 1)
     int64_t n_pad = 0, nslices = 0;
+    std::vector<int32_t> slice_sys_host;  // system of every slice
     int max_width = 0;
     int64_t* slice_ptr = nullptr;
     int32_t* slice_width = nullptr;
@@ -555,6 +556,45 @@ void tridiagonal_minmax(const std::vector<double>& a, const std::vector<double>&
 
 // ==================================================================================== C-ABI
 namespace {
+// Workgroup w of the batched step owns the contiguous slices [tile_begin[w], tile_begin[w+1]);
+// its per-system partial sums go to slots slot_begin[w] ... (one per system it meets), and
+// slot_first[s] is the first slot of system s.  Rebuilding the partition drops the captured
+// graph and the partial-sum buffer (both depend on it).
+void batch_partition(kpmx_batch* b, int num_wg) {
+    kpmx_ctx* ctx = b->ctx;
+    const int64_t ns_tot = b->nslices;
+    num_wg = (int)std::max<int64_t>(1, std::min<int64_t>(ns_tot, num_wg));
+    std::vector<int64_t> tb(num_wg + 1), sbeg(num_wg);
+    for (int w = 0; w <= num_wg; ++w) tb[w] = ns_tot * w / num_wg;
+    std::vector<int64_t> slot_sys;
+    const auto& ssys = b->slice_sys_host;
+    for (int w = 0; w < num_wg; ++w) {
+        sbeg[w] = (int64_t)slot_sys.size();
+        if (tb[w] == tb[w + 1]) continue;
+        for (int sy = ssys[tb[w]]; sy <= ssys[tb[w + 1] - 1]; ++sy) slot_sys.push_back(sy);
+    }
+    const int64_t num_slots = (int64_t)slot_sys.size();
+    std::vector<int64_t> sfirst(b->num_sys + 1, num_slots);
+    for (int64_t p = num_slots - 1; p >= 0; --p) sfirst[slot_sys[p]] = p;
+    for (int64_t sy = b->num_sys - 1; sy >= 0; --sy) sfirst[sy] = std::min(sfirst[sy], sfirst[sy + 1]);
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    b->drop_graph();
+    for (void** p : {(void**)&b->tile_begin, (void**)&b->slot_begin, (void**)&b->slot_first, (void**)&b->partials}) {
+        dev_free(*p);
+        *p = nullptr;
+    }
+    b->num_wg = num_wg;
+    b->num_slots = num_slots;
+    b->tile_begin = static_cast<int64_t*>(dev_alloc(tb.size() * sizeof(int64_t)));
+    b->slot_begin = static_cast<int64_t*>(dev_alloc(sbeg.size() * sizeof(int64_t)));
+    b->slot_first = static_cast<int64_t*>(dev_alloc(sfirst.size() * sizeof(int64_t)));
+    KPMX_HIP(hipMemcpyAsync(b->tile_begin, tb.data(), tb.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(b->slot_begin, sbeg.data(), sbeg.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(b->slot_first, sfirst.data(), sfirst.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                            ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 // Upload pieces into one workspace, each at a 256-byte aligned offset; returns their device addresses.
 std::vector<char*> upload_pieces(kpmx_ctx* ctx, Workspace& ws, std::initializer_list<std::pair<const void*, size_t>> parts) {
     size_t total = 0;
@@ -1369,20 +1409,7 @@ int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int6
             }
         }
         b->max_width = ns_tot ? *std::max_element(swidth.begin(), swidth.end()) : 0;
-        // contiguous slice ranges per workgroup and the partial slot numbering
-        b->num_wg = (int)std::max<int64_t>(1, std::min<int64_t>(ns_tot, (int64_t)ctx->num_cu * 4));
-        std::vector<int64_t> tb(b->num_wg + 1), sbeg(b->num_wg);
-        for (int w = 0; w <= b->num_wg; ++w) tb[w] = ns_tot * w / b->num_wg;
-        std::vector<int64_t> slot_sys;
-        for (int w = 0; w < b->num_wg; ++w) {
-            sbeg[w] = (int64_t)slot_sys.size();
-            if (tb[w] == tb[w + 1]) continue;
-            for (int sy = ssys[tb[w]]; sy <= ssys[tb[w + 1] - 1]; ++sy) slot_sys.push_back(sy);
-        }
-        b->num_slots = (int64_t)slot_sys.size();
-        std::vector<int64_t> sfirst(num_systems + 1, b->num_slots);
-        for (int64_t p = b->num_slots - 1; p >= 0; --p) sfirst[slot_sys[p]] = p;
-        for (int64_t sy = num_systems - 1; sy >= 0; --sy) sfirst[sy] = std::min(sfirst[sy], sfirst[sy + 1]);
+        b->slice_sys_host = ssys;
         auto up = [&](void* dst, const void* src, size_t bytes) {
             if (bytes) KPMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
         };
@@ -1391,18 +1418,12 @@ int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int6
         b->slice_sys = static_cast<int32_t*>(dev_alloc(ns_tot * sizeof(int32_t)));
         b->sidx = static_cast<int32_t*>(dev_alloc(sidx.size() * sizeof(int32_t)));
         b->svals = dev_alloc(svals.size());
-        b->tile_begin = static_cast<int64_t*>(dev_alloc(tb.size() * sizeof(int64_t)));
-        b->slot_begin = static_cast<int64_t*>(dev_alloc(sbeg.size() * sizeof(int64_t)));
-        b->slot_first = static_cast<int64_t*>(dev_alloc(sfirst.size() * sizeof(int64_t)));
         up(b->slice_ptr, sptr.data(), sptr.size() * sizeof(int64_t));
         up(b->slice_width, swidth.data(), swidth.size() * sizeof(int32_t));
         up(b->slice_sys, ssys.data(), ssys.size() * sizeof(int32_t));
         up(b->sidx, sidx.data(), sidx.size() * sizeof(int32_t));
         up(b->svals, svals.data(), svals.size());
-        up(b->tile_begin, tb.data(), tb.size() * sizeof(int64_t));
-        up(b->slot_begin, sbeg.data(), sbeg.size() * sizeof(int64_t));
-        up(b->slot_first, sfirst.data(), sfirst.size() * sizeof(int64_t));
-        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        batch_partition(b.get(), (int)std::min<int64_t>(ns_tot, (int64_t)ctx->num_cu * 4));
         *out = b.release();
     });
 }
@@ -1523,6 +1544,49 @@ int kpmx_batch_run(kpmx_batch* b, int64_t M, int use_graph) {
             b->graph_moments = M;
         }
         KPMX_HIP(hipGraphLaunch(b->graph_exec, ctx->stream));
+    });
+}
+
+int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups) {
+    return guarded([&] {
+        KPMX_REQUIRE(b, "null batch");
+        KPMX_REQUIRE(b->r0, "no starters uploaded");
+        kpmx_ctx* ctx = b->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        if (!ctx->t0) KPMX_HIP(hipEventCreate(&ctx->t0));
+        if (!ctx->t1) KPMX_HIP(hipEventCreate(&ctx->t1));
+        // about 1 GB of vector traffic per timed sample (at least 8 steps)
+        const double step_bytes = 3.0 * (double)b->n_pad * b->vw * scalar_bytes(b->scalar);
+        const int64_t steps = std::max<int64_t>(8, std::min<int64_t>(64, (int64_t)(1e9 / step_bytes)));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        dev_free(b->sums);
+        b->sums = static_cast<double*>(dev_alloc((size_t)steps * b->num_sys * 64 * 4 * sizeof(double)));
+        b->sums_steps = -1;  // the timing runs leave no result to fetch
+        int best = 0;
+        float best_ms = 0;
+        for (int per_cu : {1, 2, 4, 8, 16, 32, 64}) {
+            const int wg = (int)std::min<int64_t>(b->nslices, (int64_t)ctx->num_cu * per_cu);
+            if (per_cu > 1 && wg == b->num_wg && wg < ctx->num_cu * per_cu) break;  // no more slices to spread
+            batch_partition(b, wg);
+            b->partials = static_cast<double*>(dev_alloc((size_t)b->num_slots * 64 * 4 * sizeof(double)));
+            float ms_w = 0;
+            for (int rep = 0; rep < 3; ++rep) {  // the first repetition warms caches and code
+                KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
+                batch_enqueue(b, steps);
+                KPMX_HIP(hipGetLastError());
+                KPMX_HIP(hipEventRecord(ctx->t1, ctx->stream));
+                KPMX_HIP(hipEventSynchronize(ctx->t1));
+                float ms = 0;
+                KPMX_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+                if (rep == 1 || (rep > 1 && ms < ms_w)) ms_w = ms;
+            }
+            if (best == 0 || ms_w < best_ms) {
+                best_ms = ms_w;
+                best = b->num_wg;
+            }
+        }
+        if (best != b->num_wg) batch_partition(b, best);
+        if (num_workgroups) *num_workgroups = b->num_wg;
     });
 }
 

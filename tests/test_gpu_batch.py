@@ -55,3 +55,25 @@ def test_batched_rejects_coupled_systems():
                                                ctypes.c_int64(len(b.data)), abi._p(b.indptr), abi._p(b.indices),
                                                abi._p(b.data), ctypes.byref(handle)))
     ctx.close()
+
+
+def test_batched_autotune_keeps_results(oracle):
+    """The autotuner's partitions (256, 512 and 523 workgroups over 523 slices) change only how
+    each system's sums are grouped: every system still matches the oracle, and the tuned run agrees
+    with the default partition to rounding."""
+    hs, ranges = systems(np.float64, count=40, seed=8)
+    kb = kpm_batch(hs, ranges)
+    M = 66
+    before = kb.moments(M, 4)
+    wg = kb.autotune(4)
+    after = kb.moments(M, 4)
+    again = kb.moments(M, 4, use_graph=False)
+    kb.close()
+    assert wg >= 1
+    np.testing.assert_array_equal(after, again)
+    assert np.max(np.abs(after - before) / np.abs(before[:, :, :1])) <= 1e-13
+    for s, (h, (lo, hi)) in enumerate(zip(hs, ranges)):
+        a, b = oracle.scale_factors(lo, hi)
+        h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
+        ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], 4, np.float64), M)
+        assert np.max(np.abs(after[s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
