@@ -108,9 +108,10 @@ int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_
 
 /* Autotuner (no counterpart in the reference, whose CPU kernels have one fixed shape): time the
  * variants of the fused diagonal step kernel (three tile shapes, each on the full resident grid,
- * half of it or a quarter) on this matrix and starter block and keep the fastest for every later
- * recurrence of that scalar type and vector width on the context.
- * variant (optional): the chosen variant index, tile shape + 3 * grid fraction (0: full). */
+ * half of it, a quarter, twice or four times it) on this matrix and starter block and keep the
+ * fastest for every later recurrence of that scalar type and vector width on the context.
+ * variant (optional): the chosen variant index, tile shape + 3 * grid (0 resident, 1 half,
+ * 2 quarter, 3 twice, 4 four times). */
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant);
 
 /* Graph policy (no counterpart in the reference, whose CPU loop has no launch cost): with

@@ -855,12 +855,15 @@ static int launch_sellb_off(StepArgs<T> a, const StepLaunch& L, hipStream_t stre
     auto kernel = k_step_sellb<T, VW, MODE, KMAX, RW, OFF32>;
     constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
     a.num_tiles = (L.n + tr - 1) / tr;
-    // tuned variants 3..8 run 1/2 or 1/4 of the resident grid (fewer waves in flight per CU)
-    const int shift = (MODE == MODE_DIAG) ? L.tune / NUM_TILE_SHAPES : 0;
+    // the tuned variant's grid: the resident grid, 1/2 or 1/4 of it (fewer waves in flight per CU),
+    // or 2x / 4x of it (workgroups queue behind the resident ones and take shorter tile runs)
+    const int gsel = (MODE == MODE_DIAG) ? L.tune / NUM_TILE_SHAPES : 0;
     auto grid_of = [&] {
-        int g = resident_grid(kernel, L.num_cu, a.num_tiles) >> shift;
+        int64_t g = resident_grid(kernel, L.num_cu, a.num_tiles);
+        if (gsel <= 2) g >>= gsel;
+        else g = std::min<int64_t>(a.num_tiles, g << (gsel - 2));
         if (g >= 8) g -= g % 8;
-        return g < 1 ? 1 : g;
+        return (int)(g < 1 ? 1 : g);
     };
     if (L.query_grid) return grid_of();
     const int grid = L.num_wg > 0 ? L.num_wg : grid_of();
