@@ -1187,11 +1187,13 @@ static void adjoint_rw_launch(AdjArgs<T> a, const AdjointLaunch& A, hipStream_t 
     constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
     a.num_tiles = (A.n + tr - 1) / tr;
     const bool off32 = (uint64_t)A.n * VW * sizeof(T) < (1ull << 32);
-    const int shift = A.tune / NUM_TILE_SHAPES;
+    const int gsel = A.tune / NUM_TILE_SHAPES;
     auto go = [&](auto kernel) {
-        int g = resident_grid(kernel, A.num_cu, a.num_tiles) >> shift;
+        int64_t g = resident_grid(kernel, A.num_cu, a.num_tiles);
+        if (gsel <= 2) g >>= gsel;
+        else g = std::min<int64_t>(a.num_tiles, g << (gsel - 2));
         if (g >= 8) g -= g % 8;
-        hipLaunchKernelGGL(kernel, dim3(g < 1 ? 1 : g), dim3(BLOCK), 0, stream, a, A.slice_ptr, A.slice_width);
+        hipLaunchKernelGGL(kernel, dim3((unsigned)(g < 1 ? 1 : g)), dim3(BLOCK), 0, stream, a, A.slice_ptr, A.slice_width);
     };
     if (A.max_width <= 4) {
         if (off32) go(k_adjoint_sellb<T, VW, 4, RW, true>);
