@@ -87,3 +87,99 @@ def test_fp32_against_fp64(workload):
     blk.close()
     mat.close()
     assert np.max(np.abs(mu32.astype(np.float64) - mu64) / mu64[:, :1]) <= 1e-4
+
+
+def dos_window_weights(M, a, b, energy=0.5, sigma=0.1, points=4096):
+    """w_n of L = integral of the Jackson-damped DOS against a Gaussian window, L = sum_n w_n mu_n."""
+    xq = np.cos(np.pi * (np.arange(points) + 0.5) / points)
+    win = np.exp(-0.5 * ((a * xq + b - energy) / sigma) ** 2)
+    n = np.arange(M)
+    g = ((M + 1 - n) * np.cos(np.pi * n / (M + 1)) + np.sin(np.pi * n / (M + 1)) / np.tan(np.pi / (M + 1))) / (M + 1)
+    return g * (2 / np.pi) * np.pi / points * (np.cos(n[:, None] * np.arccos(xq)[None, :]) @ win)
+
+
+def test_adjoint_full_size():
+    """Strained graphene flake of ~1 M sites, 1024 moments, 16 vectors: the gradient with checkpoints
+    every 64 steps equals the stored-tape gradient, and its contraction with H2 equals the central
+    difference of the loss along H2 itself (two forward passes at (1 +- 1e-5) H2)."""
+    from pybinding_amd.autograd import TightBinding
+    n, bonds, hop, onsite, pos = generators.strained_graphene_flake(161.8, 161.8, return_positions=True)
+    M, R = 1024, 16
+    tb = TightBinding(n, bonds, (-9.0, 9.0), M, R, checkpoint=64, positions=(pos[:, 0], pos[:, 1]))
+    try:
+        h2 = tb.scaled(onsite, hop)
+        w = dos_window_weights(M, *tb.scale)
+        mat = tb.ctx.mat(h2, fmt=1)
+        g_ck, mu = abi.adjoint_diagonal(mat, tb.block, M, w, checkpoint=64)
+        g_all, _ = abi.adjoint_diagonal(mat, tb.block, M, w, checkpoint=0)
+        mat.close()
+        assert np.abs(g_ck - g_all).max() <= 1e-12 * np.abs(g_all).max()
+        eps = 1e-5
+        loss = []
+        for s in (1.0, -1.0):
+            m_s = tb.ctx.mat(h2 * (1.0 + s * eps), fmt=1)
+            loss.append(np.dot(w, abi.moments_diagonal(m_s, tb.block, M).mean(axis=0)))
+            m_s.close()
+        fd = (loss[0] - loss[1]) / (2 * eps)
+        ad = float(np.dot(g_all, h2.data))
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (fd, ad)
+    finally:
+        tb.close()
+
+
+def test_kubo_full_size_precisions():
+    """Peierls graphene (B = 20 T, W = 1 eV, complex128) of 1.05 M sites, 16 vectors, 128 x 128
+    moments: the FP32-split and bf16x3 contractions against the FP64 one."""
+    from pybinding_amd._native import load_module
+    native = load_module()
+    h, x, _ = generators.peierls_honeycomb_supercell(724, 724, 20.0, disorder=1.0)
+    ip, ix, dv, rmap, _, _, _, (a, b) = native.optimize(h.shape, h.indptr, h.indices, h.data, (-9.5, 9.5), True,
+                                                        [0], [0])
+    h2 = sp.csr_matrix((dv, ix, ip), shape=h.shape)
+    xo = x.astype(np.float32)[np.argsort(rmap)]
+    hh = h2.tocoo()
+    v = sp.csr_matrix(((xo[hh.row] - xo[hh.col]).astype(np.float64) * hh.data * (a / 2), (hh.row, hh.col)),
+                      shape=h.shape)
+    r0 = native.random_starters(h.shape[0], 16, "complex128", rmap)
+    ctx = abi.Ctx(0)
+    try:
+        mat, vm, blk = ctx.mat(h2, fmt=1), ctx.mat(v, fmt=1), ctx.block(r0)
+        mus = {mode: abi.kubo_moments(mat, vm, vm, blk, 128, mode=mode) for mode in (0, 1, 2)}
+        for hnd in (blk, vm, mat):
+            hnd.close()
+    finally:
+        ctx.close()
+    scale = np.abs(mus[0]).max()
+    assert np.all(np.isfinite(mus[0])) and scale > 0
+    assert np.abs(mus[1] - mus[0]).max() <= 1e-6 * scale
+    assert np.abs(mus[2] - mus[0]).max() <= 1e-6 * scale
+
+
+def test_batched_full_size(oracle):
+    """256 square flakes of 10^4 to 5 x 10^5 sites (31 M in all), 1024 moments, 4 vectors each, one
+    graph: every system has mu_0 = N_s / 2 exactly and |mu_n| <= 2 mu_0, and the three smallest
+    systems match the oracle."""
+    from pybinding_amd.batch import kpm_batch
+    rs = np.random.default_rng(3)
+    sizes = np.exp(rs.uniform(np.log(1e4), np.log(5e5), 256))
+    rw = np.random.default_rng(4)
+    Ws = rw.uniform(0, 4, 256)
+    hs, ranges = [], []
+    for sz, Wf in zip(sizes, Ws):
+        side = max(2, int(round(np.sqrt(sz))))
+        hs.append(generators.square_flake(side, side, t=-1.0, disorder=Wf, rng=rw))
+        ranges.append((-(4 + Wf / 2), 4 + Wf / 2))
+    kb = kpm_batch(hs, ranges)
+    try:
+        M = 1024
+        mu = kb.moments(M, 4)
+    finally:
+        kb.close()
+    for s, h in enumerate(hs):
+        assert np.all(mu[s, :, 0] == h.shape[0] / 2), s
+        assert np.all(np.abs(mu[s, :, 1:]) <= 2 * mu[s, :, :1] * (1 + 1e-12)), s
+    for s in np.argsort([h.shape[0] for h in hs])[:3]:
+        a, b = oracle.scale_factors(*ranges[s])
+        h2 = oracle.scale_hamiltonian(hs[s], a, b, np.float64)
+        ref = oracle.diag_moments(h2, oracle.random_starter(hs[s].shape[0], 4, np.float64), M)
+        assert np.max(np.abs(mu[s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
