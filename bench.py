@@ -426,11 +426,12 @@ def main():
         ms = e0.elapsed_time(e1) / 5
         from pybinding_amd import abi
         pctx = abi.Ctx(0)
-        cg, rg = pctx.stream_probe(1 << 31)
+        cg, rg, tg = pctx.stream_probe(1 << 31)
         pctx.close()
         print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8,
                           "copy_ms": ms, "copy_GBps": 2 * a.numel() * 8 / (ms / 1000) / 1e9,
-                          "stream_probe_copy_GBps": cg, "stream_probe_read_GBps": rg}))
+                          "stream_probe_copy_GBps": cg, "stream_probe_read_GBps": rg,
+                          "stream_probe_triad_GBps": tg}))
         return
 
     if args.gpus != 1:
@@ -481,11 +482,13 @@ def main():
 
     traffic_key = "%dx%d %s v%d %s %s" % (args.cells[0], args.cells[1], dtype.name, R, args.format, args.order)
     extra = {}
-    cg, rg = ctx.stream_probe(1 << 31)
-    extra["stream_probe"] = {"copy_GBps": cg, "read_GBps": rg,
-                             "step_traffic_over_copy": (counted_traffic(traffic_key) or float("nan")) /
-                             (step_ms / 1000) / 1e9 / cg}
-    log("stream probe: copy %.0f GB/s, read %.0f GB/s" % (cg, rg))
+    cg, rg, tg = ctx.stream_probe(1 << 31)
+    step_traffic_gbps = (counted_traffic(traffic_key) or float("nan")) / (step_ms / 1000) / 1e9
+    extra["stream_probe"] = {"copy_GBps": cg, "read_GBps": rg, "triad_GBps": tg,
+                             "step_traffic_over_copy": step_traffic_gbps / cg,
+                             "step_traffic_over_triad": step_traffic_gbps / tg}
+    log("stream probe: copy %.0f GB/s, read %.0f GB/s, triad (b = a + b) %.0f GB/s; step traffic at %.2f of the triad"
+        % (cg, rg, tg, step_traffic_gbps / tg))
     if not args.no_plain:
         # the plain single-vector path (one vector, SELL-64 kernel) on the same matrix, timed over
         # 512 moments and reported in the same metric

@@ -233,9 +233,10 @@ int kpmx_kubo_bastin_sums(kpmx_ctx* ctx, int64_t num_moments, const double* mu_n
 
 /* Streaming probe (no counterpart in the reference; the practical bandwidth reference for the
  * roofline reports): best GB/s over a few grid sizes of a non-temporal 16-byte copy of `bytes`
- * (bytes read + bytes written) and of a 16-byte read-only sweep, on buffers the call allocates
- * and frees. */
-int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* read_gbps);
+ * (bytes read + bytes written), of a 16-byte read-only sweep, and (triad_gbps, optional) of
+ * b = a + b in place with b non-temporal -- the step kernel's two reads and one write per element
+ * with the gathers made contiguous (3 x bytes moved) -- on buffers the call allocates and frees. */
+int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* read_gbps, double* triad_gbps);
 
 #ifdef __cplusplus
 }

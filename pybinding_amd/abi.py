@@ -81,10 +81,11 @@ class Ctx:
         _check(lib().kpmx_ctx_set_graph_policy(self.h, int(policy)))
 
     def stream_probe(self, nbytes=1 << 31):
-        """(copy GB/s, read GB/s) of the device's streaming probe (kpmx_stream_probe)."""
-        c, r = ctypes.c_double(), ctypes.c_double()
-        _check(lib().kpmx_stream_probe(self.h, ctypes.c_int64(nbytes), ctypes.byref(c), ctypes.byref(r)))
-        return c.value, r.value
+        """(copy GB/s, read GB/s, triad GB/s) of the device's streaming probe (kpmx_stream_probe)."""
+        c, r, t = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().kpmx_stream_probe(self.h, ctypes.c_int64(nbytes), ctypes.byref(c), ctypes.byref(r),
+                                       ctypes.byref(t)))
+        return c.value, r.value, t.value
 
 
 class Mat:

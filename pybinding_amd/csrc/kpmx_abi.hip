@@ -1883,7 +1883,7 @@ int kpmx_kubo_bastin_sums(kpmx_ctx* ctx, int64_t num_moments, const double* mu_n
     });
 }
 
-int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* read_gbps) {
+int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* read_gbps, double* triad_gbps) {
     return guarded([&] {
         KPMX_REQUIRE(ctx && copy_gbps && read_gbps, "null argument");
         KPMX_REQUIRE(bytes >= (1 << 20) && bytes % 16 == 0, "bytes must be a multiple of 16, at least 1 MiB");
@@ -1901,8 +1901,8 @@ int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* r
         KPMX_HIP(hipMemsetAsync(b, 0, (size_t)bytes, ctx->stream));
         if (!ctx->t0) KPMX_HIP(hipEventCreate(&ctx->t0));
         if (!ctx->t1) KPMX_HIP(hipEventCreate(&ctx->t1));
-        double best[2] = {0, 0};
-        for (int kind = 0; kind < 2; ++kind)
+        double best[3] = {0, 0, 0};
+        for (int kind = 0; kind < (triad_gbps ? 3 : 2); ++kind)
             for (int per_cu : {2, 4, 8, 16})
                 for (int rep = 0; rep < 3; ++rep) {
                     KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
@@ -1912,11 +1912,12 @@ int kpmx_stream_probe(kpmx_ctx* ctx, int64_t bytes, double* copy_gbps, double* r
                     KPMX_HIP(hipEventSynchronize(ctx->t1));
                     float ms = 0;
                     KPMX_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
-                    const double moved = (kind == 0 ? 2.0 : 1.0) * (double)bytes;
+                    const double moved = (kind == 0 ? 2.0 : kind == 1 ? 1.0 : 3.0) * (double)bytes;
                     if (rep > 0 && ms > 0) best[kind] = std::max(best[kind], moved / (ms * 1e-3) / 1e9);
                 }
         *copy_gbps = best[0];
         *read_gbps = best[1];
+        if (triad_gbps) *triad_gbps = best[2];
     });
 }
 

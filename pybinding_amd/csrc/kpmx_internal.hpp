@@ -118,7 +118,7 @@ int launch_cheb_series(int real, bool exp_series, const void* mu, int64_t M, int
 int kubo_sums_groups(int64_t M, int64_t P);
 void launch_kubo_sums(const double* mu, int64_t M, const double* samples, int64_t P, double* t, double* a,
                       double* partial, int groups, double* out, hipStream_t s);
-// streaming probes: kind 0 copies `bytes` from a to b, kind 1 reads a
+// streaming probes: kind 0 copies `bytes` from a to b, kind 1 reads a, kind 2 adds a into b in place
 void launch_probe(int kind, const void* a, void* b, int64_t bytes, int grid, hipStream_t stream);
 void launch_zero(int scalar, void* dst, int64_t count, hipStream_t s);
 void launch_zero_f64(double* dst, int64_t count, hipStream_t s);
