@@ -108,11 +108,15 @@ int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_
 
 /* Autotuner (no counterpart in the reference, whose CPU kernels have one fixed shape): time the
  * variants of the fused diagonal step kernel (three tile shapes, each on the full resident grid,
- * half of it, a quarter, twice or four times it) on this matrix and starter block and keep the
- * fastest for every later recurrence of that scalar type and vector width on the context.
- * variant (optional): the chosen variant index, tile shape + 3 * grid (0 resident, 1 half,
- * 2 quarter, 3 twice, 4 four times). */
+ * half of it, a quarter, twice or four times it, each with the tile's matrix entries staged
+ * through workgroup LDS or handed out by lane shuffles within each wave) on this matrix and
+ * starter block and keep the fastest for every later recurrence of that scalar type and vector
+ * width on the context.  variant (optional): the chosen variant index, tile shape + 3 * grid
+ * (0 resident, 1 half, 2 quarter, 3 twice, 4 four times) + 15 * staging (0 LDS, 1 shuffles). */
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant);
+/* Pin the diagonal step variant (as numbered above) for one scalar type and vector width on the
+ * context, e.g. a choice an earlier autotuning run reported (no counterpart in the reference). */
+int kpmx_ctx_set_step_variant(kpmx_ctx* ctx, int scalar, int vector_width, int variant);
 
 /* Graph policy (no counterpart in the reference, whose CPU loop has no launch cost): with
  * KPMX_GRAPH_ALWAYS the diagonal recurrence of a starter block is captured into a HIP graph on its

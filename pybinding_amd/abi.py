@@ -75,6 +75,11 @@ class Ctx:
         _check(lib().kpmx_ctx_timer_stop(self.h, ctypes.byref(ms)))
         return ms.value
 
+    def set_step_variant(self, dtype, vector_width, variant):
+        """Pin the diagonal step kernel's variant for one scalar type and block width."""
+        _check(lib().kpmx_ctx_set_step_variant(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.c_int(vector_width),
+                                               ctypes.c_int(variant)))
+
     def set_graph_policy(self, policy):
         """Graph policy of the diagonal recurrence: 0 never (default), 1 auto (small starter blocks,
         captured on the second identical run), 2 always."""

@@ -1301,6 +1301,17 @@ int kpmx_ctx_timer_stop(kpmx_ctx* ctx, double* ms) {
     });
 }
 
+int kpmx_ctx_set_step_variant(kpmx_ctx* ctx, int scalar, int vector_width, int variant) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        KPMX_REQUIRE(vector_width >= 1 && vector_width <= 64 && (vector_width & (vector_width - 1)) == 0,
+                     "vector width must be a power of two up to 64");
+        KPMX_REQUIRE(variant >= 0 && variant < NUM_TUNE_VARIANTS, "variant out of range");
+        ctx->tune[scalar][log2i(vector_width)] = variant;
+    });
+}
+
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant) {
     return guarded([&] {
         check_pair(h2, r0);

@@ -84,7 +84,8 @@ int launch_adjoint_step(const AdjointLaunch& A, hipStream_t stream);
 int launch_step(const StepLaunch& L, hipStream_t stream);
 int step_grid(const StepLaunch& L);
 constexpr int NUM_TILE_SHAPES = 3;     // rows per wave per tile (k_step_sellb)
-constexpr int NUM_TUNE_VARIANTS = 15;  // tile shape x grid (resident, 1/2, 1/4, 2x, 4x)
+constexpr int NUM_GRID_VARIANTS = 5;   // resident grid, 1/2, 1/4, 2x, 4x
+constexpr int NUM_TUNE_VARIANTS = 30;  // tile shape x grid x staging (workgroup LDS, wave shuffles)
 int64_t step_rows_per_tile(int format, int vw);
 void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
                             hipStream_t stream);

@@ -346,6 +346,131 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellb(StepArgs<T> a, const int64
     if constexpr (MODE != MODE_NONE) block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials);
 }
 
+// A lane-shuffle broadcast of one scalar of any engine type.
+template <class T>
+__device__ __forceinline__ T shfl_any(T v, int src) {
+    if constexpr (std::is_same<T, float>::value || std::is_same<T, double>::value) {
+        return __shfl(v, src, WAVE);
+    } else {
+        T r;
+        r.re = __shfl(v.re, src, WAVE);
+        r.im = __shfl(v.im, src, WAVE);
+        return r;
+    }
+}
+
+// The diagonal step of k_step_sellb with wave-private staging: each wave loads the first KMAX
+// entries of its own RW rows itself (entry k of row r in lane k * RW + r, so KMAX * RW <= 64)
+// and hands them to the lanes that gather with them by lane shuffles, so a tile needs no LDS
+// staging and no workgroup barrier.  Otherwise as k_step_sellb (same tile walk, software
+// pipeline of the next tile's entries, non-temporal y, wide rows from global memory).
+template <class T, int VW, int KMAX, int RW, bool OFF32>
+__global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
+                                                      const int32_t* __restrict__ slice_width) {
+    constexpr int RPW = WAVE / VW;
+    constexpr int U = RW / RPW;
+    constexpr int TR = RW * WAVES_PER_BLOCK;
+    constexpr int TPS = WAVE / TR;
+    static_assert(VW >= 4 && RW % RPW == 0 && WAVE % TR == 0 && KMAX * RW <= WAVE, "bad wave-staged configuration");
+    const int lane = threadIdx.x % WAVE;
+    const int wave = threadIdx.x / WAVE;
+    const int rsub = lane / VW;
+    const int col = lane % VW;
+    const int32_t* __restrict__ sidx = a.indices;
+    const T* __restrict__ sval = a.vals;
+    const T* __restrict__ x = a.x;
+    double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
+    const TileWalk tw = tile_walk(a.num_tiles);
+    const int sk = lane / RW, sr = lane % RW;  // the entry this lane stages: entry sk of row sr
+    int32_t p_idx = 0;
+    T p_val = zero(T{});
+    auto prefetch = [&](int64_t tile, int64_t& pbase, int& pwidth) {
+        const int64_t slc = tile / TPS;
+        const int rr0 = (int)(tile % TPS) * TR;
+        pbase = slice_ptr[slc];
+        pwidth = slice_width[slc];
+        const int kk = pwidth < KMAX ? pwidth : KMAX;
+        const int rin = wave * RW + sr;
+        if (lane < KMAX * RW && sk < kk) {
+            p_idx = sidx[pbase + (int64_t)sk * WAVE + rr0 + rin];
+            p_val = sval[pbase + (int64_t)sk * WAVE + rr0 + rin];
+        } else {
+            const int64_t own = slc * WAVE + rr0 + rin;
+            p_idx = (int32_t)(own < a.n ? own : a.n - 1);
+            p_val = zero(T{});
+        }
+    };
+    int64_t next_base = 0;
+    int next_width = 0;
+    if (tw.first < tw.end) prefetch(tw.first, next_base, next_width);
+    for (int64_t tile = tw.first; tile < tw.end; tile += tw.stride) {
+        const int64_t slice = tile / TPS;
+        const int r0 = (int)(tile % TPS) * TR;
+        const int64_t base = next_base;
+        const int width = next_width;
+        const int32_t c_idx = p_idx;
+        const T c_val = p_val;
+        int rin[U];
+        int64_t row[U];
+        bool ok[U];
+        T yv[U], xo[U];
+        T xg[U][KMAX];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            rin[u] = wave * RW + u * RPW + rsub;
+            row[u] = slice * WAVE + r0 + rin[u];
+            ok[u] = row[u] < a.n;
+            const int64_t e = (ok[u] ? row[u] : a.n - 1) * VW + col;
+            yv[u] = a.sub_y ? nt_load(&a.yin[e]) : zero(T{});
+            xo[u] = x[e];
+        }
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int32_t c = __shfl(c_idx, k * RW + u * RPW + rsub, WAVE);
+                if constexpr (OFF32) {
+                    const uint32_t off = ((uint32_t)c * VW + col) * (uint32_t)sizeof(T);
+                    xg[u][k] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(x) + off);
+                } else {
+                    xg[u][k] = x[(int64_t)c * VW + col];
+                }
+            }
+        if (tile + tw.stride < tw.end) prefetch(tile + tw.stride, next_base, next_width);
+        T acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc[u] = zero(T{});
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) acc[u] = fmac(shfl_any(c_val, k * RW + u * RPW + rsub), xg[u][k], acc[u]);
+        }
+        for (int k = KMAX; k < width; ++k) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t off = base + (int64_t)k * WAVE + r0 + rin[u];
+                acc[u] = fmac(sval[off], x[(int64_t)sidx[off] * VW + col], acc[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (!ok[u]) continue;
+            const int64_t e = row[u] * VW + col;
+            T ynew = scal(acc[u], a.hscale);
+            if (a.sub_y) ynew = sub(ynew, yv[u]);
+            if (a.yout) nt_store(ynew, &a.yout[e]);
+            if (a.mark) {
+                const int32_t slot = a.mark[row[u]];
+                if (slot >= 0) a.mark_out[(int64_t)slot * a.mark_stride + col] = ynew;
+            }
+            a_re += sqabs(xo[u]);
+            const d2 p = cdot(ynew, xo[u]);
+            b_re += p.re;
+            b_im += p.im;
+        }
+    }
+    block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials);
+}
+
 // Rows per wave per tile of the SELL block kernel: the tunable choices for a vector width
 // (tune index 0 is the default).  Each must be a multiple of 64/VW and at most 16.
 template <int VW>
@@ -850,14 +975,37 @@ static int resident_grid(K kernel, int num_cu, int64_t num_tiles) {
     return (int)g;
 }
 
+template <class T, int VW, int KMAX, int RW, bool OFF32>
+static int launch_sellw_off(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
+    auto kernel = k_step_sellw<T, VW, KMAX, RW, OFF32>;
+    constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
+    a.num_tiles = (L.n + tr - 1) / tr;
+    const int gsel = (L.tune / NUM_TILE_SHAPES) % NUM_GRID_VARIANTS;
+    auto grid_of = [&] {
+        int64_t g = resident_grid(kernel, L.num_cu, a.num_tiles);
+        if (gsel <= 2) g >>= gsel;
+        else g = std::min<int64_t>(a.num_tiles, g << (gsel - 2));
+        if (g >= 8) g -= g % 8;
+        return (int)(g < 1 ? 1 : g);
+    };
+    if (L.query_grid) return grid_of();
+    const int grid = L.num_wg > 0 ? L.num_wg : grid_of();
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, L.slice_ptr, L.slice_width);
+    return 0;
+}
+
 template <class T, int VW, int MODE, int KMAX, int RW, bool OFF32>
 static int launch_sellb_off(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
+    // tuned diagonal variants 15..29: wave-private staging where a wave's entries fit its lanes
+    if constexpr (MODE == MODE_DIAG && KMAX * RW <= WAVE) {
+        if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) == 1) return launch_sellw_off<T, VW, KMAX, RW, OFF32>(a, L, stream);
+    }
     auto kernel = k_step_sellb<T, VW, MODE, KMAX, RW, OFF32>;
     constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
     a.num_tiles = (L.n + tr - 1) / tr;
     // the tuned variant's grid: the resident grid, 1/2 or 1/4 of it (fewer waves in flight per CU),
     // or 2x / 4x of it (workgroups queue behind the resident ones and take shorter tile runs)
-    const int gsel = (MODE == MODE_DIAG) ? L.tune / NUM_TILE_SHAPES : 0;
+    const int gsel = (MODE == MODE_DIAG) ? (L.tune / NUM_TILE_SHAPES) % NUM_GRID_VARIANTS : 0;
     auto grid_of = [&] {
         int64_t g = resident_grid(kernel, L.num_cu, a.num_tiles);
         if (gsel <= 2) g >>= gsel;
@@ -1187,7 +1335,7 @@ static void adjoint_rw_launch(AdjArgs<T> a, const AdjointLaunch& A, hipStream_t 
     constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
     a.num_tiles = (A.n + tr - 1) / tr;
     const bool off32 = (uint64_t)A.n * VW * sizeof(T) < (1ull << 32);
-    const int gsel = A.tune / NUM_TILE_SHAPES;
+    const int gsel = (A.tune / NUM_TILE_SHAPES) % NUM_GRID_VARIANTS;
     auto go = [&](auto kernel) {
         int64_t g = resident_grid(kernel, A.num_cu, a.num_tiles);
         if (gsel <= 2) g >>= gsel;
