@@ -300,6 +300,7 @@ def plain_dos_flake(M=2050, R=8, reps=5):
     ctx = abi.Ctx(0)
     mat = ctx.mat(h2, fmt=1)
     blk = ctx.block(r0)
+    variant = abi.autotune_diagonal(mat, blk)
     timings = {}
     for policy, name in ((1, "graph"), (0, "eager")):
         ctx.set_graph_policy(policy)
@@ -334,7 +335,7 @@ def plain_dos_flake(M=2050, R=8, reps=5):
     out = {"sites": n, "nnz": int(h2.nnz), "moments": M, "vectors": R,
            "gpu_ms": gpu_ms, "gpu_value": h2.nnz * M * R / (gpu_ms / 1000) / 1e9, "gpu_us_per_step": gpu_ms * 1000 / (M // 2),
            "gpu_ms_step_launches": timings["eager"], "gpu_ms_graph_replay": timings["graph"],
-           "facade_calc_dos_ms": facade_ms,
+           "facade_calc_dos_ms": facade_ms, "kernel_variant": variant,
            "cpu_ms": t_cpu * 1000, "cpu_value": h2.nnz * M * R / t_cpu / 1e9, "cpu_threads": threads,
            "max_rel_diff_vs_cpu": err, "unit": "GNNZ*moments/s",
            "workload": "graphene 40x40 nm flake (%d sites), fp64, 2050 moments, 8 vectors" % n}

@@ -1318,10 +1318,10 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
         kpmx_ctx* ctx = h2->ctx;
         KPMX_HIP(hipSetDevice(ctx->device));
         const Tile& t = r0->tiles.front();
-        // About 1 GB of vector traffic per timed sample (at least 9 steps): small systems get
-        // enough steps that launch jitter does not decide the choice.
+        // About 2 GB of vector traffic per timed sample (at least 9 steps, at most 255): small
+        // systems get enough steps that launch jitter does not decide the choice.
         const double step_bytes = 3.0 * (double)h2->rows * t.vw * scalar_bytes(h2->scalar);
-        const int64_t steps = std::max<int64_t>(9, std::min<int64_t>(101, (int64_t)(1e9 / step_bytes)));
+        const int64_t steps = std::max<int64_t>(9, std::min<int64_t>(255, (int64_t)(2e9 / step_bytes)));
         double* sums = static_cast<double*>(ctx->sums.get((size_t)steps * 4 * t.vw * sizeof(double)));
         if (!ctx->t0) KPMX_HIP(hipEventCreate(&ctx->t0));
         if (!ctx->t1) KPMX_HIP(hipEventCreate(&ctx->t1));
@@ -1329,7 +1329,7 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
         float best_ms = 0;
         for (int v = 0; v < NUM_TUNE_VARIANTS; ++v) {
             float ms_v = 0;
-            for (int rep = 0; rep < 3; ++rep) {  // the first repetition warms caches and code
+            for (int rep = 0; rep < 4; ++rep) {  // the first repetition warms caches and code
                 KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
                 run_recurrence(h2, t, MODE_DIAG, steps, nullptr, sums, [](StepLaunch&, int64_t) {},
                                [](StepLaunch const&, int64_t) {}, v);
