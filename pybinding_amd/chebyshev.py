@@ -254,7 +254,9 @@ def kpm(model, energy_range=None, kernel="default", num_threads="auto", silent=F
         Don't show any progress messages.
     **kwargs
         matrix_format ("ELL" or "CSR"), optimal_size, interleaved, lanczos_precision,
-        device (GPU index), kubo_precision ("fp64", "fp32_split" or "bf16x3"),
+        device (GPU index), kubo_precision ("fp64", "fp32_split" or "bf16x3"), autotune (time the
+        step kernel's variants on the first block of random vectors of each width and keep the
+        fastest; the choice stays with the device for later calculations),
         site_order ("auto": a Hilbert-curve order of the model's site positions for whole-system
         calculations, "bfs": the reference's BFS reorder only, or an explicit permutation).
     """

@@ -154,6 +154,7 @@ struct Config {
     AlgorithmConfig algorithm;
     float lanczos_precision = 0.002f;
     int device = 0;
+    bool autotune = false;  // time the diagonal step variants on the first block of each width
 };
 
 // ------------------------------------------------------------------------------------- Indices

@@ -32,7 +32,7 @@ class CoreT final : public CoreBase {
 public:
     CoreT(Csr<T> h, Config cfg, ProgressCallback progress, int kubo_mode)
         : h_(std::move(h)), cfg_(std::move(cfg)), oh_(&h_, cfg_.algorithm.reorder()),
-          compute_(cfg_.device, std::move(progress)), kubo_mode_(kubo_mode) {
+          compute_(cfg_.device, std::move(progress), cfg_.autotune), kubo_mode_(kubo_mode) {
         if (cfg_.min_energy > cfg_.max_energy) throw std::invalid_argument("KPM: Invalid energy range specified (min > max).");
         if (cfg_.min_energy != cfg_.max_energy) {
             min_ = cfg_.min_energy;

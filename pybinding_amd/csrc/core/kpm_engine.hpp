@@ -186,7 +186,8 @@ std::unique_ptr<BlockHandle> upload_block(kpmx_ctx* ctx, idx_t n, idx_t nvec, st
 template <class T>
 class HipCompute {
 public:
-    HipCompute(int device, ProgressCallback progress) : dev_(Device::instance(device)), progress_(std::move(progress)) {}
+    HipCompute(int device, ProgressCallback progress, bool autotune = false)
+        : dev_(Device::instance(device)), progress_(std::move(progress)), autotune_(autotune) {}
 
     kpmx_ctx* ctx() const { return dev_->get(); }
 
@@ -230,6 +231,14 @@ public:
                             std::copy(col.begin(), col.end(), r0.begin() + v * n);
                         }
                         auto blk = upload_block(ctx(), n, nv, r0);
+                        if (autotune_ && nv >= 4) {
+                            int lw = 0;
+                            while ((idx_t(1) << lw) < nv) ++lw;   // the block width: next power of two
+                            if (!tuned_[lw]) {
+                                check(kpmx_autotune_diagonal(h2, blk->b, nullptr));
+                                tuned_[lw] = true;
+                            }
+                        }
                         std::vector<T> mu((size_t)(nv * M_));
                         check(kpmx_moments_diagonal(h2, blk->b, M_, mu.data()));
                         for (idx_t v = 0; v < nv; ++v) {
@@ -307,6 +316,8 @@ public:
 private:
     std::shared_ptr<Device> dev_;
     ProgressCallback progress_;
+    bool autotune_ = false;
+    bool tuned_[7] = {};  // block widths 1 .. 64 already tuned on this matrix
     std::unique_ptr<MatHandle> h2_;
     uint64_t version_ = 0;
     int format_ = -1;

@@ -153,8 +153,9 @@ PYBIND11_MODULE(_kpmx, m) {
         "kpm",
         [](py::tuple shape, py::array indptr, py::array indices, py::array data, std::pair<float, float> energy,
            py::object kernel, std::string matrix_format, bool optimal_size, bool interleaved, float lanczos_precision,
-           py::object progress_callback, int device, std::string kubo_precision) {
+           py::object progress_callback, int device, std::string kubo_precision, bool autotune) {
             Config cfg;
+            cfg.autotune = autotune;
             cfg.min_energy = energy.first;
             cfg.max_energy = energy.second;
             if (!kernel.is_none()) cfg.kernel = kernel.cast<Kernel>();
@@ -202,7 +203,7 @@ PYBIND11_MODULE(_kpmx, m) {
         py::arg("energy_range") = std::make_pair(0.0f, 0.0f), py::arg("kernel") = py::none(),
         py::arg("matrix_format") = "ELL", py::arg("optimal_size") = true, py::arg("interleaved") = true,
         py::arg("lanczos_precision") = 0.002f, py::arg("progress_callback") = py::none(), py::arg("device") = 0,
-        py::arg("kubo_precision") = "fp64");
+        py::arg("kubo_precision") = "fp64", py::arg("autotune") = false);
     // The optimized Hamiltonian the engine uploads: H2 = (H - b) 2/a, optionally BFS-reordered
     // from index `src` (OptimizedHamiltonian::optimize_for), with the reorder map and slice borders.
     m.def(

@@ -35,6 +35,20 @@ def test_dos_moments_match_oracle(oracle, dtype, tol, fmt):
     assert np.max(np.abs(mu - ref)) <= tol * abs(ref[0])
 
 
+def test_autotuned_dos_moments_match_oracle(oracle):
+    """kpm(..., autotune=True): the first block of each width is timed over the step variants
+    before the moments are taken; the moments stay on the oracle."""
+    h = generators.honeycomb_supercell(40, 30, disorder=0.5, dtype=np.float64)
+    n = h.shape[0]
+    kpm = pb.kpm(h, energy_range=(-9, 9), silent=True, autotune=True)
+    M, R = 130, 16
+    mu = kpm.dos_moments(M, R)
+    a, b = oracle.scale_factors(-9, 9)
+    h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
+    ref = oracle.diag_moments(h2, oracle.random_starter(n, R, np.float64), M).mean(axis=0)
+    assert np.max(np.abs(mu - ref)) <= 1e-10 * abs(ref[0])
+
+
 def test_calc_dos_matches_oracle_reconstruction(oracle):
     h = flake()
     n = h.shape[0]
