@@ -436,8 +436,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU path (vectors split across GPUs "
-              "with one moment all-reduce) is described in DESIGN.md but not built.", file=sys.stderr)
+        print("bench.py: runs on one GPU only; a multi-GPU run (the random vectors split across GPUs with "
+              "one moment all-reduce at the end) is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     from pybinding_amd import abi

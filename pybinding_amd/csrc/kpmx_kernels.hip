@@ -618,7 +618,8 @@ __global__ void k_reduce_segments(const double* partials, const int64_t* slot_fi
 // Adjoint Chebyshev step for real H2 (symmetric), block of VW >= 4 vectors, SELL-64 layout:
 //   update:    lam_out = H2 lam1 - lam2 + ca v_j + chv (H2 v_j)
 //   gradient:  grad[e] += gscale * sum_cols (lam1[row] + cg v_j[row]) * v_j[col(e)]   for every stored entry e
-// lam1 = lambda_{j+1}, lam2 = lambda_{j+2} (lam_out may alias lam2).  The driver writes the loss
+// lam1 = lambda_{j+1}, lam2 = lambda_{j+2} (lam_out may alias lam2).  The host loop
+// (kpmx_adjoint_backward) writes the loss
 // over A_j = <v_j|v_j> and C_j = <v_j|H2|v_j> (the B_j = <v_j+1|v_j> terms become alternating
 // sums of C's through v_j+1 = H2 v_j - v_j-1), so the source term 2 alpha_j v_j + 2 gamma_j H2 v_j
 // and the explicit gradient gamma_j v_j v_j^T need only v_j, whose gathers the H2 v_j product
