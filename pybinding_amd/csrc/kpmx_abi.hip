@@ -634,7 +634,7 @@ extern "C" {
 
 const char* kpmx_last_error(void) { return g_last_error.c_str(); }
 
-int kpmx_version(void) { return 100; }
+int kpmx_version(void) { return 101; }
 
 int kpmx_device_count(int* count) {
     return guarded([&] {
