@@ -175,6 +175,8 @@ int kpmx_adjoint_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_
 int kpmx_adjoint_forward(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, int64_t checkpoint, void* mu,
                          kpmx_tape** out);
 int kpmx_adjoint_backward(kpmx_tape* tape, const double* w, double* grad);
+/* Frees a tape; the vector storage of the largest finished tape stays with the context (until it
+ * is destroyed) and is reused by the next tape that fits, so repeated gradients do not allocate. */
 int kpmx_tape_destroy(kpmx_tape* tape);
 
 /* Batched independent systems -- many disorder realizations of uneven size advanced together.

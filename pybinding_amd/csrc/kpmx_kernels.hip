@@ -658,7 +658,7 @@ __device__ __forceinline__ double reduce4(double p0, double p1, double p2, doubl
     return r;
 }
 
-template <class T, int VW, int KMAX, int RW, bool OFF32>
+template <class T, int VW, int KMAX, int RW, bool OFF32, bool WS>
 __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int64_t* __restrict__ slice_ptr,
                                                          const int32_t* __restrict__ slice_width) {
     constexpr int RPW = WAVE / VW;
@@ -667,8 +667,11 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
     constexpr int TPS = WAVE / TR;
     constexpr int Q = VW / 4;   // lanes per reduced value in reduce4
     static_assert(VW >= 4 && KMAX % 4 == 0 && RW % RPW == 0 && WAVE % TR == 0, "bad block configuration");
-    __shared__ int32_t s_idx[KMAX * TR];
-    __shared__ T s_val[KMAX * TR];
+    static_assert(!WS || KMAX * RW <= WAVE, "wave staging needs one staged entry per lane at most");
+    // WS: each wave stages its own rows' entries (entry k of row r in lane k * RW + r) and hands
+    // them out by lane shuffles, without LDS or a workgroup barrier; otherwise LDS staging
+    __shared__ int32_t s_idx[WS ? 1 : KMAX * TR];
+    __shared__ T s_val[WS ? 1 : KMAX * TR];
     const int lane = threadIdx.x % WAVE;
     const int wave = threadIdx.x / WAVE;
     const int rsub = lane / VW;
@@ -704,6 +707,25 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
             }
         }
     };
+    const int sk = lane / RW, sr = lane % RW;
+    int32_t w_idx = 0;
+    T w_val = zero(T{});
+    auto prefetch_ws = [&](int64_t tile, int64_t& pbase, int& pwidth) {
+        const int64_t slc = tile / TPS;
+        const int rr0 = (int)(tile % TPS) * TR;
+        pbase = slice_ptr[slc];
+        pwidth = slice_width[slc];
+        const int kk = pwidth < KMAX ? pwidth : KMAX;
+        const int rin = wave * RW + sr;
+        if (lane < KMAX * RW && sk < kk) {
+            w_idx = sidx[pbase + (int64_t)sk * WAVE + rr0 + rin];
+            w_val = sval[pbase + (int64_t)sk * WAVE + rr0 + rin];
+        } else {
+            const int64_t own = slc * WAVE + rr0 + rin;
+            w_idx = (int32_t)(own < a.n ? own : a.n - 1);
+            w_val = zero(T{});
+        }
+    };
     auto gather = [&](const T* __restrict__ v, int32_t c) -> T {
         if constexpr (OFF32) {
             // the whole block is below 4 GiB: a 32-bit byte offset from the uniform base
@@ -715,23 +737,30 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
     };
     int64_t next_base = 0;
     int next_width = 0;
-    if (tw.first < tw.end) prefetch(tw.first, next_base, next_width);
+    if (tw.first < tw.end) {
+        if constexpr (WS) prefetch_ws(tw.first, next_base, next_width);
+        else prefetch(tw.first, next_base, next_width);
+    }
     for (int64_t tile = tw.first; tile < tw.end; tile += tw.stride) {
         const int64_t slice = tile / TPS;
         const int r0 = (int)(tile % TPS) * TR;
         const int64_t base = next_base;
         const int width = next_width;
         const int kst = width < KMAX ? width : KMAX;
-        __syncthreads();
+        const int32_t c_idx = w_idx;
+        const T c_val = w_val;
+        if constexpr (!WS) {
+            __syncthreads();
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int e = threadIdx.x + q * BLOCK;
-            if (e < KMAX * TR) {
-                s_idx[e] = p_idx[q];
-                s_val[e] = p_val[q];
+            for (int q = 0; q < E; ++q) {
+                const int e = threadIdx.x + q * BLOCK;
+                if (e < KMAX * TR) {
+                    s_idx[e] = p_idx[q];
+                    s_val[e] = p_val[q];
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
         int rin[U];
         int64_t row[U];
         bool ok[U];
@@ -759,11 +788,16 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
         for (int k = 0; k < KMAX; ++k)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int32_t c = s_idx[k * TR + rin[u]];
+                int32_t c;
+                if constexpr (WS) c = __shfl(c_idx, k * RW + u * RPW + rsub, WAVE);
+                else c = s_idx[k * TR + rin[u]];
                 lg[u][k] = gather(lam1, c);
                 vg[u][k] = gather(vj, c);
             }
-        if (tile + tw.stride < tw.end) prefetch(tile + tw.stride, next_base, next_width);
+        if (tile + tw.stride < tw.end) {
+            if constexpr (WS) prefetch_ws(tile + tw.stride, next_base, next_width);
+            else prefetch(tile + tw.stride, next_base, next_width);
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t e0 = base + r0 + rin[u];
@@ -779,7 +813,9 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
             T acc_l = zero(T{}), acc_v = zero(T{});
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
-                const T av = s_val[k * TR + rin[u]];
+                T av;
+                if constexpr (WS) av = shfl_any(c_val, k * RW + u * RPW + rsub);
+                else av = s_val[k * TR + rin[u]];
                 acc_l = fmac(av, lg[u][k], acc_l);
                 acc_v = fmac(av, vg[u][k], acc_v);
             }
@@ -1344,12 +1380,28 @@ static void adjoint_rw_launch(AdjArgs<T> a, const AdjointLaunch& A, hipStream_t 
         if (g >= 8) g -= g % 8;
         hipLaunchKernelGGL(kernel, dim3((unsigned)(g < 1 ? 1 : g)), dim3(BLOCK), 0, stream, a, A.slice_ptr, A.slice_width);
     };
+    // variants 15..29 stage the entries per wave where they fit its lanes
+    const bool ws = A.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) == 1;
     if (A.max_width <= 4) {
-        if (off32) go(k_adjoint_sellb<T, VW, 4, RW, true>);
-        else go(k_adjoint_sellb<T, VW, 4, RW, false>);
+        if constexpr (4 * RW <= WAVE) {
+            if (ws) {
+                if (off32) go(k_adjoint_sellb<T, VW, 4, RW, true, true>);
+                else go(k_adjoint_sellb<T, VW, 4, RW, false, true>);
+                return;
+            }
+        }
+        if (off32) go(k_adjoint_sellb<T, VW, 4, RW, true, false>);
+        else go(k_adjoint_sellb<T, VW, 4, RW, false, false>);
     } else {
-        if (off32) go(k_adjoint_sellb<T, VW, 8, RW, true>);
-        else go(k_adjoint_sellb<T, VW, 8, RW, false>);
+        if constexpr (8 * RW <= WAVE) {
+            if (ws) {
+                if (off32) go(k_adjoint_sellb<T, VW, 8, RW, true, true>);
+                else go(k_adjoint_sellb<T, VW, 8, RW, false, true>);
+                return;
+            }
+        }
+        if (off32) go(k_adjoint_sellb<T, VW, 8, RW, true, false>);
+        else go(k_adjoint_sellb<T, VW, 8, RW, false, false>);
     }
 }
 
