@@ -9,5 +9,8 @@ from .chebyshev import KPM, kpm, kpm_cuda, kpm_hip, jackson_kernel, lorentz_kern
 from .results import Series, Deferred  # noqa: F401
 from . import generators  # noqa: F401
 from .model import Model  # noqa: F401
+from .lattice import (Lattice, Polygon, rectangle, regular_polygon, primitive, translational_symmetry,  # noqa: F401
+                      onsite_energy_modifier, hopping_energy_modifier, site_position_modifier,
+                      constant_potential, force_double_precision)
 
 __version__ = "0.1.0"

@@ -7,6 +7,7 @@ use it, so that what they measure and check is the boundary itself.
 import ctypes
 import os
 import re
+import weakref
 
 import numpy as np
 
@@ -46,15 +47,46 @@ def _check(rc):
         raise RuntimeError("kpmx error %d: %s" % (rc, lib().kpmx_last_error().decode()))
 
 
-class Ctx:
-    def __init__(self, device=0):
-        self.h = ctypes.c_void_p()
-        _check(lib().kpmx_ctx_create(ctypes.c_int(device), ctypes.byref(self.h)))
+def _destroy(fn_name, handle, *keep_alive):
+    """Finalizer body: free one engine handle.  `keep_alive` holds the objects the handle depends
+    on (its context, its matrix) so that they outlive it."""
+    getattr(lib(), fn_name)(handle)
+
+
+class _Handle:
+    """An engine object owned by Python: freed by close(), or when it is garbage collected (or at
+    interpreter exit), whichever comes first.  Objects made on a context keep that context alive,
+    and closing a context first closes what is still open on it (tapes before the rest)."""
+
+    _destroy_fn = None
+    _close_rank = 1
+
+    def _own(self, ctx=None, *depends_on):
+        deps = (ctx,) + depends_on if ctx is not None else depends_on
+        self._finalizer = weakref.finalize(self, _destroy, self._destroy_fn, ctypes.c_void_p(self.h.value), *deps)
+        if ctx is not None:
+            ctx._children.add(self)
 
     def close(self):
-        if self.h:
-            lib().kpmx_ctx_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        fin = getattr(self, "_finalizer", None)
+        if fin is not None:
+            fin()
+        self.h = ctypes.c_void_p()
+
+
+class Ctx(_Handle):
+    _destroy_fn = "kpmx_ctx_destroy"
+
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        self._children = weakref.WeakSet()
+        _check(lib().kpmx_ctx_create(ctypes.c_int(device), ctypes.byref(self.h)))
+        self._own()
+
+    def close(self):
+        for child in sorted(list(self._children), key=lambda c: c._close_rank):
+            child.close()
+        _Handle.close(self)
 
     def mat(self, csr, fmt=0):
         return Mat(self, csr, fmt)
@@ -93,7 +125,9 @@ class Ctx:
         return c.value, r.value, t.value
 
 
-class Mat:
+class Mat(_Handle):
+    _destroy_fn = "kpmx_mat_destroy"
+
     def __init__(self, ctx, csr, fmt=0):
         csr = csr.tocsr()
         self.dtype = np.dtype(csr.dtype)
@@ -105,14 +139,13 @@ class Mat:
         _check(lib().kpmx_mat_upload(ctx.h, ctypes.c_int(SCALAR[self.dtype]), ctypes.c_int(fmt),
                                      ctypes.c_int64(self.n), ctypes.c_int64(len(self.data)), _p(self.indptr),
                                      _p(self.indices), _p(self.data), ctypes.byref(self.h)))
-
-    def close(self):
-        if self.h:
-            lib().kpmx_mat_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self._ctx = ctx
+        self._own(ctx)
 
 
-class Block:
+class Block(_Handle):
+    _destroy_fn = "kpmx_block_destroy"
+
     def __init__(self, ctx, r0):
         r0 = np.ascontiguousarray(np.atleast_2d(r0))
         self.dtype = r0.dtype
@@ -120,11 +153,7 @@ class Block:
         self.h = ctypes.c_void_p()
         _check(lib().kpmx_block_upload(ctx.h, ctypes.c_int(SCALAR[self.dtype]), ctypes.c_int64(self.n),
                                        ctypes.c_int64(self.nvec), _p(r0), ctypes.byref(self.h)))
-
-    def close(self):
-        if self.h:
-            lib().kpmx_block_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self._own(ctx)
 
 
 def moments_diagonal(mat, block, num_moments):
@@ -195,9 +224,11 @@ def lanczos_bounds(mat, precision_percent=0.002):
     return lo.value, hi.value, loops.value
 
 
-class Batch:
+class Batch(_Handle):
     """Batched independent systems (kpmx_batch_*): `h2s` are the already scaled H2 of each
     system; they are laid out block-diagonally."""
+
+    _destroy_fn = "kpmx_batch_destroy"
 
     def __init__(self, ctx, h2s):
         import scipy.sparse as sp
@@ -216,6 +247,8 @@ class Batch:
         _check(lib().kpmx_batch_upload(ctx.h, ctypes.c_int(SCALAR[self.dtype]), ctypes.c_int64(len(h2s)),
                                        _p(self.row_offsets), ctypes.c_int64(len(self.data)), _p(self.indptr),
                                        _p(self.indices), _p(self.data), ctypes.byref(self.h)))
+        self._ctx = ctx
+        self._own(ctx)
 
     def starters(self, r0):
         """r0: (num_vectors, N) over the concatenated systems."""
@@ -238,14 +271,13 @@ class Batch:
         _check(lib().kpmx_batch_fetch(self.h, ctypes.c_int64(num_moments), _p(out)))
         return out
 
-    def close(self):
-        if self.h:
-            lib().kpmx_batch_destroy(self.h)
-            self.h = ctypes.c_void_p()
 
+class Tape(_Handle):
+    """Forward record of an adjoint calculation (kpmx_adjoint_forward).  The tape is freed by
+    close() or when it is garbage collected; it keeps its matrix and starter block alive."""
 
-class Tape:
-    """Forward record of an adjoint calculation (kpmx_adjoint_forward)."""
+    _destroy_fn = "kpmx_tape_destroy"
+    _close_rank = 0
 
     def __init__(self, mat, block, num_moments, checkpoint=64):
         self.mat = mat
@@ -254,18 +286,16 @@ class Tape:
         self.h = ctypes.c_void_p()
         _check(lib().kpmx_adjoint_forward(mat.h, block.h, ctypes.c_int64(num_moments), ctypes.c_int64(checkpoint),
                                           _p(self.mu), ctypes.byref(self.h)))
+        self._own(mat._ctx, mat, block)
 
     def backward(self, w):
         """dL/dH2 (CSR order of the uploaded matrix) for L = sum_n w[n] * mean over vectors of mu_n."""
         w = np.ascontiguousarray(w, dtype=np.float64)
         grad = np.empty(len(self.mat.data), dtype=np.float64)
+        if not self.h:
+            raise RuntimeError("the adjoint tape was already released")
         _check(lib().kpmx_adjoint_backward(self.h, _p(w), _p(grad)))
         return grad
-
-    def close(self):
-        if self.h:
-            lib().kpmx_tape_destroy(self.h)
-            self.h = ctypes.c_void_p()
 
 
 def chebyshev_series(ctx, mu, theta, pref):

@@ -99,21 +99,28 @@ class TightBinding:
 
 
 class KPMMomentsFunction(torch.autograd.Function):
+    # The tape (device memory: every vector, or the checkpoint pairs) lives as long as this node
+    # of the autograd graph: it is freed when the graph is released -- after a backward pass
+    # without retain_graph, or when the output is dropped without one -- and right away when no
+    # input needs a gradient.
     @staticmethod
     def forward(ctx, onsite, hopping, tb):
         h2 = tb.scaled(onsite.detach().cpu().numpy(), hopping.detach().cpu().numpy())
         mat = tb.ctx.mat(h2, fmt=1)
         tape = abi.Tape(mat, tb.block, tb.num_moments, tb.checkpoint)
-        ctx.tb, ctx.mat, ctx.tape = tb, mat, tape
-        ctx.scale = tb.scale
-        return torch.from_numpy(tape.mu.astype(np.float64).mean(axis=0))
+        mu = torch.from_numpy(tape.mu.astype(np.float64).mean(axis=0))
+        if any(ctx.needs_input_grad[:2]):
+            ctx.tb, ctx.mat, ctx.tape = tb, mat, tape
+            ctx.scale = tb.scale
+        else:
+            tape.close()
+            mat.close()
+        return mu
 
     @staticmethod
     def backward(ctx, grad_mu):
         tb = ctx.tb
         g2 = ctx.tape.backward(grad_mu.detach().cpu().numpy().astype(np.float64))
-        ctx.tape.close()
-        ctx.mat.close()
         a, _ = ctx.scale
         gh = g2 * (2.0 / a)  # H2 = (H - b) 2/a
         g_on = np.zeros(tb.n)

@@ -43,10 +43,8 @@ def _as_hamiltonian(model):
 
 
 class KPM:
-    """The common interface for various KPM implementations (pybinding/chebyshev.py:65-317)
-
-    It should not be created directly but via :func:`kpm`.
-    """
+    """Front object of a KPM calculation on the GPU engine (interface of
+    pybinding/chebyshev.py:65-317).  Made by :func:`kpm`, not constructed by hand."""
 
     def __init__(self, impl, model=None):
         from .model import Model
@@ -57,22 +55,22 @@ class KPM:
 
     @property
     def model(self):
-        """The tight-binding model holding the Hamiltonian"""
+        """Model this calculation was made from (None for a bare sparse matrix)"""
         return self._model
 
     @property
     def system(self):
-        """The tight-binding system (shortcut for `KPM.model.system`)"""
+        """Sites of the model (positions, sublattices, orbitals): `model.system`"""
         return self._model.system
 
     @property
     def scaling_factors(self) -> tuple:
-        """A tuple of KPM scaling factors `a` and `b`"""
+        """(a, b) such that the scaled Hamiltonian is (H - b) / a, with its spectrum inside [-1, 1]"""
         return self.impl.scaling_factors
 
     @property
     def kernel(self):
-        """The damping kernel"""
+        """Kernel whose coefficients damp the moments before reconstruction"""
         return self.impl.kernel
 
     @property
@@ -81,7 +79,8 @@ class KPM:
         return self.impl.stats
 
     def report(self, shortform=False):
-        """Return a report of the last computation"""
+        """Summary of the most recent moment computation: spectrum bounds, number of moments,
+        processed non-zeros per second and time; `shortform` gives it on one line"""
         s = self.impl.stats
         lo, hi = self.impl.energy_range
         if shortform:
@@ -105,10 +104,9 @@ class KPM:
                                "'{}' is complex".format(name))
 
     def moments(self, num_moments, alpha, beta=None, op=None):
-        r"""Calculate KPM moments in the form of expectation values
-
-        :math:`\mu_n = <\beta|op \cdot T_n(H)|\alpha>` (pybinding/chebyshev.py:117-149).
-        """
+        r"""Chebyshev moments :math:`\mu_n = <\beta|op \cdot T_n(H)|\alpha>` of the scaled Hamiltonian,
+        damped by the kernel (pybinding/chebyshev.py:117-149).  `beta` defaults to `alpha`, `op`
+        to the identity."""
         alpha = np.asarray(alpha).ravel()
         self._check_vector("alpha", alpha)
         if beta is None or len(beta) == 0:
@@ -135,7 +133,9 @@ class KPM:
         size = self.impl.size
         energy = np.asarray(energy, dtype=float)
         cols = list(np.atleast_1d(j))
-        if i < 0 or i > size or any(c < 0 or c > size for c in cols):
+        # valid indices are 0 .. size - 1 (the reference's check also lets `size` through, which
+        # would index one element past the end of the reorder map)
+        if i < 0 or i >= size or any(c < 0 or c >= size for c in cols):
             raise RuntimeError("KPM::calc_greens(i,j): invalid value for i or j.")
         gs = self.impl.calc_greens_vector(int(i), [int(c) for c in cols], energy, float(broadening))
         return gs if np.ndim(j) > 0 else gs[0]

@@ -239,6 +239,11 @@ public:
     // because the moments do not depend on the site order and a compact order keeps the gathered
     // vector rows of a sweep in L2.  Otherwise the reference's BFS reorder is used.
     void optimize_for(Indices const& idx, Scale<> scale, bool whole_system = false) {
+        for (auto const* list : {&idx.src, &idx.dest})
+            for (auto v : *list)
+                if (v < 0 || (idx_t)v >= size())
+                    throw std::runtime_error("KPM: site index " + std::to_string(v) + " is outside the system of " +
+                                             std::to_string(size()) + " sites");
         const bool loc = whole_system && is_reordered && !locality_order.empty();
         if (has_idx && original_idx == idx && loc == used_locality) return;
         timer.tic();

@@ -22,7 +22,13 @@ same row as in the reference.  The builder therefore follows, step by step:
   H[row, col] = hopping, H[col, row] = its conjugate, zeros skipped, onsite terms only when the
   lattice has onsite energy or an onsite modifier is present;
 * scalar type (cppcore/src/Model.cpp:91-98,180-204): float32 unless a modifier asks for double
-  precision, complex when any energy or modifier is complex.
+  precision, complex when any energy or modifier is complex, or when the model is periodic;
+* periodic models (cppcore/src/system/Foundation.cpp:107-114, Symmetry.cpp:6-98,
+  System.cpp:128-158, cppcore/include/hamiltonian/Hamiltonian.hpp:118-131): a `primitive` block
+  of unit cells (no shape test, no pruning), the cells outside the symmetry area dropped, and
+  for every translation direction the hoppings that leave the area through one face re-entering
+  through the opposite one, with the second site's position shifted back by the translation
+  for the modifiers and the Bloch phase exp(i k.d) = 1 (k = 0).
 """
 import inspect
 
@@ -31,9 +37,9 @@ import scipy.sparse as sp
 
 from .model import System
 
-__all__ = ['Lattice', 'Polygon', 'rectangle', 'regular_polygon', 'onsite_energy_modifier',
-           'hopping_energy_modifier', 'site_position_modifier', 'constant_potential',
-           'force_double_precision', 'build']
+__all__ = ['Lattice', 'Polygon', 'Primitive', 'TranslationalSymmetry', 'rectangle', 'regular_polygon',
+           'primitive', 'translational_symmetry', 'onsite_energy_modifier', 'hopping_energy_modifier',
+           'site_position_modifier', 'constant_potential', 'force_double_precision', 'build']
 
 
 def _cartesian(v):
@@ -232,6 +238,77 @@ def regular_polygon(num_sides, radius, angle=0):
     return Polygon([(radius * sin(a), radius * cos(a)) for a in angles])
 
 
+class Primitive:
+    """A block of a1 x a2 x a3 unit cells: every site kept, no dangling-site removal
+    (pybinding/shape.py:233-246, cppcore/src/system/Shape.cpp:5-9)."""
+
+    def __init__(self, a1=1, a2=1, a3=1):
+        self.size = np.array([int(a1), int(a2), int(a3)], dtype=np.int64)
+        if np.any(self.size <= 0):
+            raise RuntimeError("Primitive: The size must be at least 1 in every direction.")
+
+
+def primitive(a1=1, a2=1, a3=1):
+    """Repeat the unit cell `a1`, `a2`, `a3` times along the lattice vectors"""
+    return Primitive(a1, a2, a3)
+
+
+class TranslationalSymmetry:
+    """Periodicity along lattice vectors (pybinding/shape.py:322-344,
+    cppcore/src/system/Symmetry.cpp:6-98): per direction a period length in nm, 0 for one unit
+    cell, negative for none."""
+
+    def __init__(self, a1=True, a2=True, a3=True):
+        def length(v):
+            if v is False:
+                return np.float32(-1)
+            if v is True:
+                return np.float32(0)
+            return np.float32(v)
+        self.length = np.array([length(v) for v in (a1, a2, a3)], dtype=np.float32)
+        self.enabled = self.length >= 0
+
+    def area(self, lattice, size):
+        """(left, right, middle) cell-index bounds of the periodic cell inside a foundation of
+        `size` cells: `middle` cells per period, centred (Symmetry.cpp:13-40)."""
+        left = np.zeros(3, dtype=np.int64)
+        right = np.asarray(size, dtype=np.int64) - 1
+        middle = np.zeros(3, dtype=np.int64)
+        for i in range(lattice.ndim):
+            if not self.enabled[i]:
+                continue
+            norm = np.float32(np.sqrt(np.sum(lattice.vector(i).astype(np.float32) ** 2, dtype=np.float32)))
+            cells = int(np.round(np.float32(self.length[i] / norm)))
+            cells = cells if cells > 0 else 1
+            left[i] = int((size[i] - cells) / 2)   # C++ integer division (towards zero)
+            right[i] = left[i] + cells - 1
+            middle[i] = cells
+        return left, right, middle
+
+    def translations(self, lattice, uc):
+        """Distinct translation directions: every unit-cell hopping's relative index restricted to
+        each combination of the enabled directions (Symmetry.cpp:42-89, 106-127), in first-seen
+        order, with the shift of each in cells and in nm (float32)."""
+        dirs = [bool(self.enabled[i]) if i < lattice.ndim else False for i in range(3)]
+        masks = [np.array([i, j, k]) for i in range(int(dirs[0]) + 1) for j in range(int(dirs[1]) + 1)
+                 for k in range(int(dirs[2]) + 1)]
+        out = []
+        for site in uc:
+            for rel, _, _, _ in site["hoppings"]:
+                for mask in masks:
+                    d = np.asarray(rel, dtype=np.int64) * mask
+                    if not d.any() or any(np.array_equal(d, t) for t in out):
+                        continue
+                    out.append(d)
+        return out
+
+
+def translational_symmetry(a1=True, a2=True, a3=True):
+    """Periodic boundaries along the lattice vectors.  Each argument: False (none), True (a period
+    of one unit cell) or a period length in nm."""
+    return TranslationalSymmetry(a1, a2, a3)
+
+
 # -- modifiers (pybinding/modifier.py) --
 
 class _SubId:
@@ -307,21 +384,28 @@ def force_double_precision():
 
 # -- the builder --
 
-def _foundation(lattice, shape):
-    """Positions (float32, foundation order), validity after the shape and dangling-site removal,
-    and the spatial layout (cppcore/src/system/Foundation.cpp)."""
+def _foundation(lattice, shape, symmetry=None):
+    """Positions (float32, foundation order), validity after the shape and dangling-site removal
+    (or every site of a `Primitive` block) and after the symmetry area, and the spatial layout
+    (cppcore/src/system/Foundation.cpp, cppcore/src/Model.cpp:128-131)."""
     uc = lattice.optimized_unit_cell()
     nsub = len(uc)
-    lo = np.full(3, np.iinfo(np.int32).max, dtype=np.int64)
-    hi = np.full(3, np.iinfo(np.int32).min, dtype=np.int64)
-    for p in shape.vertices:
-        v = np.trunc(lattice.translate_coordinates(p)).astype(np.int64)   # cast<int>
-        lo = np.minimum(lo, v)
-        hi = np.maximum(hi, v)
     nd = lattice.ndim
-    lo[:nd] -= 1
-    hi[:nd] += 1
-    size = hi - lo + 1
+    if isinstance(shape, Primitive):
+        if np.any(shape.size[nd:] != 1):
+            raise RuntimeError("Primitive shape has more dimensions than the lattice")
+        size = shape.size.copy()
+        lo = -(size // 2)   # the block is centred: cells -size/2 .. (size-1)/2
+    else:
+        lo = np.full(3, np.iinfo(np.int32).max, dtype=np.int64)
+        hi = np.full(3, np.iinfo(np.int32).min, dtype=np.int64)
+        for p in shape.vertices:
+            v = np.trunc(lattice.translate_coordinates(p)).astype(np.int64)   # cast<int>
+            lo = np.minimum(lo, v)
+            hi = np.maximum(hi, v)
+        lo[:nd] -= 1
+        hi[:nd] += 1
+        size = hi - lo + 1
     s0, s1, s2 = (int(s) for s in size)
     origin = lattice.calc_position(lo)
     v0, v1, v2 = lattice.vector(0), lattice.vector(1), lattice.vector(2)
@@ -335,12 +419,15 @@ def _foundation(lattice, shape):
         pb = pc[:, None, :] + fb[None, :, :]                     # (s2, s1, 3)
         pos[n] = pb[:, :, None, :] + fa[None, None, :, :]
     pos = pos.reshape(-1, 3)
-    valid = shape.contains(pos[:, 0], pos[:, 1], pos[:, 2])
+    block = s0 * s1 * s2
+    grid = np.indices((s2, s1, s0)).reshape(3, -1)   # (c, b, a) for each spatial flat index
+    if isinstance(shape, Primitive):
+        valid = np.ones(nsub * block, dtype=bool)
+    else:
+        valid = shape.contains(pos[:, 0], pos[:, 1], pos[:, 2])
 
     # neighbours of every foundation site, one column per unit-cell hopping (both directions)
     nmax = max(len(s["hoppings"]) for s in uc) if nsub else 0
-    block = s0 * s1 * s2
-    grid = np.indices((s2, s1, s0)).reshape(3, -1)   # (c, b, a) for each spatial flat index
     nbr = np.full((nsub * block, max(nmax, 1)), -1, dtype=np.int64)
     for n, site in enumerate(uc):
         for h, (rel, to, _, _) in enumerate(site["hoppings"]):
@@ -357,20 +444,71 @@ def _foundation(lattice, shape):
     # whose count never changed are kept.  The end state does not depend on the visiting order.
     min_nb = lattice.min_neighbors
     safe = np.where(inb, nbr, 0)
-    while True:
+    while not isinstance(shape, Primitive):
         bad = inb & ~valid[safe]
         nbad = bad.sum(axis=1)
         kill = valid & (nbad > 0) & (count0 - nbad < min_nb)
         if not kill.any():
             break
         valid &= ~kill
+    if symmetry is not None:
+        left, right, _ = symmetry.area(lattice, (s0, s1, s2))
+        cell = np.stack([grid[2], grid[1], grid[0]])          # (a, b, c) of each spatial cell
+        inside = np.all((cell >= left[:, None]) & (cell <= right[:, None]), axis=0)
+        valid &= np.tile(inside, nsub)
     return uc, pos, valid, nbr, (s0, s1, s2)
 
 
-def _scalar_type(lattice, modifiers):
+def _boundary_hoppings(lattice, uc, symmetry, size, index):
+    """Hoppings across the periodic boundaries (cppcore/src/system/System.cpp:128-158): for each
+    translation direction d, the valid sites on the face the translation leaves from, moved by
+    d x (cells per period), connect to every valid site their unit-cell hoppings (not the
+    conjugate copies) then reach inside the foundation.  Returns [(shift in nm, {family id:
+    (rows, cols)})], one entry per translation that has any hopping."""
+    s0, s1, s2 = size
+    block = s0 * s1 * s2
+    grid = np.indices((s2, s1, s0)).reshape(3, -1)
+    cell = np.stack([grid[2], grid[1], grid[0]])               # (a, b, c)
+    dims = np.array([s0, s1, s2])[:, None]
+    left, right, middle = symmetry.area(lattice, size)
+    out = []
+    for d in symmetry.translations(lattice, uc):
+        face = np.ones(block, dtype=bool)
+        for n in range(lattice.ndim):
+            if d[n] > 0:
+                face &= cell[n] == left[n]
+            elif d[n] < 0:
+                face &= cell[n] == right[n]
+        cells = np.nonzero(face)[0]
+        shift_index = d * middle
+        shift = np.zeros(3, dtype=np.float32)
+        for n in range(lattice.ndim):
+            shift = shift + np.float32(d[n] * middle[n]) * lattice.vector(n)
+        moved = cell[:, cells] + shift_index[:, None]
+        per_family = {}
+        for n_sub, site in enumerate(uc):
+            src = index[n_sub * block + cells]
+            for rel, to, fid, conj in site["hoppings"]:
+                if conj:
+                    continue
+                nb = moved + np.asarray(rel)[:, None]
+                inb = np.all((nb >= 0) & (nb < dims), axis=0)
+                flat = ((to * s2 + nb[2]) * s1 + nb[1]) * s0 + nb[0]
+                dst = np.where(inb, index[np.where(inb, flat, 0)], -1)
+                ok = (src >= 0) & (dst >= 0)
+                if ok.any():
+                    per_family.setdefault(fid, []).append((src[ok], dst[ok]))
+        if per_family:
+            out.append((shift, {fid: (np.concatenate([p[0] for p in pairs]), np.concatenate([p[1] for p in pairs]))
+                                for fid, pairs in sorted(per_family.items())}))
+    return out
+
+
+def _scalar_type(lattice, modifiers, periodic=False):
     is_double = any(m.is_double for m in modifiers)
     energies = [s["energy"] for s in lattice.sublattices.values()] + [f["energy"] for f in lattice.hoppings.values()]
-    is_complex = any(np.any(np.imag(e) != 0) for e in energies) or any(m.is_complex for m in modifiers)
+    is_complex = (any(np.any(np.imag(e) != 0) for e in energies) or any(m.is_complex for m in modifiers)
+                  or periodic)
     if is_complex:
         return np.complex128 if is_double else np.complex64
     return np.float64 if is_double else np.float32
@@ -384,13 +522,19 @@ def _cast(values, dtype):
 
 
 def build(lattice, shape, *modifiers):
-    """Build a model: returns (hamiltonian CSR, System) in the reference's site order."""
+    """Build a model: returns (hamiltonian CSR, System) in the reference's site order.  `shape` is
+    a polygon or a `primitive` block; `modifiers` may include one `translational_symmetry`."""
     modifiers = [m for m in modifiers if m is not None]
+    symmetry = None
     for m in modifiers:
-        if not isinstance(m, _Modifier):
+        if isinstance(m, TranslationalSymmetry):
+            symmetry = m
+        elif not isinstance(m, _Modifier):
             raise TypeError("unsupported model argument: {!r}".format(m))
-    dtype = _scalar_type(lattice, modifiers)
-    uc, pos, valid, nbr, (s0, s1, s2) = _foundation(lattice, shape)
+    modifiers = [m for m in modifiers if isinstance(m, _Modifier)]
+    periodic = symmetry is not None and bool(symmetry.enabled[:lattice.ndim].any())
+    dtype = _scalar_type(lattice, modifiers, periodic)
+    uc, pos, valid, nbr, (s0, s1, s2) = _foundation(lattice, shape, symmetry if periodic else None)
     block = s0 * s1 * s2
     nsub = len(uc)
 
@@ -402,9 +546,6 @@ def build(lattice, shape, *modifiers):
     site_sub = sub_of_flat[valid]
     x, y, z = (pos[valid, k].copy() for k in range(3))
     names = [s["name"] for s in uc]
-    alias_names = {}
-    for s in uc:   # sublattices that alias another one report the original's id
-        alias_names.setdefault(s["alias_id"], s["name"])
     norb_of_sub = np.array([s["norb"] for s in uc], dtype=np.int64)
     norb = norb_of_sub[site_sub]
     ham_start = np.concatenate([[0], np.cumsum(norb)])[:-1]
@@ -454,10 +595,15 @@ def build(lattice, shape, *modifiers):
             ok = (nb >= 0) & valid[sites] & valid[np.where(nb >= 0, nb, 0)]
             fam_rows.setdefault(fid, []).append((index[sites[ok]], index[nb[ok]]))
     fams = {f["family_id"]: (name, f) for name, f in lattice.hoppings.items()}
-    for fid in sorted(fam_rows):
+    no_shift = np.zeros(3, dtype=np.float32)
+    main = {fid: (np.concatenate([p[0] for p in fam_rows[fid]]), np.concatenate([p[1] for p in fam_rows[fid]]))
+            for fid in sorted(fam_rows)}
+    boundaries = _boundary_hoppings(lattice, uc, symmetry, (s0, s1, s2), index) if periodic else []
+    groups = [(fid, r, c, no_shift) for fid, (r, c) in main.items()]
+    for shift, fams_b in boundaries:
+        groups += [(fid, r, c, shift) for fid, (r, c) in fams_b.items()]
+    for fid, r, c, shift in groups:
         name, fam = fams[fid]
-        r = np.concatenate([p[0] for p in fam_rows[fid]])
-        c = np.concatenate([p[1] for p in fam_rows[fid]])
         if r.size == 0:
             continue
         e0 = _cast(fam["energy"], dtype)
@@ -466,8 +612,9 @@ def build(lattice, shape, *modifiers):
             energy = np.full(r.size, e0[0, 0], dtype=dtype)
         else:
             energy = np.broadcast_to(e0, (r.size, kr, kc)).copy()
+        x2, y2, z2 = x[c] - shift[0], y[c] - shift[1], z[c] - shift[2]
         for m in hop_mods:
-            energy = _cast(m(energy=energy, x1=x[r], y1=y[r], z1=z[r], x2=x[c], y2=y[c], z2=z[c], hop_id=name), dtype)
+            energy = _cast(m(energy=energy, x1=x[r], y1=y[r], z1=z[r], x2=x2, y2=y2, z2=z2, hop_id=name), dtype)
         energy = energy.reshape(r.size, kr, kc)
         for p in range(kr):
             for q in range(kc):
@@ -488,5 +635,14 @@ def build(lattice, shape, *modifiers):
     if not np.all(np.isfinite(h.data)):
         raise RuntimeError("The Hamiltonian contains invalid values: NaN or INF.\n"
                            "Check the lattice and/or modifier functions.")
-    system = System(x, y, z, sublattices=site_sub, sublattice_names=names, norb=norb)
+    # sites report the id of the sublattice they alias (their own unique id otherwise); names are
+    # listed by unique id (CompressedSublattices, cppcore/src/system/CompressedSublattices.cpp)
+    names_by_uid = [None] * len(uc)
+    for site in uc:
+        names_by_uid[site["unique_id"]] = site["name"]
+    alias_of_sub = np.array([site["alias_id"] for site in uc], dtype=np.int32)
+    system = System(x, y, z, sublattices=alias_of_sub[site_sub], sublattice_names=names_by_uid, norb=norb)
+    fam_name = {fid: name for fid, (name, _) in fams.items()}
+    system.hopping_blocks = {fam_name[fid]: rc for fid, rc in main.items()}
+    system.boundaries = [(shift, {fam_name[fid]: rc for fid, rc in fb.items()}) for shift, fb in boundaries]
     return h, system

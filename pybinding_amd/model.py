@@ -13,6 +13,8 @@ class System:
     """Site positions and sublattice ids of a built model (pybinding/system.py)."""
 
     def __init__(self, x, y, z=None, sublattices=None, sublattice_names=None, norb=None):
+        self.hopping_blocks = {}  # hopping family name -> (rows, cols) of the site pairs
+        self.boundaries = []      # periodic models: (shift in nm, {family: (rows, cols)}) per translation
         self.x = np.asarray(x, dtype=np.float32)
         self.y = np.asarray(y, dtype=np.float32)
         self.z = np.zeros_like(self.x) if z is None else np.asarray(z, dtype=np.float32)
@@ -81,13 +83,15 @@ class Model:
     """
 
     def __init__(self, hamiltonian, *args):
-        from .lattice import Lattice, build
+        from .lattice import Lattice, Polygon, Primitive, build
         if isinstance(hamiltonian, Lattice):
-            if not args:
-                raise RuntimeError("a shape is needed to build a finite model")
             self.lattice = hamiltonian
-            self.shape = args[0]
-            self.modifiers = list(args[1:])
+            shapes = [a for a in args if isinstance(a, (Polygon, Primitive))]
+            if len(shapes) > 1:
+                raise RuntimeError("a model takes one shape")
+            # no shape: a single unit cell, as the reference's default Primitive(1, 1, 1)
+            self.shape = shapes[0] if shapes else Primitive()
+            self.modifiers = [a for a in args if not isinstance(a, (Polygon, Primitive))]
             self.hamiltonian, self.system = build(self.lattice, self.shape, *self.modifiers)
         else:
             self.lattice = None

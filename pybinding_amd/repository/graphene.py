@@ -16,24 +16,45 @@ hbar = 6.58211899 * 10**-16   # [eV*s] (pybinding/constants.py)
 phi0 = 2 * pi * hbar          # [V*s] magnetic flux quantum
 
 
+# Hopping shells of the honeycomb lattice (lattice vectors a1 = (a, 0), a2 = (a/2, a sqrt(3)/2),
+# A below B on the y axis): shell s -> (energy name, [(relative index, from, to), ...]).
+# Shell 1: the 3 A-B bonds of length a_cc; shell 2: the 6 A-A / B-B bonds of length a (each listed
+# once, the opposite direction is implied); shell 3: the 3 A-B bonds of length 2 a_cc.
+HOPPING_SHELLS = {
+    1: ("t", [((0, 0), "A", "B"), ((1, -1), "A", "B"), ((0, -1), "A", "B")]),
+    2: ("t_nn", [(rel, s, s) for rel in ((0, -1), (1, -1), (1, 0)) for s in ("A", "B")]),
+    3: ("t_nnn", [((1, -2), "A", "B"), ((1, 0), "A", "B"), ((-1, 0), "A", "B")]),
+}
+
+
 def monolayer(nearest_neighbors=1, onsite=(0, 0), **kwargs):
-    """Monolayer graphene with up to third-nearest-neighbour hoppings"""
+    """Monolayer graphene with hopping shells 1 .. `nearest_neighbors` (at most 3).
+
+    With the second shell the onsite energies are raised by 3 t_nn, which moves the Dirac point
+    back to zero energy (the reference's convention).  Energies: `t`, `t_nn`, `t_nnn` keywords."""
+    if not 1 <= nearest_neighbors <= len(HOPPING_SHELLS):
+        raise ValueError("nearest_neighbors must be 1, 2 or 3")
+    energies = {"t": kwargs.get("t", t), "t_nn": kwargs.get("t_nn", t_nn), "t_nnn": kwargs.get("t_nnn", 0.05)}
+    shift = 3 * energies["t_nn"] if nearest_neighbors >= 2 else 0
     lat = Lattice(a1=[a, 0], a2=[a / 2, a / 2 * sqrt(3)])
-    onsite_offset = 0 if nearest_neighbors < 2 else 3 * kwargs.get('t_nn', t_nn)
-    lat.add_sublattices(('A', [0, -a_cc / 2], onsite[0] + onsite_offset),
-                        ('B', [0, a_cc / 2], onsite[1] + onsite_offset))
-    lat.register_hopping_energies({'t': kwargs.get('t', t), 't_nn': kwargs.get('t_nn', t_nn),
-                                   't_nnn': kwargs.get('t_nnn', 0.05)})
-    lat.add_hoppings(([0, 0], 'A', 'B', 't'), ([1, -1], 'A', 'B', 't'), ([0, -1], 'A', 'B', 't'))
-    if nearest_neighbors >= 2:
-        lat.add_hoppings(([0, -1], 'A', 'A', 't_nn'), ([0, -1], 'B', 'B', 't_nn'),
-                         ([1, -1], 'A', 'A', 't_nn'), ([1, -1], 'B', 'B', 't_nn'),
-                         ([1, 0], 'A', 'A', 't_nn'), ([1, 0], 'B', 'B', 't_nn'))
-    if nearest_neighbors >= 3:
-        lat.add_hoppings(((1, -2), 'A', 'B', 't_nnn'), ((1, 0), 'A', 'B', 't_nnn'),
-                         ((-1, 0), 'A', 'B', 't_nnn'))
-    if nearest_neighbors >= 4:
-        raise RuntimeError("No more")
+    lat.add_sublattices(("A", [0, -a_cc / 2], onsite[0] + shift), ("B", [0, a_cc / 2], onsite[1] + shift))
+    lat.register_hopping_energies(energies)
+    for shell in range(1, nearest_neighbors + 1):
+        name, terms = HOPPING_SHELLS[shell]
+        lat.add_hoppings(*[(rel, src, dst, name) for rel, src, dst in terms])
+    lat.min_neighbors = 2
+    return lat
+
+
+def monolayer_4atom(onsite=(0, 0)):
+    """Graphene with a rectangular unit cell of 4 atoms (a1 = (a, 0), a2 = (0, 3 a_cc)): A, B and
+    their aliases A2, B2 shifted by (a/2, 3 a_cc/2); nearest-neighbour hoppings only."""
+    lat = Lattice(a1=[a, 0], a2=[0, 3 * a_cc])
+    lat.add_sublattices(("A", [0, -a_cc / 2], onsite[0]), ("B", [0, a_cc / 2], onsite[1]))
+    lat.add_aliases(("A2", "A", [a / 2, a_cc]), ("B2", "B", [a / 2, 2 * a_cc]))
+    inside = [((0, 0), "A", "B"), ((0, 0), "B", "A2"), ((0, 0), "A2", "B2")]
+    across = [((-1, -1), "A", "B2"), ((0, -1), "A", "B2"), ((-1, 0), "B", "A2")]
+    lat.add_hoppings(*[(rel, src, dst, t) for rel, src, dst in inside + across])
     lat.min_neighbors = 2
     return lat
 

@@ -7,17 +7,17 @@ __all__ = ['Series', 'Deferred']
 
 
 class Series:
-    """A series of data points determined by a common relation, i.e. :math:`y = f(x)`
+    """Values sampled along one independent axis, e.g. a DOS over energies.
 
     Attributes
     ----------
     variable : array_like
-        Independent variable for which the data was computed.
+        The sample points (energies, chemical potentials, ...).
     data : array_like
-        An array of values computed as a function of `variable` (1D, or 2D with one column per
-        function).
+        One value per sample point, or one row per sample point with several columns
+        (orbitals, sites).
     labels : dict
-        Plot labels: 'variable', 'data', 'title' and 'columns'.
+        Axis names for plotting: 'variable', 'data', 'title' and 'columns'.
     """
 
     def __init__(self, variable, data, labels=None):
@@ -28,13 +28,13 @@ class Series:
         self.labels = defaults
 
     def with_data(self, data):
-        """Return a copy of this result object with different data"""
+        """Shallow copy with `data` replaced (same variable and labels)"""
         result = copy(self)
         result.data = data
         return result
 
     def reduced(self):
-        """Return a copy where the data is summed over the columns"""
+        """Shallow copy whose data is the row-wise sum over its columns"""
         return self.with_data(self.data.sum(axis=1))
 
 
