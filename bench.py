@@ -75,6 +75,88 @@ def step_bytes(n, nnz, vw, s):
     return nnz * (s + 4) + 4 * (n + 1) + 3 * n * vw * s
 
 
+def periodic_graphene_model(n1, n2, disorder, seed, field=None):
+    """The periodic honeycomb supercell through the model builder: graphene.monolayer() on a
+    primitive n1 x n2 block with translational symmetry of the whole block (a complex128
+    Hamiltonian, as the reference promotes periodic models), Anderson disorder uniform in
+    [-W/2, W/2] from numpy PCG64(seed) and, optionally, a perpendicular field (Peierls phases)."""
+    import pybinding_amd as pb
+    from pybinding_amd.repository import graphene
+    rng = np.random.default_rng(seed)
+
+    @pb.onsite_energy_modifier(is_double=True)
+    def anderson(energy):
+        return energy + rng.uniform(-disorder / 2, disorder / 2, size=energy.shape)
+
+    mods = [anderson] + ([graphene.constant_magnetic_field(field)] if field else [])
+    return pb.Model(graphene.monolayer(), pb.primitive(n1, n2),
+                    pb.translational_symmetry(a1=n1 * graphene.a, a2=n2 * graphene.a), *mods)
+
+
+def model_leg(cells=(2048, 1024), M=4096, R=64):
+    """The block-SpMM workload built by the model builder, in the reference's own scalar type for
+    it (complex128, phase starters): (1) the recurrence through the C-ABI with inputs resident, in
+    the headline metric; (2) `kpm(model).calc_dos` at 2000 energies end to end (site order,
+    starters, upload, moments, damping, reconstruction), model already built."""
+    import pybinding_amd as pb
+    from pybinding_amd import abi
+    from pybinding_amd._native import load_module
+    import scipy.sparse as sp
+    native = load_module()
+    t0 = time.time()
+    model = periodic_graphene_model(cells[0], cells[1], 1.0, 2)
+    h = model.hamiltonian
+    n, nnz = h.shape[0], h.nnz
+    log("model: %d sites, %d nnz, %s (%.1fs)" % (n, nnz, h.dtype, time.time() - t0))
+    out = {"workload": "periodic honeycomb %dx%d cells through the model builder (N=%d, nnz=%d, complex128), "
+                       "Anderson W=1 eV, %d moments, %d phase starters" % (cells[0], cells[1], n, nnz, M, R)}
+    # (1) C-ABI, Hilbert site order, resident inputs
+    from pybinding_amd import generators
+    q = generators.honeycomb_cell_order(cells[0], cells[1])
+    inv = np.argsort(q)
+    hp = h[inv][:, inv].tocsr()
+    hp.sort_indices()
+    ip, ix, dv, _, _, _, _, _ = native.optimize(hp.shape, hp.indptr, hp.indices, hp.data, (-8.9, 8.9), False, [0], [0])
+    h2 = sp.csr_matrix((dv, ix, ip), shape=h.shape)
+    t0 = time.time()
+    r0 = native.random_starters(n, R, "complex128", q.astype(np.int32))
+    out["starter_draw_s"] = time.time() - t0
+    ctx = abi.Ctx(0)
+    mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
+    del r0
+    variant = abi.autotune_diagonal(mat, blk)
+    abi.moments_diagonal_run(mat, blk, M)
+    ctx.synchronize()
+    ctx.timer_start()
+    abi.moments_diagonal_run(mat, blk, M)
+    ms = ctx.timer_stop()
+    mu = abi.moments_diagonal_fetch(mat, blk, M)
+    assert np.all(np.isfinite(mu))
+    step_ms = ms / (M // 2)
+    out.update({"value": nnz * M * R / (ms / 1000) / 1e9, "unit": "GNNZ*moments/s", "ms_per_computation": ms,
+                "step_ms": step_ms, "achieved_GBps": step_bytes(n, nnz, 64, 16) / (step_ms / 1000) / 1e9,
+                "kernel_variant": variant, "mu0_over_half_n": float(mu[:, 0].real.mean() / (n / 2))})
+    blk.close()
+    mat.close()
+    ctx.close()
+    log("model c128: %.1f ms per moment computation, %.4f ms per step, %.0f GB/s algorithmic; %d phase starters "
+        "drawn in %.2f s" % (ms, step_ms, out["achieved_GBps"], R, out["starter_draw_s"]))
+    # (2) the facade end to end
+    t0 = time.perf_counter()
+    kpm = pb.kpm(model, energy_range=(-8.9, 8.9), silent=True)
+    a = kpm.scaling_factors[0]
+    broadening = float(np.float32(np.pi)) * a / (M - 0.5)
+    dos = kpm.calc_dos(np.linspace(-8.5, 8.5, 2000), broadening, num_random=R)
+    total = time.perf_counter() - t0
+    st = kpm.stats
+    assert st.num_moments == M and np.all(np.isfinite(dos.data))
+    out["facade_calc_dos"] = {"total_s": total, "moments_time_s": st.moments_time, "eps": st.eps,
+                              "moments": st.num_moments, "vectors": R, "energies": 2000}
+    log("kpm(model).calc_dos end to end: %.2f s (moment computation incl. starters and upload %.2f s)"
+        % (total, st.moments_time))
+    return out
+
+
 def cpu_baseline(cells, dtype, nvec, target_seconds=15.0):
     """Oracle restatement of the reference CPU compute on a bounded sample (fewer moments), on the
     reference's own optimized Hamiltonian: BFS reorder + interleaved slices (its defaults)."""
@@ -231,15 +313,17 @@ def reconstruction(reps=5):
 
 
 def kubo_contraction(n1=724, n2=724, M=512, R=16):
-    """Kubo-Bastin mu_nm of a Peierls graphene supercell (B = 20 T, W = 1 eV, complex128),
-    512 x 512 moments, 16 vectors: device time of the FP64 and FP32 split-precision MFMA paths,
-    and the FP32 result's error against the FP64 one."""
-    from pybinding_amd import abi, generators
+    """Kubo-Bastin mu_nm of the periodic Peierls graphene supercell built by the model builder
+    (B = 20 T, W = 1 eV, complex128), 512 x 512 moments, 16 vectors: device time of the FP64,
+    FP32-split and bf16x3 contractions, and the split results' errors against the FP64 one."""
+    from pybinding_amd import abi
     from pybinding_amd._native import load_module
     import scipy.sparse as sp
     native = load_module()
     t0 = time.time()
-    h, x, y = generators.peierls_honeycomb_supercell(n1, n2, 20.0, disorder=1.0)
+    model = periodic_graphene_model(n1, n2, 1.0, 5, field=20.0)
+    h = model.hamiltonian
+    x = model.system.x
     n = h.shape[0]
     ip, ix, dv, rmap, _, _, _, (a, b) = native.optimize(h.shape, h.indptr, h.indices, h.data, (-9.5, 9.5), True,
                                                         [0], [0])
@@ -267,7 +351,8 @@ def kubo_contraction(n1=724, n2=724, M=512, R=16):
         out[name + "_max_rel_error_vs_fp64"] = float(np.abs(results[name] - results["fp64"]).max() / ref)
     flops = 8.0 * M * M * n * R
     out.update({"contraction_flop": flops,
-                "workload": "Peierls graphene %dx%d cells (%d sites), B=20 T, W=1 eV, %dx%d moments, %d vectors"
+                "workload": "periodic Peierls graphene %dx%d cells through the model builder (%d sites), B=20 T, W=1 eV, "
+                            "%dx%d moments, %d vectors"
                             % (n1, n2, n, M, M, R)})
     log("kubo: fp64 %.2f s, fp32-split %.2f s (error %.2e), bf16x3 %.2f s (error %.2e)"
         % (out["fp64_s"], out["fp32_split_s"], out["fp32_split_max_rel_error_vs_fp64"], out["bf16x3_s"],
@@ -409,6 +494,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the fp32 variant of the block-SpMM workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient cost run")
     ap.add_argument("--no-kubo", action="store_true", help="skip the Kubo contraction run")
+    ap.add_argument("--no-model", action="store_true",
+                    help="skip the complex128 run of the workload built by the model builder")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 2 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -447,7 +534,9 @@ def main():
     h2, rmap, borders, native = build_workload(args.cells[0], args.cells[1], 0.5, dtype, args.order)
     n, nnz = h2.shape[0], h2.nnz
     M, R = args.moments, args.vectors
+    t_draw = time.time()
     r0 = starters(native, n, R, dtype, rmap)
+    t_draw = time.time() - t_draw
 
     ctx = abi.Ctx(0)
     mat = ctx.mat(h2, fmt=1 if args.format == "SELL" else 0)
@@ -483,7 +572,7 @@ def main():
         % (gpu_ms / args.steps, step_ms, achieved / 1e9))
 
     traffic_key = "%dx%d %s v%d %s %s" % (args.cells[0], args.cells[1], dtype.name, R, args.format, args.order)
-    extra = {}
+    extra = {"starter_draw_s": t_draw}
     cg, rg, tg = ctx.stream_probe(1 << 31)
     step_traffic_gbps = (counted_traffic(traffic_key) or float("nan")) / (step_ms / 1000) / 1e9
     extra["stream_probe"] = {"copy_GBps": cg, "read_GBps": rg, "triad_GBps": tg,
@@ -514,6 +603,8 @@ def main():
         extra["fp32_variant"] = fp32_variant(args.cells, M, R, args.format, args.order)
     if not args.no_flake:
         extra["plain_dos_flake"] = plain_dos_flake()
+    if not args.no_model and dtype == np.float64:
+        extra["model_complex128"] = model_leg(args.cells, M, R)
     if not args.no_batched:
         extra["batched_realizations"] = batched_realizations()
     if not args.no_adjoint:

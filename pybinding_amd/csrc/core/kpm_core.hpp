@@ -291,12 +291,18 @@ public:
         return out;
     }
 
-    size_t num_nonzeros(idx_t num_moments, bool optimal_size) const {
+    // Processed matrix entries (OptimizedHamiltonian.cpp:163-192): CSR counts the stored entries of
+    // the rows a step covers, ELL counts rows x the ELL width (padding included).
+    size_t num_nonzeros(idx_t num_moments, bool optimal_size, bool ell = false) const {
+        size_t width = 0;
+        if (ell)
+            for (idx_t r = 0; r < h2.rows; ++r) width = std::max(width, (size_t)(h2.indptr[(size_t)r + 1] - h2.indptr[(size_t)r]));
+        auto entries = [&](idx_t rows) { return ell ? (size_t)rows * width : (size_t)h2.indptr[(size_t)rows]; };
         size_t result = 0;
         if (!optimal_size) {
-            result = (size_t)num_moments * (size_t)h2.nnz();
+            result = (size_t)num_moments * entries(h2.rows);
         } else {
-            for (idx_t n = 0; n < num_moments; ++n) result += (size_t)h2.indptr[(size_t)slice_map.optimal_size(n, num_moments)];
+            for (idx_t n = 0; n < num_moments; ++n) result += entries(slice_map.optimal_size(n, num_moments));
         }
         if (optimized_idx.is_diagonal()) result /= 2;
         return result;
@@ -431,11 +437,13 @@ struct Stats {
     Chrono hamiltonian_timer, moments_timer;
 
     template <class T>
-    void reset(idx_t nm, OptimizedHamiltonian<T> const& oh, AlgorithmConfig const& ac, idx_t mult = 1) {
+    void reset(idx_t nm, OptimizedHamiltonian<T> const& oh, AlgorithmConfig const& ac, idx_t mult = 1,
+               MatrixFormat fmt = MatrixFormat::CSR) {
         num_moments = nm;
         uses_full_system = oh.slice_map.uses_full_system(nm);
-        nnz = oh.num_nonzeros(nm, false);
-        opt_nnz = oh.num_nonzeros(nm, ac.optimal_size);
+        const bool ell = fmt == MatrixFormat::ELL;
+        nnz = oh.num_nonzeros(nm, false, ell);
+        opt_nnz = oh.num_nonzeros(nm, ac.optimal_size, ell);
         vec = oh.num_vec_elements(nm, false);
         opt_vec = oh.num_vec_elements(nm, ac.optimal_size);
         multiplier = (double)mult;

@@ -58,7 +58,7 @@ public:
         auto alg = cfg_.algorithm;
         alg.optimal_size = false;
         oh_.optimize_for({0, 0}, scaling_factors(), true);
-        stats_.reset(num_moments, oh_, alg);
+        stats_.reset(num_moments, oh_, alg, 1, cfg_.matrix_format);
         auto starter = constant_starter(oh_, alpha);
         const idx_t M = round_num_moments(num_moments);
         std::vector<T> data;
@@ -85,7 +85,7 @@ public:
         const idx_t M = cfg_.kernel.required_num_moments(broadening / scale.a);
         const idx_t K = (idx_t)idx.size();
         oh_.optimize_for(Indices(idx, idx), scale);
-        stats_.reset(M, oh_, cfg_.algorithm, K);
+        stats_.reset(M, oh_, cfg_.algorithm, K, cfg_.matrix_format);
         auto starter = unit_starter(oh_);
         BatchDiagonalMoments<T> m{M, K, false, {}};
         timed_compute(&m, starter, cfg_.algorithm);
@@ -100,7 +100,7 @@ public:
         auto alg = cfg_.algorithm;
         alg.optimal_size = false;
         oh_.optimize_for({0, 0}, scale, true);
-        stats_.reset(M, oh_, alg, num_random);
+        stats_.reset(M, oh_, alg, num_random, cfg_.matrix_format);
         auto starter = random_starter(oh_);
         BatchDiagonalMoments<T> m{M, num_random, true, {}};
         timed_compute(&m, starter, alg);
@@ -113,7 +113,7 @@ public:
         auto alg = cfg_.algorithm;
         alg.optimal_size = false;
         oh_.optimize_for({0, 0}, scaling_factors(), true);
-        stats_.reset(num_moments, oh_, alg, num_random);
+        stats_.reset(num_moments, oh_, alg, num_random, cfg_.matrix_format);
         auto starter = random_starter(oh_);
         BatchDiagonalMoments<T> m{num_moments, num_random, true, {}};
         timed_compute(&m, starter, alg);
@@ -128,7 +128,7 @@ public:
         const auto scale = scaling_factors();
         const idx_t M = cfg_.kernel.required_num_moments(broadening / scale.a);
         oh_.optimize_for(Indices(row, cols), scale);
-        stats_.reset(M, oh_, cfg_.algorithm);
+        stats_.reset(M, oh_, cfg_.algorithm, 1, cfg_.matrix_format);
         if (oh_.optimized_idx.is_diagonal()) {
             DiagonalMoments<T> m{M, {}};
             timed_compute(&m, unit_starter(oh_), cfg_.algorithm);
@@ -153,7 +153,7 @@ public:
         auto alg = cfg_.algorithm;
         alg.optimal_size = false;
         oh_.optimize_for({0, 0}, scale, true);
-        stats_.reset(M, oh_, alg, num_random);
+        stats_.reset(M, oh_, alg, num_random, cfg_.matrix_format);
         const Csr<T> vl = velocity(h_, left), vr = velocity(h_, right);
         stats_.moments_timer.tic();
         auto mu = compute_.kubo(oh_, format(), vl, vr, M, num_random, kubo_mode_);

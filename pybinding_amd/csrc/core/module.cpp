@@ -48,6 +48,14 @@ struct KPMCore {
 };
 
 py::array_t<double> as_array(std::vector<double> const& v) { return py::array_t<double>(v.size(), v.data()); }
+
+// numpy array -> std::vector in one copy (a std::vector parameter would convert element by element)
+template <class T>
+using carray = py::array_t<T, py::array::c_style | py::array::forcecast>;
+template <class T>
+std::vector<T> to_vec(carray<T> const& a) {
+    return std::vector<T>(a.data(), a.data() + a.size());
+}
 py::array_t<cd> as_array(std::vector<cd> const& v) { return py::array_t<cd>(v.size(), v.data()); }
 
 }  // namespace
@@ -92,10 +100,11 @@ PYBIND11_MODULE(_kpmx, m) {
         .def_property_readonly("dtype", [](KPMCore& k) { return k.dtype; })
         .def_property_readonly("size", [](KPMCore& k) { return k.core->size(); })
         .def_property_readonly("is_complex", [](KPMCore& k) { return k.core->is_complex(); })
-        .def("set_locality_order", [](KPMCore& k, std::vector<int32_t> order) { k.core->set_locality_order(order); })
+        .def("set_locality_order",
+             [](KPMCore& k, carray<int32_t> order) { k.core->set_locality_order(to_vec(order)); })
         .def("moments",
-             [](KPMCore& k, idx_t num_moments, std::vector<cd> alpha, std::vector<cd> beta, py::object op) {
-                 return as_array(k.core->moments(num_moments, alpha, beta, csr_cd(op)));
+             [](KPMCore& k, idx_t num_moments, carray<cd> alpha, carray<cd> beta, py::object op) {
+                 return as_array(k.core->moments(num_moments, to_vec(alpha), to_vec(beta), csr_cd(op)));
              })
         .def("calc_ldos_indices",
              [](KPMCore& k, std::vector<double> energy, double broadening, std::vector<idx_t> idx) {
@@ -139,9 +148,10 @@ PYBIND11_MODULE(_kpmx, m) {
                  return out;
              })
         .def("calc_conductivity",
-             [](KPMCore& k, std::vector<double> mu, double broadening, double temperature, std::vector<float> left,
-                std::vector<float> right, idx_t num_random, idx_t num_points) {
+             [](KPMCore& k, std::vector<double> mu, double broadening, double temperature, carray<float> left_a,
+                carray<float> right_a, idx_t num_random, idx_t num_points) {
                  std::vector<cd> r;
+                 const std::vector<float> left = to_vec(left_a), right = to_vec(right_a);
                  {
                      py::gil_scoped_release nogil;
                      r = k.core->conductivity(left, right, mu, broadening, temperature, num_random, num_points);
@@ -237,25 +247,27 @@ PYBIND11_MODULE(_kpmx, m) {
         py::arg("reorder") = true, py::arg("src") = std::vector<idx_t>{0}, py::arg("dest") = std::vector<idx_t>{0});
 
     // Reference random starters (std::mt19937, default seed), `nvec` columns in original order,
-    // returned vector-major and permuted by `reorder_map` (empty: identity).
+    // returned vector-major and permuted by `reorder_map` (empty: identity); drawn `group`
+    // columns per call of the starter (0: all at once).
     m.def(
         "random_starters",
-        [](idx_t n, idx_t nvec, std::string dtype, std::vector<int32_t> reorder_map) -> py::array {
+        [](idx_t n, idx_t nvec, std::string dtype, py::array_t<int32_t, py::array::c_style | py::array::forcecast> reorder_map,
+           idx_t group) -> py::array {
+            if (reorder_map.size() != 0 && reorder_map.size() != n)
+                throw std::invalid_argument("random_starters: reorder_map must be empty or of length n");
             auto run = [&](auto tag) -> py::array {
                 using T = decltype(tag);
                 Csr<T> dummy;
                 dummy.rows = n;
                 OptimizedHamiltonian<T> oh(&dummy, false);
-                oh.reorder_map = reorder_map;
+                oh.reorder_map.assign(reorder_map.data(), reorder_map.data() + reorder_map.size());
                 auto st = random_starter(oh);
                 py::array_t<T> out({(py::ssize_t)nvec, (py::ssize_t)n});
                 T* p = out.mutable_data();
                 {
                     py::gil_scoped_release nogil;
-                    for (idx_t v = 0; v < nvec; ++v) {
-                        auto col = st.make();
-                        std::copy(col.begin(), col.end(), p + v * n);
-                    }
+                    const idx_t g = group > 0 ? group : std::max<idx_t>(nvec, 1);
+                    for (idx_t v = 0; v < nvec; v += g) st.make_block(std::min(g, nvec - v), p + v * n);
                 }
                 return out;
             };
@@ -265,7 +277,8 @@ PYBIND11_MODULE(_kpmx, m) {
             if (dtype == "complex128") return run(cd{});
             throw std::invalid_argument("unsupported dtype");
         },
-        py::arg("n"), py::arg("nvec"), py::arg("dtype"), py::arg("reorder_map") = std::vector<int32_t>{});
+        py::arg("n"), py::arg("nvec"), py::arg("dtype"),
+        py::arg("reorder_map") = py::array_t<int32_t>(0), py::arg("group") = 0);
 
     // The reconstructions on their own (the KPM methods call them after the moments): KuboBastin
     // of damped M x M moments, SpectralDensity and GreensFunction of damped moments; the sums run

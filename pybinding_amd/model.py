@@ -67,12 +67,20 @@ class System:
         """Hamiltonian indices of all orbitals of `site`"""
         return list(range(int(self._offsets[site]), int(self._offsets[site + 1])))
 
+    def with_data(self, data):
+        """The sites of this system with one value each (a StructureMap)"""
+        from .results import StructureMap
+        return StructureMap(data, self)
+
+    @property
+    def sub(self):
+        """Sublattice name of every site"""
+        return self.sublattices_str
+
     def __getitem__(self, mask):
         idx = np.nonzero(mask)[0] if np.asarray(mask).dtype == bool else np.asarray(mask)
-        s = System(self.x[idx], self.y[idx], self.z[idx], self.sublattices[idx], self.sublattice_names,
-                   self.norb[idx])
-        s.sub = s.sublattices_str
-        return s
+        return System(self.x[idx], self.y[idx], self.z[idx], self.sublattices[idx], self.sublattice_names,
+                      self.norb[idx])
 
 
 class Model:

@@ -3,7 +3,7 @@ from copy import copy
 
 import numpy as np
 
-__all__ = ['Series', 'Deferred']
+__all__ = ['Series', 'StructureMap', 'Deferred']
 
 
 class Series:
@@ -36,6 +36,71 @@ class Series:
     def reduced(self):
         """Shallow copy whose data is the row-wise sum over its columns"""
         return self.with_data(self.data.sum(axis=1))
+
+
+class StructureMap:
+    """Per-site values over the sites of a system: the data beside the sites' positions and
+    sublattices (the interface of pybinding/results.py:217-300,387-420 that `SpatialLDOS` uses).
+    Indexing (a mask, indices, or a shape) selects sites and their values together."""
+
+    def __init__(self, data, system):
+        data = np.atleast_1d(data)
+        if data.size != system.num_sites:
+            raise RuntimeError("Data size doesn't match number of sites")
+        self._data = data
+        self._system = system
+
+    @property
+    def data(self):
+        """One value per site"""
+        return self._data
+
+    @data.setter
+    def data(self, value):
+        self._data = value
+
+    @property
+    def num_sites(self):
+        return self._system.num_sites
+
+    @property
+    def positions(self):
+        return self._system.positions
+
+    @property
+    def x(self):
+        return self._system.x
+
+    @property
+    def y(self):
+        return self._system.y
+
+    @property
+    def z(self):
+        return self._system.z
+
+    @property
+    def sublattices(self):
+        return self._system.sublattices
+
+    @property
+    def sub(self):
+        """Sublattice name of every site"""
+        return self._system.sublattices_str
+
+    def with_data(self, data):
+        """Shallow copy mapping `data` to the same sites"""
+        result = copy(self)
+        result._data = np.atleast_1d(data)
+        return result
+
+    def find_nearest(self, position, sublattice=""):
+        return self._system.find_nearest(position, sublattice)
+
+    def __getitem__(self, idx):
+        if hasattr(idx, "contains"):
+            idx = idx.contains(*self.positions)
+        return StructureMap(self._data[idx], self._system[idx])
 
 
 class Deferred:

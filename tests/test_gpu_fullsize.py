@@ -127,34 +127,6 @@ def test_adjoint_full_size():
         tb.close()
 
 
-def test_kubo_full_size_precisions():
-    """Peierls graphene (B = 20 T, W = 1 eV, complex128) of 1.05 M sites, 16 vectors, 128 x 128
-    moments: the FP32-split and bf16x3 contractions against the FP64 one."""
-    from pybinding_amd._native import load_module
-    native = load_module()
-    h, x, _ = generators.peierls_honeycomb_supercell(724, 724, 20.0, disorder=1.0)
-    ip, ix, dv, rmap, _, _, _, (a, b) = native.optimize(h.shape, h.indptr, h.indices, h.data, (-9.5, 9.5), True,
-                                                        [0], [0])
-    h2 = sp.csr_matrix((dv, ix, ip), shape=h.shape)
-    xo = x.astype(np.float32)[np.argsort(rmap)]
-    hh = h2.tocoo()
-    v = sp.csr_matrix(((xo[hh.row] - xo[hh.col]).astype(np.float64) * hh.data * (a / 2), (hh.row, hh.col)),
-                      shape=h.shape)
-    r0 = native.random_starters(h.shape[0], 16, "complex128", rmap)
-    ctx = abi.Ctx(0)
-    try:
-        mat, vm, blk = ctx.mat(h2, fmt=1), ctx.mat(v, fmt=1), ctx.block(r0)
-        mus = {mode: abi.kubo_moments(mat, vm, vm, blk, 128, mode=mode) for mode in (0, 1, 2)}
-        for hnd in (blk, vm, mat):
-            hnd.close()
-    finally:
-        ctx.close()
-    scale = np.abs(mus[0]).max()
-    assert np.all(np.isfinite(mus[0])) and scale > 0
-    assert np.abs(mus[1] - mus[0]).max() <= 1e-6 * scale
-    assert np.abs(mus[2] - mus[0]).max() <= 1e-6 * scale
-
-
 def test_batched_full_size(oracle):
     """256 square flakes of 10^4 to 5 x 10^5 sites (31 M in all), 1024 moments, 4 vectors each, one
     graph: every system has mu_0 = N_s / 2 exactly and |mu_n| <= 2 mu_0, and the three smallest

@@ -19,7 +19,8 @@ def flake(dtype=np.float64):
     return (h + sp.diags(d)).tocsr().astype(dtype)
 
 
-@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-4), (np.complex128, 1e-10)])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-4), (np.complex128, 1e-10),
+                                       (np.complex64, 1e-4)])
 @pytest.mark.parametrize("fmt", ["ELL", "CSR"])
 def test_dos_moments_match_oracle(oracle, dtype, tol, fmt):
     h = flake(dtype)

@@ -60,3 +60,26 @@ def test_uniform_is_canonical(oracle):
     u = oracle.uniform(n, np.float64)
     d = np.random.RandomState(5489).randint(0, 2**32, size=2 * n, dtype=np.uint64).astype(np.float64)
     np.testing.assert_array_equal(u, (d[0::2] + d[1::2] * 2.0**32) / 2.0**64)
+
+
+def test_uniform_f32_is_canonical(oracle):
+    # generate_canonical<float,24> takes one draw: float(d) / 2^32, clamped below 1
+    n = 5000
+    u = oracle.uniform(n, np.float32)
+    d = np.random.RandomState(5489).randint(0, 2**32, size=n, dtype=np.uint64)
+    ref = d.astype(np.float32) / np.float32(2.0**32)
+    ref = np.where(ref >= 1, np.nextafter(np.float32(1), np.float32(0)), ref)
+    np.testing.assert_array_equal(u, ref)
+
+
+def test_complex64_phases(oracle):
+    # complex<float> starters: exp(i * 2 pi_f * u) with u the float draws (Starter.cpp:72-82)
+    n = 20000
+    z = oracle.random_starter(n, 2, np.complex64)
+    assert z.dtype == np.complex64
+    u = oracle.uniform(2 * n, np.float32).reshape(2, n)
+    theta = (np.float32(2) * np.float32(np.pi)) * u
+    np.testing.assert_allclose(np.abs(z), 1.0, rtol=0, atol=3e-7)
+    np.testing.assert_allclose(z, np.exp(1j * theta.astype(np.float64)), rtol=0, atol=4e-7)
+    # the two columns continue one stream
+    np.testing.assert_array_equal(z.ravel(), oracle.random_starter(2 * n, 1, np.complex64)[0])
