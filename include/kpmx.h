@@ -132,6 +132,17 @@ int kpmx_ctx_set_graph_policy(kpmx_ctx* ctx, int policy);
 int kpmx_moments_diagonal_run(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments);
 int kpmx_moments_diagonal_fetch(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu);
 
+/* kpmx_moments_diagonal for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same inputs, output and kernels as kpmx_moments_diagonal, but the recurrence's work
+ * vectors, partial and raw sums and the collected moments live in a static device array of
+ * 64 MiB per GPU (held by the process until it ends): 2 x n x (vector width) scalars, plus
+ * (num_moments / 2) x 4 x (vector width) doubles of sums, plus num_vectors x num_moments
+ * scalars, plus one step's partial sums, must fit, otherwise KPMX_EINVAL with the sizes in the
+ * message.  No device memory is allocated or freed (h2 and r0 are uploaded beforehand).  Calls
+ * that overlap on one GPU share the array and race. */
+int kpmx_moments_diagonal_noalloc(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu);
+
 /* GenericMoments: mu_n = <beta| op T_n(H~) |r0> for the first vector of r0
  * (cppcore/src/kpm/default/collectors.cpp:36-58).  op may be null (op_nnz = 0): identity.
  * beta: host, n scalars.  op: host CSR (n x n).  mu: host, num_moments scalars. */

@@ -162,6 +162,14 @@ def moments_diagonal(mat, block, num_moments):
     return out
 
 
+def moments_diagonal_noalloc(mat, block, num_moments):
+    """kpmx_moments_diagonal_noalloc: the diagonal moments with the recurrence's buffers in the
+    library's static device scratch array (no device allocation)."""
+    out = np.empty((block.nvec, num_moments), dtype=mat.dtype)
+    _check(lib().kpmx_moments_diagonal_noalloc(mat.h, block.h, ctypes.c_int64(num_moments), _p(out)))
+    return out
+
+
 def autotune_diagonal(mat, block):
     """Pick the fastest tile shape of the diagonal step kernel for this matrix and block."""
     v = ctypes.c_int()

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -102,9 +103,34 @@ bool is_complex(int s) { return s >= 2; }
 
 }  // namespace
 
+// Device scratch of the allocation-free diagonal entry point (kpmx_moments_diagonal_noalloc):
+// one static array per GPU, part of the library's code object and held until the process ends.
+// In host code `scratch` is only a symbol: its device address comes from hipGetSymbolAddress,
+// taken once per device with that device current (scratch_address, called by kpmx_ctx_create).
+constexpr size_t KPMX_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[KPMX_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // device address of `scratch`, per device
+
+void* scratch_address(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[(size_t)device]) {
+        void* p = nullptr;
+        KPMX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        KPMX_REQUIRE(p != nullptr, "no device address for the scratch array");
+        g_scratch_ptr[(size_t)device] = p;
+    }
+    return g_scratch_ptr[(size_t)device];
+}
+}  // namespace
+
 struct kpmx_ctx {
     int device = 0;
     int num_cu = 256;
+    char* scratch = nullptr;  // this device's `scratch` array (device address)
     hipStream_t stream = nullptr;
     Workspace partials, sums, work0, work1, aux0, aux1, aux2, panel_l, panel_r;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -319,14 +345,22 @@ int64_t partials_chunk(size_t per_step_bytes, int64_t steps) {
     return std::max<int64_t>(1, std::min<int64_t>(steps, (int64_t)(budget / per_step_bytes)));
 }
 
+// Buffers a caller hands to run_recurrence instead of the context's grow-only workspaces.
+struct FixedBuffers {
+    void* w0 = nullptr;
+    void* w1 = nullptr;
+    double* partials = nullptr;
+    size_t partials_bytes = 0;
+};
+
 template <class PerStep, class PostStep>
 void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, const void* w, double* sums,
-                    PerStep per_step, PostStep post_step, int tune = -1) {
+                    PerStep per_step, PostStep post_step, int tune = -1, const FixedBuffers* fixed = nullptr) {
     kpmx_ctx* ctx = h2->ctx;
     const int64_t n = h2->rows;
     const size_t sb = scalar_bytes(h2->scalar);
-    void* W0 = ctx->work0.get((size_t)n * t.vw * sb);
-    void* W1 = ctx->work1.get((size_t)n * t.vw * sb);
+    void* W0 = fixed ? fixed->w0 : ctx->work0.get((size_t)n * t.vw * sb);
+    void* W1 = fixed ? fixed->w1 : ctx->work1.get((size_t)n * t.vw * sb);
 
     StepLaunch L = base_launch(h2, t.vw, true);
     L.mode = mode;
@@ -338,8 +372,14 @@ void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, 
     double* partials = nullptr;
     if (mode != MODE_NONE) {
         const size_t per_step_bytes = (size_t)L.num_wg * width * sizeof(double);
-        chunk = partials_chunk(per_step_bytes, steps);
-        partials = static_cast<double*>(ctx->partials.get(per_step_bytes * chunk));
+        if (fixed) {
+            KPMX_REQUIRE(fixed->partials_bytes >= per_step_bytes, "partial-sum buffer too small");
+            chunk = std::max<int64_t>(1, std::min<int64_t>(steps, (int64_t)(fixed->partials_bytes / per_step_bytes)));
+            partials = fixed->partials;
+        } else {
+            chunk = partials_chunk(per_step_bytes, steps);
+            partials = static_cast<double*>(ctx->partials.get(per_step_bytes * chunk));
+        }
     }
     int64_t chunk_start = 0;
     for (int64_t k = 0; k < steps; ++k) {
@@ -682,6 +722,7 @@ int kpmx_ctx_create(int device, kpmx_ctx** out) {
             KPMX_HIP(hipGetDeviceProperties(&prop, device));
             ctx->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
             KPMX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+            ctx->scratch = static_cast<char*>(scratch_address(device));
         } catch (...) {
             delete ctx;
             throw;
@@ -876,6 +917,59 @@ int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_
         KPMX_HIP(hipSetDevice(h2->ctx->device));
         diag_run(h2, r0, num_moments);
         diag_fetch(h2, r0, num_moments, mu);
+    });
+}
+
+// kpmx_moments_diagonal without device allocation: the work vectors, partial sums, raw sums and
+// collected moments of the recurrence live in this device's `scratch` array (layout below, each
+// piece 256-byte aligned); nothing in it needs clearing (every launch overwrites what it reads
+// later).  Concurrent calls on one device share the array and race.
+int kpmx_moments_diagonal_noalloc(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu) {
+    return guarded([&] {
+        KPMX_REQUIRE(h2 && r0 && mu, "null argument");
+        check_pair(h2, r0);
+        const int64_t M = num_moments;
+        KPMX_REQUIRE(M >= 2 && M % 2 == 0, "num_moments must be even and >= 2");
+        kpmx_ctx* ctx = h2->ctx;
+        KPMX_HIP(hipSetDevice(ctx->device));
+        KPMX_REQUIRE(ctx->scratch != nullptr, "no scratch array on this device");
+        const int64_t steps = M / 2;
+        const size_t sb = scalar_bytes(h2->scalar);
+        auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+        int vw_max = 1;
+        size_t part_step = 0;
+        for (const Tile& t : r0->tiles) {
+            vw_max = std::max(vw_max, t.vw);
+            StepLaunch L = base_launch(h2, t.vw, true);
+            L.mode = MODE_DIAG;
+            L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
+            L.num_wg = step_grid(L);
+            part_step = std::max(part_step, (size_t)L.num_wg * 4 * t.vw * sizeof(double));
+        }
+        // layout: [W0 | W1 | raw sums (steps x 4 vw_max) | moments (nvec x M) | partial sums (rest)]
+        const size_t work = align((size_t)h2->rows * vw_max * sb);
+        const size_t sums = align((size_t)steps * 4 * vw_max * sizeof(double));
+        const size_t out = align((size_t)r0->nvec * M * sb);
+        const size_t fixed_part = 2 * work + sums + out;
+        if (fixed_part + part_step > KPMX_SCRATCH_BYTES)
+            throw Error{KPMX_EINVAL, "kpmx_moments_diagonal_noalloc: this matrix, block and moment count need " +
+                                         std::to_string(fixed_part + part_step) + " bytes of device scratch; the limit is " +
+                                         std::to_string(KPMX_SCRATCH_BYTES) + " bytes"};
+        FixedBuffers fb;
+        fb.w0 = ctx->scratch;
+        fb.w1 = ctx->scratch + work;
+        double* d_sums = reinterpret_cast<double*>(ctx->scratch + 2 * work);
+        char* d_mu = ctx->scratch + 2 * work + sums;
+        fb.partials = reinterpret_cast<double*>(ctx->scratch + fixed_part);
+        fb.partials_bytes = KPMX_SCRATCH_BYTES - fixed_part;
+        for (const Tile& t : r0->tiles) {
+            run_recurrence(h2, t, MODE_DIAG, steps, nullptr, d_sums, [](StepLaunch&, int64_t) {},
+                           [](StepLaunch const&, int64_t) {}, -1, &fb);
+            launch_diag_collect(h2->scalar, d_sums, t.vw, t.cols, M, t.c0, d_mu, ctx->stream);
+            KPMX_HIP(hipGetLastError());
+        }
+        KPMX_HIP(hipMemcpyAsync(mu, d_mu, (size_t)r0->nvec * M * sb, hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
 

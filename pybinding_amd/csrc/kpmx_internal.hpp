@@ -89,6 +89,10 @@ constexpr int NUM_TUNE_VARIANTS = 30;  // tile shape x grid x staging (workgroup
 int64_t step_rows_per_tile(int format, int vw);
 void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
                             hipStream_t stream);
+// diagonal moments of `cols` columns from the raw per-step sums (steps x 4 vw doubles) into mu
+// (device; column c at mu + (c0 + c) * M scalars of the scalar type's real kind, complex as pairs)
+void launch_diag_collect(int scalar, const double* sums, int vw, int64_t cols, int64_t M, int64_t c0, void* mu,
+                         hipStream_t s);
 void launch_gather_block(int scalar, const void* src, int64_t n, int64_t c0, int64_t cols, int vw, void* dst,
                          hipStream_t s);
 void launch_scatter_block(int scalar, const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst,

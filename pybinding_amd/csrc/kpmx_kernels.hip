@@ -1189,6 +1189,53 @@ void launch_reduce_partials(const double* partials, int64_t num_wg, int width, i
                        sums);
 }
 
+// The diagonal collector (cppcore/src/kpm/default/collectors.cpp:5-34) on the device: raw
+// per-step sums s[k][4 vw] -> mu_n of `cols` columns (column c at mu + (c0 + c) M, complex as
+// {re, im} pairs), with the same double arithmetic as the host collector:
+// mu_0 = A_0 / 2, mu_1 = B_0, mu_2k = 2 (A_k - mu_0), mu_2k+1 = 2 B_k - mu_1.
+template <class R>
+__global__ void k_diag_collect(const double* __restrict__ s, int vw, int64_t cols, int64_t M, int cplx,
+                               R* __restrict__ mu, int64_t c0) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= cols * M) return;
+    const int64_t c = idx / M, m = idx % M, k = m / 2;
+    const int width = 4 * vw;
+    const double m0 = 0.5 * s[c];
+    const double m1r = s[2 * vw + c], m1i = s[3 * vw + c];
+    double re, im = 0.0;
+    if (m == 0) {
+        re = m0;
+    } else if (m == 1) {
+        re = m1r;
+        im = m1i;
+    } else if (m % 2 == 0) {
+        re = 2.0 * (s[k * width + c] - m0);
+    } else {
+        re = 2.0 * s[k * width + 2 * vw + c] - m1r;
+        im = 2.0 * s[k * width + 3 * vw + c] - m1i;
+    }
+    R* out = mu + (c0 + c) * M * (cplx ? 2 : 1);
+    if (cplx) {
+        out[2 * m] = (R)re;
+        out[2 * m + 1] = (R)im;
+    } else {
+        out[m] = (R)re;
+    }
+}
+
+void launch_diag_collect(int scalar, const double* sums, int vw, int64_t cols, int64_t M, int64_t c0, void* mu,
+                         hipStream_t s) {
+    const int64_t total = cols * M;
+    if (total <= 0) return;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    const int cplx = scalar >= 2 ? 1 : 0;
+    if (scalar == 0 || scalar == 2)
+        hipLaunchKernelGGL(k_diag_collect<float>, grid, dim3(256), 0, s, sums, vw, cols, M, cplx, static_cast<float*>(mu), c0);
+    else
+        hipLaunchKernelGGL(k_diag_collect<double>, grid, dim3(256), 0, s, sums, vw, cols, M, cplx, static_cast<double*>(mu),
+                           c0);
+}
+
 static dim3 grid_for(int64_t count) {
     int64_t g = (count + 255) / 256;
     if (g < 1) g = 1;
