@@ -359,6 +359,23 @@ __device__ __forceinline__ T shfl_any(T v, int src) {
     }
 }
 
+// Value of `v` in lane `src` for every lane, when `src` is the same for the whole wave: one
+// v_readlane per 32 bits, so the value lands in scalar registers (no LDS crossbar traffic and no
+// vector registers, unlike a per-lane shuffle).
+__device__ __forceinline__ float readlane_any(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double readlane_any(double v, int src) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <class R>
+__device__ __forceinline__ cplx<R> readlane_any(cplx<R> v, int src) {
+    return {readlane_any(v.re, src), readlane_any(v.im, src)};
+}
+
 // The diagonal step of k_step_sellb with wave-private staging: each wave loads the first KMAX
 // entries of its own RW rows itself (entry k of row r in lane k * RW + r, so KMAX * RW <= 64)
 // and hands them to the lanes that gather with them by lane shuffles, so a tile needs no LDS
@@ -373,7 +390,9 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
     constexpr int TPS = WAVE / TR;
     static_assert(VW >= 4 && RW % RPW == 0 && WAVE % TR == 0 && KMAX * RW <= WAVE, "bad wave-staged configuration");
     const int lane = threadIdx.x % WAVE;
-    const int wave = threadIdx.x / WAVE;
+    // the wave index is uniform within the wave: made scalar, so that with one row per wave
+    // iteration (VW = 64) every row index and row address below is scalar too
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const int rsub = lane / VW;
     const int col = lane % VW;
     const int32_t* __restrict__ sidx = a.indices;
@@ -382,6 +401,16 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
     double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
     const TileWalk tw = tile_walk(a.num_tiles);
     const int sk = lane / RW, sr = lane % RW;  // the entry this lane stages: entry sk of row sr
+    // entry k of row u of the tile, for every lane of the row: with one row per wave iteration the
+    // source lane is the same for the whole wave (a scalar read-lane), otherwise a shuffle
+    auto entry_idx = [&](int32_t v, int k, int u) -> int32_t {
+        if constexpr (RPW == 1) return __builtin_amdgcn_readlane(v, k * RW + u);
+        else return __shfl(v, k * RW + u * RPW + rsub, WAVE);
+    };
+    auto entry_val = [&](T v, int k, int u) -> T {
+        if constexpr (RPW == 1) return readlane_any(v, k * RW + u);
+        else return shfl_any(v, k * RW + u * RPW + rsub);
+    };
     int32_t p_idx = 0;
     T p_val = zero(T{});
     auto prefetch = [&](int64_t tile, int64_t& pbase, int& pwidth) {
@@ -420,15 +449,21 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
             rin[u] = wave * RW + u * RPW + rsub;
             row[u] = slice * WAVE + r0 + rin[u];
             ok[u] = row[u] < a.n;
-            const int64_t e = (ok[u] ? row[u] : a.n - 1) * VW + col;
-            yv[u] = a.sub_y ? nt_load(&a.yin[e]) : zero(T{});
-            xo[u] = x[e];
         }
+        if (a.sub_y) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) yv[u] = nt_load(&a.yin[(ok[u] ? row[u] : a.n - 1) * VW + col]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) yv[u] = zero(T{});
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xo[u] = x[(ok[u] ? row[u] : a.n - 1) * VW + col];
 #pragma unroll
         for (int k = 0; k < KMAX; ++k)
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int32_t c = __shfl(c_idx, k * RW + u * RPW + rsub, WAVE);
+                const int32_t c = entry_idx(c_idx, k, u);
                 if constexpr (OFF32) {
                     const uint32_t off = ((uint32_t)c * VW + col) * (uint32_t)sizeof(T);
                     xg[u][k] = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(x) + off);
@@ -442,7 +477,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
         for (int u = 0; u < U; ++u) {
             acc[u] = zero(T{});
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) acc[u] = fmac(shfl_any(c_val, k * RW + u * RPW + rsub), xg[u][k], acc[u]);
+            for (int k = 0; k < KMAX; ++k) acc[u] = fmac(entry_val(c_val, k, u), xg[u][k], acc[u]);
         }
         for (int k = KMAX; k < width; ++k) {
 #pragma unroll
