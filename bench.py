@@ -149,7 +149,9 @@ def model_leg(cells=(2048, 1024), M=4096, R=64):
     dos = kpm.calc_dos(np.linspace(-8.5, 8.5, 2000), broadening, num_random=R)
     total = time.perf_counter() - t0
     st = kpm.stats
-    assert st.num_moments == M and np.all(np.isfinite(dos.data))
+    # the kernel asks for M moments; the count is rounded up so that (M - 2) % 4 == 0
+    # (cppcore/include/kpm/Kernel.hpp:9-13)
+    assert st.num_moments == M + (-(M - 2)) % 4 and np.all(np.isfinite(dos.data))
     out["facade_calc_dos"] = {"total_s": total, "moments_time_s": st.moments_time, "eps": st.eps,
                               "moments": st.num_moments, "vectors": R, "energies": 2000}
     log("kpm(model).calc_dos end to end: %.2f s (moment computation incl. starters and upload %.2f s)"
@@ -363,6 +365,33 @@ def kubo_contraction(n1=724, n2=724, M=512, R=16):
     return out
 
 
+def ldos_light_cone(reps=3):
+    """LDOS at the centre of a 1 M-site graphene flake (161.8 x 161.8 nm, fp64) through
+    kpm(model).calc_ldos at 0.05 eV broadening: with the reference's optimal_size light cone (each
+    step computes only the BFS slices reached so far) and with full sweeps.  Best of `reps`."""
+    import pybinding_amd as pb
+    from pybinding_amd.repository import graphene
+    model = pb.Model(graphene.monolayer(), pb.rectangle(161.8, 161.8), pb.force_double_precision())
+    n = model.hamiltonian.shape[0]
+    energy = np.linspace(-1, 1, 200)
+    out = {"sites": n, "broadening": 0.05}
+    for opt in (True, False):
+        kpm = pb.kpm(model, energy_range=(-9, 9), silent=True, optimal_size=opt, interleaved=False)
+        kpm.calc_ldos(energy, broadening=0.05, position=[0, 0])   # BFS reorder and upload
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            kpm.calc_ldos(energy, broadening=0.05, position=[0, 0])
+            best = min(best, time.perf_counter() - t0)
+        st = kpm.stats
+        out["cone_ms" if opt else "full_ms"] = best * 1000
+        out["cone_fraction_of_rows" if opt else "moments"] = (st.opt_vec / st.vec) if opt else st.num_moments
+    out["speedup"] = out["full_ms"] / out["cone_ms"]
+    log("LDOS light cone: %d sites, %d moments: %.1f ms with the cone (%.2f of the rows), %.1f ms full sweeps"
+        % (n, out["moments"], out["cone_ms"], out["cone_fraction_of_rows"], out["full_ms"]))
+    return out
+
+
 def plain_dos_flake(M=2050, R=8, reps=5):
     """The reference's plain DOS case: graphene 40 x 40 nm flake (the model builder, reference site
     order), fp64, energy range (-8.6, 8.6), 2050 moments (2048 requested, rounded as the
@@ -494,6 +523,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the fp32 variant of the block-SpMM workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient cost run")
     ap.add_argument("--no-kubo", action="store_true", help="skip the Kubo contraction run")
+    ap.add_argument("--no-ldos", action="store_true", help="skip the LDOS light-cone run")
     ap.add_argument("--no-model", action="store_true",
                     help="skip the complex128 run of the workload built by the model builder")
     ap.add_argument("--copy-probe", action="store_true",
@@ -603,6 +633,8 @@ def main():
         extra["fp32_variant"] = fp32_variant(args.cells, M, R, args.format, args.order)
     if not args.no_flake:
         extra["plain_dos_flake"] = plain_dos_flake()
+    if not args.no_ldos:
+        extra["ldos_light_cone"] = ldos_light_cone()
     if not args.no_model and dtype == np.float64:
         extra["model_complex128"] = model_leg(args.cells, M, R)
     if not args.no_batched:

@@ -95,6 +95,15 @@ int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t
                     const int32_t* indices, const void* values, kpmx_mat** out);
 int kpmx_mat_destroy(kpmx_mat* m);
 
+/* Light cone of the next recurrences on this matrix (the reference's `optimal_size` algorithm,
+ * cppcore/include/kpm/calc_moments.hpp:35-51,102-114 with SliceMap::optimal_size,
+ * cppcore/include/kpm/OptimizedHamiltonian.hpp:54-96): step k >= 1 of a recurrence computes only
+ * rows [0, rows[k]) of the BFS-ordered matrix, the rows that can be nonzero (or still reach the
+ * collected rows); rows[0] is ignored (the first step always covers the whole matrix) and steps
+ * past num_steps cover it too.  num_steps = 0 clears the cone.  The caller computes rows[] from
+ * its slice map; for the diagonal recurrence step k is reference iteration k + 1. */
+int kpmx_mat_set_step_rows(kpmx_mat* h2, int64_t num_steps, const int64_t* rows);
+
 /* Upload starter vectors (host, vector-major num_vectors x n) and keep them resident.
  * Replaces make_r0 for vector and block starters (cppcore/include/kpm/Starter.hpp:36-50). */
 int kpmx_block_upload(kpmx_ctx* ctx, int scalar, int64_t n, int64_t num_vectors, const void* r0, kpmx_block** out);
@@ -112,7 +121,9 @@ int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_
  * through workgroup LDS or handed out by lane shuffles within each wave) on this matrix and
  * starter block and keep the fastest for every later recurrence of that scalar type and vector
  * width on the context.  variant (optional): the chosen variant index, tile shape + 3 * grid
- * (0 resident, 1 half, 2 quarter, 3 twice, 4 four times) + 15 * staging (0 LDS, 1 shuffles). */
+ * (0 resident, 1 half, 2 quarter, 3 twice, 4 four times) + 15 * staging (0 LDS, 1 shuffles,
+ * 2 column pairs: an fp64 block of 8 or more vectors read as pairs of adjacent columns, 16 bytes
+ * per lane; for other blocks it runs as 1). */
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant);
 /* Pin the diagonal step variant (as numbered above) for one scalar type and vector width on the
  * context, e.g. a choice an earlier autotuning run reported (no counterpart in the reference). */

@@ -6,7 +6,8 @@ include/kpmx.h) for every Chebyshev moment.
 """
 from .chebyshev import *  # noqa: F401,F403
 from .chebyshev import KPM, kpm, kpm_cuda, kpm_hip, jackson_kernel, lorentz_kernel, dirichlet_kernel  # noqa: F401
-from .results import Series, Deferred  # noqa: F401
+from .results import Series, StructureMap, Deferred  # noqa: F401
+from .io import save, load  # noqa: F401
 from . import generators  # noqa: F401
 from .model import Model  # noqa: F401
 from .lattice import (Lattice, Polygon, rectangle, regular_polygon, primitive, translational_symmetry,  # noqa: F401

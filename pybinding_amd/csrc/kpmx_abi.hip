@@ -165,6 +165,8 @@ struct kpmx_mat {
     void* sell_vals = nullptr;
     int64_t sell_total = 0;
     std::vector<int64_t> sell_pos;  // SELL storage position of every CSR entry
+    // light cone (kpmx_mat_set_step_rows): rows computed by each step of the next recurrences
+    std::vector<int64_t> step_rows;
     mutable int64_t* d_sell_pos = nullptr;  // device copy, uploaded by the first adjoint pass
     ~kpmx_mat() {
         dev_free(d_sell_pos);
@@ -367,11 +369,15 @@ void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, 
     L.w = w;
     L.tune = tune >= 0 ? tune : ctx->tune[h2->scalar][log2i(t.vw)];
     L.num_wg = step_grid(L);  // the SELL block kernel runs a resident grid
-    const int width = 4 * t.vw;
+    const int width = 4 * t.vw;  // raw sums per step
+    // the fp64 column-pair kernel writes half-width partials in its own layout; the reduction
+    // brings them to the standard one
+    const bool pairs = mode == MODE_DIAG && step_uses_pairs(L);
+    const int pwidth = pairs ? 2 * t.vw : width;
     int64_t chunk = 1;
     double* partials = nullptr;
     if (mode != MODE_NONE) {
-        const size_t per_step_bytes = (size_t)L.num_wg * width * sizeof(double);
+        const size_t per_step_bytes = (size_t)L.num_wg * pwidth * sizeof(double);
         if (fixed) {
             KPMX_REQUIRE(fixed->partials_bytes >= per_step_bytes, "partial-sum buffer too small");
             chunk = std::max<int64_t>(1, std::min<int64_t>(steps, (int64_t)(fixed->partials_bytes / per_step_bytes)));
@@ -381,8 +387,16 @@ void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, 
             partials = static_cast<double*>(ctx->partials.get(per_step_bytes * chunk));
         }
     }
+    // light cone: step k computes only rows [0, step_rows[k]); rows past it must read as the zeros
+    // they are in the exact recurrence, so the first work block written after step 0 starts zeroed
+    const std::vector<int64_t>& cone = h2->step_rows;
+    if (!cone.empty()) {
+        launch_zero(h2->scalar, W0, n * t.vw, ctx->stream);
+        KPMX_HIP(hipGetLastError());
+    }
     int64_t chunk_start = 0;
     for (int64_t k = 0; k < steps; ++k) {
+        L.n = (k > 0 && k < (int64_t)cone.size()) ? std::max<int64_t>(1, std::min<int64_t>(n, cone[(size_t)k])) : n;
         if (k == 0) {
             L.x = t.data;
             L.yin = nullptr;
@@ -397,13 +411,13 @@ void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, 
             L.hscale = 1.0;
             L.sub_y = 1;
         }
-        if (partials) L.partials = partials + (size_t)(k - chunk_start) * L.num_wg * width;
+        if (partials) L.partials = partials + (size_t)(k - chunk_start) * L.num_wg * pwidth;
         per_step(L, k);
         launch(L, ctx->stream);
         post_step(L, k);
         if (partials && (k - chunk_start + 1 == chunk || k == steps - 1)) {
-            launch_reduce_partials(partials, L.num_wg, width, k - chunk_start + 1, sums + (size_t)chunk_start * width,
-                                   ctx->stream);
+            launch_reduce_partials(partials, L.num_wg, pwidth, k - chunk_start + 1, sums + (size_t)chunk_start * width,
+                                   ctx->stream, pairs ? t.vw : 0);
             KPMX_HIP(hipGetLastError());
             chunk_start = k + 1;
         }
@@ -459,6 +473,7 @@ void diag_enqueue(const kpmx_mat* h2, const kpmx_block* r0, int64_t steps) {
 // Small systems are launch-bound (a step of a 60k-site flake is a few microseconds of device
 // work), so their whole recurrence is captured once and replayed as one graph.
 bool diag_graph_candidate(const kpmx_mat* h2, const kpmx_block* r0) {
+    if (!h2->step_rows.empty()) return false;  // per-call row limits are not part of the graph key
     const int policy = h2->ctx->graph_policy;
     if (policy == KPMX_GRAPH_NEVER) return false;
     if (policy == KPMX_GRAPH_ALWAYS) return true;
@@ -847,6 +862,16 @@ int kpmx_mat_destroy(kpmx_mat* m) {
     });
 }
 
+int kpmx_mat_set_step_rows(kpmx_mat* h2, int64_t num_steps, const int64_t* rows) {
+    return guarded([&] {
+        KPMX_REQUIRE(h2, "null matrix");
+        KPMX_REQUIRE(num_steps >= 0 && (num_steps == 0 || rows), "null step rows");
+        for (int64_t k = 0; k < num_steps; ++k)
+            KPMX_REQUIRE(rows[k] >= 1 && rows[k] <= h2->rows, "step rows must lie in [1, matrix rows]");
+        h2->step_rows.assign(rows, rows + num_steps);
+    });
+}
+
 int kpmx_block_upload(kpmx_ctx* ctx, int scalar, int64_t n, int64_t num_vectors, const void* r0, kpmx_block** out) {
     return guarded([&] {
         KPMX_REQUIRE(ctx && out, "null context or output");
@@ -1056,6 +1081,10 @@ int kpmx_moments_multi_unit(const kpmx_mat* h2, const kpmx_block* r0, int64_t M,
         int32_t* dmark = static_cast<int32_t*>(ctx->aux1.get((size_t)n * sizeof(int32_t)));
         KPMX_HIP(hipMemcpyAsync(dmark, mark.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
         char* dmu = static_cast<char*>(ctx->aux2.get((size_t)std::max<int64_t>(nu, 1) * M * sb));
+        // a step writes the slots of the rows it computes; under a light cone a destination the
+        // recurrence has not reached keeps the zero it has in the exact recurrence
+        launch_zero(h2->scalar, dmu, std::max<int64_t>(nu, 1) * M, ctx->stream);
+        KPMX_HIP(hipGetLastError());
         const Tile& first = r0->tiles.front();
         Tile t;
         t.vw = 1;
@@ -1441,6 +1470,9 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
         int best = 0;
         float best_ms = 0;
         for (int v = 0; v < NUM_TUNE_VARIANTS; ++v) {
+            // the column-pair variants exist for fp64 blocks of 8 or more vectors only (elsewhere
+            // they run the wave-staged kernel, already timed)
+            if (v >= 2 * NUM_TILE_SHAPES * NUM_GRID_VARIANTS && !(h2->scalar == KPMX_F64 && t.vw >= 8)) continue;
             float ms_v = 0;
             for (int rep = 0; rep < 4; ++rep) {  // the first repetition warms caches and code
                 KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));

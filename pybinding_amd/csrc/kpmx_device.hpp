@@ -15,11 +15,20 @@ struct cplx {
 using cf32 = cplx<float>;
 using cf64 = cplx<double>;
 
+// Two adjacent real fp64 vector columns handled by one lane (16-byte accesses; two rows of a
+// 64-column block per wave instruction).  Arithmetic is per column; the matrix stays real double.
+struct dpair {
+    double a, b;
+};
+
+// real: the type of the scale factor and sums; value: the matrix entry type (a dpair block is
+// multiplied by a real double matrix)
 template <class T> struct traits;
-template <> struct traits<float>  { using real = float;  static constexpr bool is_complex = false; static constexpr int code = 0; };
-template <> struct traits<double> { using real = double; static constexpr bool is_complex = false; static constexpr int code = 1; };
-template <> struct traits<cf32>   { using real = float;  static constexpr bool is_complex = true;  static constexpr int code = 2; };
-template <> struct traits<cf64>   { using real = double; static constexpr bool is_complex = true;  static constexpr int code = 3; };
+template <> struct traits<float>  { using real = float;  using value = float;  static constexpr bool is_complex = false; static constexpr int code = 0; };
+template <> struct traits<double> { using real = double; using value = double; static constexpr bool is_complex = false; static constexpr int code = 1; };
+template <> struct traits<cf32>   { using real = float;  using value = cf32;   static constexpr bool is_complex = true;  static constexpr int code = 2; };
+template <> struct traits<cf64>   { using real = double; using value = cf64;   static constexpr bool is_complex = true;  static constexpr int code = 3; };
+template <> struct traits<dpair>  { using real = double; using value = double; static constexpr bool is_complex = false; static constexpr int code = 1; };
 
 // ----------------------------------------------------------------------------- arithmetic
 __host__ __device__ __forceinline__ float  zero(float)  { return 0.f; }
@@ -42,6 +51,11 @@ __device__ __forceinline__ float  sub(float a, float b)   { return a - b; }
 __device__ __forceinline__ double sub(double a, double b) { return a - b; }
 template <class R> __device__ __forceinline__ cplx<R> sub(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
 
+__host__ __device__ __forceinline__ dpair zero(dpair) { return {0.0, 0.0}; }
+__device__ __forceinline__ dpair fmac(double a, dpair b, dpair c) { return {a * b.a + c.a, a * b.b + c.b}; }
+__device__ __forceinline__ dpair scal(dpair a, double s) { return {a.a * s, a.b * s}; }
+__device__ __forceinline__ dpair sub(dpair a, dpair b) { return {a.a - b.a, a.b - b.b}; }
+
 // |a|^2 in double
 __device__ __forceinline__ double sqabs(float a)  { return double(a) * double(a); }
 __device__ __forceinline__ double sqabs(double a) { return a * a; }
@@ -58,6 +72,24 @@ template <class R> __device__ __forceinline__ d2 cdot(cplx<R> a, cplx<R> b) {
             double(a.re) * double(b.im) - double(a.im) * double(b.re)};
 }
 
+// The two diagonal-moment sums of one element pair (x, y_new) added to the running sums
+// (A = sum |x|^2, B = sum conj(y_new) x): a real or complex column fills (a_re, b_re, b_im); a
+// column pair puts column 2p's sums in (a_re, b_re) and column 2p+1's in (a_im, b_im).
+template <class T>
+__device__ __forceinline__ void add_diag_sums(T ynew, T x, double& a_re, double& a_im, double& b_re, double& b_im) {
+    a_re += sqabs(x);
+    const d2 p = cdot(ynew, x);
+    b_re += p.re;
+    b_im += p.im;
+}
+__device__ __forceinline__ void add_diag_sums(dpair ynew, dpair x, double& a_re, double& a_im, double& b_re,
+                                              double& b_im) {
+    a_re += x.a * x.a;
+    a_im += x.b * x.b;
+    b_re += ynew.a * x.a;
+    b_im += ynew.b * x.b;
+}
+
 // Non-temporal (streaming) loads and stores: data touched once per step, kept from evicting
 // reused data in L2.
 __device__ __forceinline__ float nt_load(const float* p) { return __builtin_nontemporal_load(p); }
@@ -65,6 +97,15 @@ __device__ __forceinline__ double nt_load(const double* p) { return __builtin_no
 template <class R> __device__ __forceinline__ cplx<R> nt_load(const cplx<R>* p) {
     const R* q = reinterpret_cast<const R*>(p);
     return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
+__device__ __forceinline__ dpair nt_load(const dpair* p) {
+    const double* q = reinterpret_cast<const double*>(p);
+    return {__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
+__device__ __forceinline__ void nt_store(dpair v, dpair* p) {
+    double* q = reinterpret_cast<double*>(p);
+    __builtin_nontemporal_store(v.a, q);
+    __builtin_nontemporal_store(v.b, q + 1);
 }
 __device__ __forceinline__ void nt_store(float v, float* p) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void nt_store(double v, double* p) { __builtin_nontemporal_store(v, p); }

@@ -413,6 +413,133 @@ X3Plan x3_plan(int64_t M, int64_t kdim, int num_cu) {
     return p;
 }
 
+// ------------------------------------------------------------------- FP64 on the matrix cores
+// The FP64 contraction with v_mfma_f64_16x16x4_f64: the work decomposition of the bf16x3 kernel
+// (persistent, XCD-aware: 128x64 output tiles of one K range on one XCD, partial tiles per K range
+// summed in order), every product and sum in double.  One 16x16x4 MFMA takes lane l's A value at
+// (row l % 16, k l / 16) and B value at (k l / 16, column l % 16) and accumulates a 16x16 block
+// whose lane l holds rows 4 j + l / 16, j < 4, of column l % 16 (result j in the j-th register
+// pair: the f64 form interleaves the lane groups row by row).  A wave owns a 32x32 block (2x2
+// MFMA blocks, real and imaginary accumulators); complex a conj(b) = (ar br + ai bi) + i (ai br - ar bi)
+// takes four real MFMAs.  K panels of D64K = 16 are staged in LDS as [k][row] pairs; the next
+// panel is loaded into registers while the current one is multiplied.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int D64K = 16;
+
+template <class T>
+__global__ __launch_bounds__(XTHREADS) void k_gemm_nc_mfma64(const T* __restrict__ A, int64_t lda,
+                                                             const T* __restrict__ B, int64_t ldb, int64_t M,
+                                                             int64_t kdim, int64_t kslice, int groups_per_xcd,
+                                                             int64_t tiles_per_wg, double* __restrict__ P) {
+    constexpr bool CPLX = traits<T>::is_complex;
+    constexpr int NP = CPLX ? 2 : 1;
+    __shared__ d2 As[D64K][XTN + 1];
+    __shared__ d2 Bs[D64K][XTM + 1];
+    constexpr int EA = XTN * D64K / XTHREADS;   // elements staged per thread (A: 4, B: 2)
+    constexpr int EB = XTM * D64K / XTHREADS;
+    const int t = threadIdx.x;
+    const int lane = t % 64, wave = t / 64;
+    const int br = wave / 2, bc = wave % 2;
+    const int64_t tiles_n = (M + XTN - 1) / XTN, tiles_m = (M + XTM - 1) / XTM;
+    const int64_t T2 = tiles_n * tiles_m;
+    const int xcd = blockIdx.x % 8;
+    const int slot = blockIdx.x / 8;
+    const int slots = gridDim.x / 8;
+    int64_t kg, tile0, tile_step;
+    if (T2 <= slots) {
+        if (slot >= groups_per_xcd * T2) return;   // idle workgroup (the whole block exits before any barrier)
+        kg = (int64_t)xcd * groups_per_xcd + slot / T2;
+        tile0 = slot % T2;
+        tile_step = T2;
+    } else {
+        kg = xcd;
+        tile0 = slot;
+        tile_step = slots;
+    }
+    const int64_t kbeg = kg * kslice;
+    const int64_t kend = (kbeg + kslice < kdim) ? kbeg + kslice : kdim;
+    d2 ra[EA], rb[EB];
+    // element e of a stage: row e / D64K, k e % D64K (16 consecutive k of one row per 16 threads)
+    auto load = [&](const T* __restrict__ src, int64_t ld, int64_t r0, int64_t k0, int e) -> d2 {
+        const int r = e / D64K, kk = e % D64K;
+        const int64_t k = k0 + kk;
+        if (r0 + r >= M || k >= kend) return d2{0.0, 0.0};
+        return to_d2(src[(r0 + r) * ld + k]);
+    };
+    for (int64_t tile = tile0, it = 0; tile < T2 && it < tiles_per_wg; tile += tile_step, ++it) {
+        const int64_t n0 = (tile / tiles_m) * XTN, m0 = (tile % tiles_m) * XTM;
+        f64x4 cr[2][2], ci[2][2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) cr[i][j] = ci[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        if (kbeg < kend) {
+#pragma unroll
+            for (int q = 0; q < EA; ++q) ra[q] = load(A, lda, n0, kbeg, t + q * XTHREADS);
+#pragma unroll
+            for (int q = 0; q < EB; ++q) rb[q] = load(B, ldb, m0, kbeg, t + q * XTHREADS);
+        }
+        for (int64_t k0 = kbeg; k0 < kend; k0 += D64K) {
+            __syncthreads();   // the previous stage's LDS operands are no longer read
+#pragma unroll
+            for (int q = 0; q < EA; ++q) {
+                const int e = t + q * XTHREADS;
+                As[e % D64K][e / D64K] = ra[q];
+            }
+#pragma unroll
+            for (int q = 0; q < EB; ++q) {
+                const int e = t + q * XTHREADS;
+                Bs[e % D64K][e / D64K] = rb[q];
+            }
+            __syncthreads();
+            if (k0 + D64K < kend) {   // the next panel is in flight during the multiplications
+#pragma unroll
+                for (int q = 0; q < EA; ++q) ra[q] = load(A, lda, n0, k0 + D64K, t + q * XTHREADS);
+#pragma unroll
+                for (int q = 0; q < EB; ++q) rb[q] = load(B, ldb, m0, k0 + D64K, t + q * XTHREADS);
+            }
+#pragma unroll
+            for (int ks = 0; ks < D64K; ks += 4) {
+                const int k = ks + lane / 16;
+                d2 a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) a[i] = As[k][br * 32 + i * 16 + lane % 16];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) b[j] = Bs[k][bc * 32 + j * 16 + lane % 16];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].re, b[j].re, cr[i][j], 0, 0, 0);
+                        if constexpr (CPLX) {
+                            cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].im, b[j].im, cr[i][j], 0, 0, 0);
+                            ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].im, b[j].re, ci[i][j], 0, 0, 0);
+                            ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[i].re, b[j].im, ci[i][j], 0, 0, 0);
+                        }
+                    }
+            }
+        }
+        double* out = P + kg * M * M * NP;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int64_t n = n0 + br * 32 + i * 16 + 4 * q + (lane / 16);
+                    const int64_t m = m0 + bc * 32 + j * 16 + (lane % 16);
+                    if (n < M && m < M) {
+                        if constexpr (CPLX) {
+                            out[(n * M + m) * 2] = cr[i][j][q];
+                            out[(n * M + m) * 2 + 1] = ci[i][j][q];
+                        } else {
+                            out[n * M + m] = cr[i][j][q];
+                        }
+                    }
+                }
+    }
+}
+
 __global__ void k_gemm_reduce(const double* P, int64_t slices, int64_t count, double* C) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
         double s = C[e];
@@ -434,7 +561,7 @@ int64_t num_slices(int64_t M, int64_t kdim) {
 }  // namespace
 
 int64_t gemm_workspace_doubles(int64_t M, int64_t kdim, int scalar, int mode, int num_cu) {
-    const int64_t S = mode == 2 ? x3_plan(M, kdim, num_cu).kg_total : num_slices(M, kdim);
+    const int64_t S = (mode == 2 || mode == 0) ? x3_plan(M, kdim, num_cu).kg_total : num_slices(M, kdim);
     return S * M * M * (scalar >= 2 ? 2 : 1);
 }
 
@@ -442,6 +569,15 @@ template <class T>
 static void gemm_t(int mode, const void* A, int64_t lda, const void* B, int64_t ldb, int64_t M, int64_t kdim, double* C,
                    double* work, int num_cu, hipStream_t s) {
     const int64_t count = M * M * (traits<T>::is_complex ? 2 : 1);
+    if (mode == 0) {   // FP64 on the matrix cores, XCD-aware persistent grid
+        const X3Plan p = x3_plan(M, kdim, num_cu);
+        hipLaunchKernelGGL(k_gemm_nc_mfma64<T>, dim3((unsigned)p.grid), dim3(XTHREADS), 0, s, static_cast<const T*>(A),
+                           lda, static_cast<const T*>(B), ldb, M, kdim, p.kslice, p.groups_per_xcd, p.tiles_per_wg, work);
+        int64_t g = (count + 255) / 256;
+        if (g > 65536) g = 65536;
+        hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)g), dim3(256), 0, s, work, p.kg_total, count, C);
+        return;
+    }
     const int64_t S = num_slices(M, kdim);
     {
         int64_t kslice = (kdim + S - 1) / S;

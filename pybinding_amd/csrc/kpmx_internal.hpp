@@ -83,12 +83,19 @@ int launch_adjoint_step(const AdjointLaunch& A, hipStream_t stream);
 
 int launch_step(const StepLaunch& L, hipStream_t stream);
 int step_grid(const StepLaunch& L);
+// whether this launch runs the fp64 column-pair kernel, whose partial sums need the pair remap of
+// launch_reduce_partials (pair_cols)
+bool step_uses_pairs(const StepLaunch& L);
 constexpr int NUM_TILE_SHAPES = 3;     // rows per wave per tile (k_step_sellb)
 constexpr int NUM_GRID_VARIANTS = 5;   // resident grid, 1/2, 1/4, 2x, 4x
-constexpr int NUM_TUNE_VARIANTS = 30;  // tile shape x grid x staging (workgroup LDS, wave shuffles)
+// tile shape x grid x staging (workgroup LDS, wave shuffles, fp64 column pairs)
+constexpr int NUM_TUNE_VARIANTS = 45;
 int64_t step_rows_per_tile(int format, int vw);
+// sums[s][j] = sum over workgroups of partials[s][w][j] (width doubles per workgroup); with
+// pair_cols > 0 the partials are column-pair sums of pair_cols columns and the output has the
+// standard [4][pair_cols] layout
 void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
-                            hipStream_t stream);
+                            hipStream_t stream, int pair_cols = 0);
 // diagonal moments of `cols` columns from the raw per-step sums (steps x 4 vw doubles) into mu
 // (device; column c at mu + (c0 + c) * M scalars of the scalar type's real kind, complex as pairs)
 void launch_diag_collect(int scalar, const double* sums, int vw, int64_t cols, int64_t M, int64_t c0, void* mu,

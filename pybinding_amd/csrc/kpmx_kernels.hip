@@ -34,9 +34,10 @@ constexpr int BLOCK = WAVE * WAVES_PER_BLOCK;
 template <class T>
 struct StepArgs {
     using real = typename traits<T>::real;
+    using value = typename traits<T>::value;
     const int32_t* indptr;
     const int32_t* indices;
-    const T* vals;
+    const value* vals;
     const T* x;
     const T* yin;        // may be null when !sub_y
     T* yout;             // may alias yin
@@ -395,24 +396,40 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const int rsub = lane / VW;
     const int col = lane % VW;
+    using V = typename traits<T>::value;
     const int32_t* __restrict__ sidx = a.indices;
-    const T* __restrict__ sval = a.vals;
+    const V* __restrict__ sval = a.vals;
     const T* __restrict__ x = a.x;
     double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
     const TileWalk tw = tile_walk(a.num_tiles);
     const int sk = lane / RW, sr = lane % RW;  // the entry this lane stages: entry sk of row sr
     // entry k of row u of the tile, for every lane of the row: with one row per wave iteration the
-    // source lane is the same for the whole wave (a scalar read-lane), otherwise a shuffle
+    // source lane is the same for the whole wave (a scalar read-lane); with two, each half-wave
+    // picks one of two scalar read-lanes; otherwise a shuffle
     auto entry_idx = [&](int32_t v, int k, int u) -> int32_t {
-        if constexpr (RPW == 1) return __builtin_amdgcn_readlane(v, k * RW + u);
-        else return __shfl(v, k * RW + u * RPW + rsub, WAVE);
+        if constexpr (RPW == 1) {
+            return __builtin_amdgcn_readlane(v, k * RW + u);
+        } else if constexpr (RPW == 2) {
+            const int32_t lo = __builtin_amdgcn_readlane(v, k * RW + u * 2);
+            const int32_t hi = __builtin_amdgcn_readlane(v, k * RW + u * 2 + 1);
+            return rsub ? hi : lo;
+        } else {
+            return __shfl(v, k * RW + u * RPW + rsub, WAVE);
+        }
     };
-    auto entry_val = [&](T v, int k, int u) -> T {
-        if constexpr (RPW == 1) return readlane_any(v, k * RW + u);
-        else return shfl_any(v, k * RW + u * RPW + rsub);
+    auto entry_val = [&](V v, int k, int u) -> V {
+        if constexpr (RPW == 1) {
+            return readlane_any(v, k * RW + u);
+        } else if constexpr (RPW == 2) {
+            const V lo = readlane_any(v, k * RW + u * 2);
+            const V hi = readlane_any(v, k * RW + u * 2 + 1);
+            return rsub ? hi : lo;
+        } else {
+            return shfl_any(v, k * RW + u * RPW + rsub);
+        }
     };
     int32_t p_idx = 0;
-    T p_val = zero(T{});
+    V p_val = zero(V{});
     auto prefetch = [&](int64_t tile, int64_t& pbase, int& pwidth) {
         const int64_t slc = tile / TPS;
         const int rr0 = (int)(tile % TPS) * TR;
@@ -426,7 +443,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
         } else {
             const int64_t own = slc * WAVE + rr0 + rin;
             p_idx = (int32_t)(own < a.n ? own : a.n - 1);
-            p_val = zero(T{});
+            p_val = zero(V{});
         }
     };
     int64_t next_base = 0;
@@ -438,7 +455,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
         const int64_t base = next_base;
         const int width = next_width;
         const int32_t c_idx = p_idx;
-        const T c_val = p_val;
+        const V c_val = p_val;
         int rin[U];
         int64_t row[U];
         bool ok[U];
@@ -497,10 +514,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
                 const int32_t slot = a.mark[row[u]];
                 if (slot >= 0) a.mark_out[(int64_t)slot * a.mark_stride + col] = ynew;
             }
-            a_re += sqabs(xo[u]);
-            const d2 p = cdot(ynew, xo[u]);
-            b_re += p.re;
-            b_im += p.im;
+            add_diag_sums(ynew, xo[u], a_re, a_im, b_re, b_im);
         }
     }
     block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials);
@@ -508,11 +522,20 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
 
 // Rows per wave per tile of the SELL block kernel: the tunable choices for a vector width
 // (tune index 0 is the default).  Each must be a multiple of 64/VW and at most 16.
+constexpr int rw_of(int vw, int tune) {
+    const int rpw = WAVE / vw;
+    const int base = rpw > 4 ? rpw : 4;
+    return tune == 1 ? (base > rpw ? base / 2 : base) : tune == 2 ? (base * 2 <= 16 ? base * 2 : base) : base;
+}
 template <int VW>
 constexpr int sellb_rw(int tune) {
-    constexpr int RPW = WAVE / VW;
-    constexpr int base = RPW > 4 ? RPW : 4;
-    return tune == 1 ? (base > RPW ? base / 2 : base) : tune == 2 ? (base * 2 <= 16 ? base * 2 : base) : base;
+    return rw_of(VW, tune);
+}
+// The column-pair form of a vw-wide fp64 block (vw / 2 pairs per row) fits the wave-staged kernel
+// for every tile shape when each wave's staged entries (kmax per row) fit its 64 lanes.
+constexpr bool pairs_fit(int vw, int kmax) {
+    return vw >= 8 && vw <= 64 && kmax * rw_of(vw / 2, 0) <= WAVE && kmax * rw_of(vw / 2, 1) <= WAVE &&
+           kmax * rw_of(vw / 2, 2) <= WAVE;
 }
 
 // Batched independent systems: a block-diagonal SELL-64 matrix whose diagonal blocks (one per
@@ -880,23 +903,33 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
 // sums[s][j] = sum over workgroups g (in order) of partials[s][g][j], for s in [0, nsteps).
 // Block (s, chunk) covers columns [64 chunk, 64 chunk + 64) of step s: 4 groups of 64 threads
 // sum interleaved workgroup subsets, then the 4 subtotals are combined in order through LDS.
+// Sums of one step's per-workgroup partials (width doubles each) in workgroup order.  With
+// pair_cols > 0 the partials are column-pair sums ([4][pair_cols / 2]: A of the even columns, A of
+// the odd ones, B of the even, B of the odd) and come out in the standard [4][pair_cols] layout
+// (A, imaginary A = 0, B, imaginary B = 0).
 __global__ __launch_bounds__(256) void k_reduce_partials(const double* partials, int64_t num_wg, int width,
-                                                         double* sums) {
+                                                         double* sums, int pair_cols) {
     __shared__ double red[4][64];
-    const int chunks = (width + 63) / 64;
+    const int out_width = pair_cols > 0 ? 4 * pair_cols : width;
+    const int chunks = (out_width + 63) / 64;
     const int s = blockIdx.x / chunks;
     const int chunk = blockIdx.x % chunks;
     const double* p = partials + (int64_t)s * num_wg * width;
     const int t = threadIdx.x;
     const int cl = t % 64, g = t / 64;
     const int j = chunk * 64 + cl;
+    int src = j;
+    if (pair_cols > 0 && j < out_width) {
+        const int q = j / pair_cols, c = j % pair_cols;
+        src = (q == 1 || q == 3) ? -1 : (q + (c % 2)) * (pair_cols / 2) + c / 2;
+    }
     double acc = 0;
-    if (j < width) {
-        for (int64_t w = g; w < num_wg; w += 4) acc += p[w * width + j];
+    if (j < out_width && src >= 0) {
+        for (int64_t w = g; w < num_wg; w += 4) acc += p[w * width + src];
     }
     red[g][cl] = acc;
     __syncthreads();
-    if (g == 0 && j < width) sums[(int64_t)s * width + j] = ((red[0][cl] + red[1][cl]) + red[2][cl]) + red[3][cl];
+    if (g == 0 && j < out_width) sums[(int64_t)s * out_width + j] = ((red[0][cl] + red[1][cl]) + red[2][cl]) + red[3][cl];
 }
 
 // dst (n x vw, row-major) <- columns [c0, c0 + cols) of src (nvec x n, vector-major), zero padded.
@@ -1066,11 +1099,66 @@ static int launch_sellw_off(StepArgs<T> a, const StepLaunch& L, hipStream_t stre
     return 0;
 }
 
+// The fp64 diagonal step on column pairs: the n x VW block read as n x VW/2 pairs of doubles, so
+// every lane moves 16 bytes and a wave instruction covers 2 (VW = 64) or more rows; the matrix
+// entries stay double.  Partial sums come out in the pair layout (see k_reduce_partials).
+template <int VW, int KMAX, int RW, bool OFF32>
+static int launch_pairs_off(const StepArgs<double>& d, const StepLaunch& L, hipStream_t stream) {
+    constexpr int VP = VW / 2;
+    StepArgs<dpair> a{};
+    a.indptr = d.indptr;
+    a.indices = d.indices;
+    a.vals = d.vals;
+    a.x = reinterpret_cast<const dpair*>(d.x);
+    a.yin = reinterpret_cast<const dpair*>(d.yin);
+    a.yout = reinterpret_cast<dpair*>(d.yout);
+    a.partials = d.partials;
+    a.n = d.n;
+    a.hscale = d.hscale;
+    a.sub_y = d.sub_y;
+    auto kernel = k_step_sellw<dpair, VP, KMAX, RW, OFF32>;
+    constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
+    a.num_tiles = (L.n + tr - 1) / tr;
+    const int gsel = (L.tune / NUM_TILE_SHAPES) % NUM_GRID_VARIANTS;
+    auto grid_of = [&] {
+        int64_t g = resident_grid(kernel, L.num_cu, a.num_tiles);
+        if (gsel <= 2) g >>= gsel;
+        else g = std::min<int64_t>(a.num_tiles, g << (gsel - 2));
+        if (g >= 8) g -= g % 8;
+        return (int)(g < 1 ? 1 : g);
+    };
+    if (L.query_grid) return grid_of();
+    const int grid = L.num_wg > 0 ? L.num_wg : grid_of();
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, L.slice_ptr, L.slice_width);
+    return 0;
+}
+
+template <int VW, int KMAX, int RW>
+static int launch_pairs_rw(const StepArgs<double>& d, const StepLaunch& L, hipStream_t stream) {
+    const bool off32 = (uint64_t)L.n * VW * sizeof(double) < ((uint64_t)1 << 32);
+    return off32 ? launch_pairs_off<VW, KMAX, RW, true>(d, L, stream) : launch_pairs_off<VW, KMAX, RW, false>(d, L, stream);
+}
+
+// rows per wave per tile of the pair kernel for a tile shape (those of k_step_sellb at VW / 2)
+template <int VW>
+constexpr int pairs_rw(int tune) {
+    return sellb_rw<VW / 2>(tune);
+}
+
+template <int VW, int KMAX>
+static int launch_pairs_k(const StepArgs<double>& d, const StepLaunch& L, hipStream_t stream) {
+    switch (L.tune % NUM_TILE_SHAPES) {
+        case 1: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(1)>(d, L, stream);
+        case 2: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(2)>(d, L, stream);
+        default: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(0)>(d, L, stream);
+    }
+}
+
 template <class T, int VW, int MODE, int KMAX, int RW, bool OFF32>
 static int launch_sellb_off(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
     // tuned diagonal variants 15..29: wave-private staging where a wave's entries fit its lanes
     if constexpr (MODE == MODE_DIAG && KMAX * RW <= WAVE) {
-        if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) == 1) return launch_sellw_off<T, VW, KMAX, RW, OFF32>(a, L, stream);
+        if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) >= 1) return launch_sellw_off<T, VW, KMAX, RW, OFF32>(a, L, stream);
     }
     auto kernel = k_step_sellb<T, VW, MODE, KMAX, RW, OFF32>;
     constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
@@ -1100,6 +1188,10 @@ static int launch_sellb_rw(StepArgs<T> a, const StepLaunch& L, hipStream_t strea
 
 template <class T, int VW, int MODE, int KMAX>
 static int launch_sellb_k(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
+    // tuned diagonal variants 30..44: fp64 blocks of 8 or more vectors as column pairs
+    if constexpr (MODE == MODE_DIAG && std::is_same<T, double>::value && pairs_fit(VW, KMAX)) {
+        if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) == 2) return launch_pairs_k<VW, KMAX>(a, L, stream);
+    }
     // the diagonal (hot) path carries the tuned variants, the other modes use the default shape
     const int tune = (MODE == MODE_DIAG) ? L.tune % NUM_TILE_SHAPES : 0;
     switch (tune) {
@@ -1193,6 +1285,15 @@ static int launch_step_t(const StepLaunch& L, hipStream_t stream) {
 }
 
 // Resident grid of the SELL block kernel for this configuration (L.num_wg otherwise).
+bool step_uses_pairs(const StepLaunch& L) {
+    if (!(L.scalar == 1 && L.format == FORMAT_SELL && L.mode == MODE_DIAG && L.vw >= 8)) return false;
+    if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) != 2) return false;
+    // the staged-entry count the launcher picks (launch_sellb): 4 for rows of at most 4 entries, 5
+    // for rows of exactly 5, 8 otherwise
+    const int kmax = L.max_width <= 4 ? 4 : (L.max_width == 5 ? 5 : 8);
+    return pairs_fit(L.vw, kmax);
+}
+
 int step_grid(const StepLaunch& L) {
     if (!(L.format == FORMAT_SELL && L.vw >= 4)) return L.num_wg;
     StepLaunch q = L;
@@ -1217,11 +1318,12 @@ int64_t step_rows_per_tile(int format, int vw) {
 }
 
 void launch_reduce_partials(const double* partials, int64_t num_wg, int width, int64_t nsteps, double* sums,
-                            hipStream_t stream) {
+                            hipStream_t stream, int pair_cols) {
     if (nsteps <= 0) return;
-    const int chunks = (width + 63) / 64;
+    const int out_width = pair_cols > 0 ? 4 * pair_cols : width;
+    const int chunks = (out_width + 63) / 64;
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)(nsteps * chunks)), dim3(256), 0, stream, partials, num_wg, width,
-                       sums);
+                       sums, pair_cols);
 }
 
 // The diagonal collector (cppcore/src/kpm/default/collectors.cpp:5-34) on the device: raw

@@ -102,3 +102,24 @@ def test_ldos_stats_light_cone(flake, oracle):
     assert s.opt_nnz == sum(int(prefix[o]) for o in opt) // 2
     assert s.opt_vec == sum(opt) // 2
     assert s.nnz == M * h.nnz // 2
+
+
+@pytest.mark.parametrize("fmt", ["CSR", "ELL"])
+def test_light_cone_matches_full_sweeps(fmt):
+    """optimal_size: each step computes only the rows of the BFS slices the recurrence has reached
+    (LDOS, diagonal Green's function) or that can still reach the collected rows (off-diagonal
+    Green's functions); the results equal the full sweeps'."""
+    model = pb.Model(graphene.monolayer(), pb.rectangle(14.0), pb.force_double_precision())
+    energy = np.linspace(-2, 2, 41)
+    res = {}
+    for opt in (True, False):
+        kpm = pb.kpm(model, energy_range=(-9, 9), silent=True, matrix_format=fmt, optimal_size=opt,
+                     interleaved=False)
+        ldos = kpm.calc_ldos(energy, broadening=0.1, position=[0.3, -0.2]).data
+        spatial = kpm.calc_spatial_ldos(energy, broadening=0.1, shape=pb.rectangle(0.8)).data
+        i = model.system.find_nearest([0, 0])
+        greens = kpm.calc_greens(i, [i, i + 1, i + 7], energy, broadening=0.1)
+        res[opt] = (ldos, spatial, greens)
+    for a, b in zip(res[True], res[False]):
+        a, b = np.asarray(a), np.asarray(b)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()

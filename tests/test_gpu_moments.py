@@ -89,19 +89,20 @@ def test_diagonal_block_vs_oracle(ctx, oracle, dtype, nvec, fmt):
     assert normwise(mu, ref) <= TOL[dtype]
 
 
-@pytest.mark.parametrize("dtype,nvec", [(np.float64, 64), (np.complex64, 8), (np.float32, 16),
-                                         (np.complex128, 4)])
+@pytest.mark.parametrize("dtype,nvec", [(np.float64, 64), (np.float64, 32), (np.float64, 16), (np.float64, 8),
+                                         (np.complex64, 8), (np.float32, 16), (np.complex128, 4)])
 def test_every_step_variant(ctx, oracle, dtype, nvec):
-    """All 30 variants of the diagonal step kernel (tile shape x grid x LDS or wave-shuffle
-    staging), pinned one by one, against the oracle: a graphene supercell (4 entries per row) and
-    a random matrix (rows of 1 to 27 entries: the 8-entry staging plus the wide-row loop)."""
+    """All 45 variants of the diagonal step kernel (tile shape x grid x LDS, wave-shuffle or
+    fp64 column-pair staging), pinned one by one, against the oracle: a graphene supercell (4
+    entries per row) and a random matrix (rows of 1 to 27 entries: the 8-entry staging plus the
+    wide-row loop)."""
     dtype = np.dtype(dtype)
     for seed, h2 in ((30, graphene_h2(40, 30, dtype)), (31, scaled_random(3000, dtype, seed=31))):
         r0 = oracle.random_starter(h2.shape[0], nvec, dtype)
         ref = oracle.diag_moments(h2, r0, 66)
         mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
         try:
-            for variant in range(30):
+            for variant in range(45):
                 ctx.set_step_variant(dtype, nvec, variant)
                 mu = abi.moments_diagonal(mat, blk, 66)
                 assert normwise(mu, ref) <= TOL[dtype], (variant, seed)
