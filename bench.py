@@ -246,12 +246,20 @@ def adjoint_cost(steps=2):
          np.sin(np.pi * np.arange(M) / (M + 1)) / np.tan(np.pi / (M + 1))) / (M + 1)
     w = g * (2 / np.pi) * np.pi / 4096 * (np.cos(np.arange(M)[:, None] * np.arccos(xq)[None, :]) @ win)
     log("adjoint: %d sites, %d bonds (%.1fs setup)" % (n, len(bonds), time.time() - t0))
-    abi.moments_diagonal_run(mat, tb.block, M)
-    tb.ctx.synchronize()
-    tb.ctx.timer_start()
-    for _ in range(steps):
+
+    def forward_ms():
         abi.moments_diagonal_run(mat, tb.block, M)
-    fwd = tb.ctx.timer_stop() / steps
+        tb.ctx.synchronize()
+        tb.ctx.timer_start()
+        for _ in range(steps):
+            abi.moments_diagonal_run(mat, tb.block, M)
+        return tb.ctx.timer_stop() / steps
+
+    fwd_default = forward_ms()
+    # the forward step kernel tuned for this matrix and block; the gradient's own forward pass
+    # (with its checkpoint copies) runs the same tuned kernel
+    variant = abi.autotune_diagonal(mat, tb.block)
+    fwd = forward_ms()
     grad, _ = abi.adjoint_diagonal(mat, tb.block, M, w, checkpoint=64)
     tb.ctx.timer_start()
     for _ in range(steps):
@@ -274,9 +282,11 @@ def adjoint_cost(steps=2):
     diff = float(np.abs(grad - grad_all).max() / np.abs(grad_all).max())
     mat.close()
     tb.close()
-    log("adjoint: forward %.1f ms, gradient %.1f ms (checkpoint 64) = %.2fx, %.1f ms storing all = %.2fx, "
-        "max |g_ck - g_all| / max |g_all| = %.1e" % (fwd, adj, adj / fwd, adj_all, adj_all / fwd, diff))
-    return {"forward_ms": fwd, "gradient_ms_checkpoint64": adj, "ratio_checkpoint64": adj / fwd,
+    log("adjoint: forward %.1f ms (tuned step variant %d; %.1f ms with the default one), gradient %.1f ms "
+        "(checkpoint 64) = %.2fx, %.1f ms storing all = %.2fx, max |g_ck - g_all| / max |g_all| = %.1e"
+        % (fwd, variant, fwd_default, adj, adj / fwd, adj_all, adj_all / fwd, diff))
+    return {"forward_ms": fwd, "forward_step_variant": variant, "forward_ms_default_variant": fwd_default,
+            "gradient_ms_checkpoint64": adj, "ratio_checkpoint64": adj / fwd,
             "gradient_ms_store_all": adj_all, "ratio_store_all": adj_all / fwd,
             "checkpoint64_vs_store_all_max_rel_diff": diff,
             "backward_only_ms_checkpoint64": bwd, "backward_only_ratio_checkpoint64": bwd / fwd,

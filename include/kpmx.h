@@ -129,6 +129,11 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
  * context, e.g. a choice an earlier autotuning run reported (no counterpart in the reference). */
 int kpmx_ctx_set_step_variant(kpmx_ctx* ctx, int scalar, int vector_width, int variant);
 
+/* Pin the variant of the adjoint step kernel (numbered as the diagonal step variants; no
+ * counterpart in the reference) on the context; -1 restores the default (wave-staged entries on
+ * twice the resident grid, fp64 blocks of 8 or more vectors as column pairs). */
+int kpmx_ctx_set_adjoint_variant(kpmx_ctx* ctx, int variant);
+
 /* Graph policy (no counterpart in the reference, whose CPU loop has no launch cost): with
  * KPMX_GRAPH_ALWAYS the diagonal recurrence of a starter block is captured into a HIP graph on its
  * first run (KPMX_GRAPH_AUTO: on its second run) and replayed by later runs with the same matrix,

@@ -112,6 +112,10 @@ class Ctx(_Handle):
         _check(lib().kpmx_ctx_set_step_variant(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.c_int(vector_width),
                                                ctypes.c_int(variant)))
 
+    def set_adjoint_variant(self, variant):
+        """Pin the adjoint step kernel's variant (-1: the default)."""
+        _check(lib().kpmx_ctx_set_adjoint_variant(self.h, ctypes.c_int(variant)))
+
     def set_graph_policy(self, policy):
         """Graph policy of the diagonal recurrence: 0 never (default), 1 auto (small starter blocks,
         captured on the second identical run), 2 always."""

@@ -136,6 +136,7 @@ struct kpmx_ctx {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int tune[4][7] = {};  // tile-shape variant per (scalar, log2 vw), chosen by kpmx_autotune_diagonal
     int graph_policy = KPMX_GRAPH_NEVER;
+    int adjoint_variant = -1;  // adjoint step variant (kpmx_ctx_set_adjoint_variant); -1: the default
     // device memory of the largest finished adjoint tape, reused by the next tape that fits: a
     // stored tape of a 1 M-site, 16-vector, 1024-moment gradient is 65 GB, whose allocation and
     // mapping would otherwise be paid (and vary widely) on every gradient
@@ -1443,6 +1444,14 @@ int kpmx_ctx_timer_stop(kpmx_ctx* ctx, double* ms) {
     });
 }
 
+int kpmx_ctx_set_adjoint_variant(kpmx_ctx* ctx, int variant) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_REQUIRE(variant >= -1 && variant < NUM_TUNE_VARIANTS, "variant out of range");
+        ctx->adjoint_variant = variant;
+    });
+}
+
 int kpmx_ctx_set_step_variant(kpmx_ctx* ctx, int scalar, int vector_width, int variant) {
     return guarded([&] {
         KPMX_REQUIRE(ctx, "null context");
@@ -1924,9 +1933,12 @@ int kpmx_adjoint_backward(kpmx_tape* T, const double* w, double* grad) {
         A.grad = dgrad;
         A.n = n;
         A.max_width = h2->max_width;
-        // wave-staged entries on twice the resident grid: 155.8 ms against 160.5 ms for the
-        // 1 M-site checkpointed gradient (LDS staging, resident grid)
-        A.tune = NUM_TILE_SHAPES * NUM_GRID_VARIANTS + 3 * NUM_TILE_SHAPES;
+        // default: wave-staged entries on twice the resident grid (155.8 ms against 160.5 ms for
+        // the 1 M-site checkpointed gradient with LDS staging on the resident grid), fp64 blocks of
+        // 8 or more vectors as column pairs
+        const int staging = (h2->scalar == KPMX_F64 && vw >= 8) ? 2 : 1;
+        A.tune = ctx->adjoint_variant >= 0 ? ctx->adjoint_variant
+                                           : staging * NUM_TILE_SHAPES * NUM_GRID_VARIANTS + 3 * NUM_TILE_SHAPES;
         // With C_j = <v_j|H2|v_j>, B_0 = C_0 / 2 and B_j = C_j - B_j-1 (j >= 1, from
         // v_j+1 = H2 v_j - v_j-1), so L = sum_j alpha_j A_j + sum_{j>=1} gamma_j C_j + gamma_0 C_0 / 2
         // with gamma_j = sum_{i>=j} (-1)^(i-j) beta_i.  v_S no longer enters, and

@@ -687,7 +687,7 @@ __global__ void k_reduce_segments(const double* partials, const int64_t* slot_fi
 template <class T>
 struct AdjArgs {
     const int32_t* indices;
-    const T* vals;
+    const typename traits<T>::value* vals;
     const T* lam1;
     const T* lam2;
     T* lam_out;
@@ -700,6 +700,12 @@ struct AdjArgs {
     int64_t n;
     int64_t num_tiles;
 };
+
+// One lane's share of sum over the block's columns of a[c] b[c]: one column, or both columns of a
+// pair (the product of a real column in its own precision, as the single-column sums take it)
+__device__ __forceinline__ double col_product(float a, float b) { return (double)(a * b); }
+__device__ __forceinline__ double col_product(double a, double b) { return a * b; }
+__device__ __forceinline__ double col_product(dpair a, dpair b) { return a.a * b.a + a.b * b.b; }
 
 template <int VW>
 __device__ __forceinline__ double reduce4(double p0, double p1, double p2, double p3, int col) {
@@ -728,8 +734,9 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
     static_assert(!WS || KMAX * RW <= WAVE, "wave staging needs one staged entry per lane at most");
     // WS: each wave stages its own rows' entries (entry k of row r in lane k * RW + r) and hands
     // them out by lane shuffles, without LDS or a workgroup barrier; otherwise LDS staging
+    using V = typename traits<T>::value;   // matrix entries (double for a column-pair block)
     __shared__ int32_t s_idx[WS ? 1 : KMAX * TR];
-    __shared__ T s_val[WS ? 1 : KMAX * TR];
+    __shared__ V s_val[WS ? 1 : KMAX * TR];
     const int lane = threadIdx.x % WAVE;
     const int wave = threadIdx.x / WAVE;
     const int rsub = lane / VW;
@@ -739,12 +746,12 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
     const T* __restrict__ lam1 = a.lam1;
     const T* __restrict__ vj = a.v_j;
     const int32_t* __restrict__ sidx = a.indices;
-    const T* __restrict__ sval = a.vals;
+    const V* __restrict__ sval = a.vals;
     const TileWalk tw = tile_walk(a.num_tiles);
     // software pipeline of the tile's indices and values, as in k_step_sellb
     constexpr int E = (KMAX * TR + BLOCK - 1) / BLOCK;
     int32_t p_idx[E];
-    T p_val[E];
+    V p_val[E];
     auto prefetch = [&](int64_t tile, int64_t& pbase, int& pwidth) {
         const int64_t slc = tile / TPS;
         const int rr0 = (int)(tile % TPS) * TR;
@@ -761,13 +768,13 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
             } else {
                 const int64_t own = slc * WAVE + rr0 + r;
                 p_idx[q] = (int32_t)(own < a.n ? own : a.n - 1);
-                p_val[q] = zero(T{});
+                p_val[q] = zero(V{});
             }
         }
     };
     const int sk = lane / RW, sr = lane % RW;
     int32_t w_idx = 0;
-    T w_val = zero(T{});
+    V w_val = zero(V{});
     auto prefetch_ws = [&](int64_t tile, int64_t& pbase, int& pwidth) {
         const int64_t slc = tile / TPS;
         const int rr0 = (int)(tile % TPS) * TR;
@@ -781,7 +788,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
         } else {
             const int64_t own = slc * WAVE + rr0 + rin;
             w_idx = (int32_t)(own < a.n ? own : a.n - 1);
-            w_val = zero(T{});
+            w_val = zero(V{});
         }
     };
     auto gather = [&](const T* __restrict__ v, int32_t c) -> T {
@@ -806,7 +813,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
         const int width = next_width;
         const int kst = width < KMAX ? width : KMAX;
         const int32_t c_idx = w_idx;
-        const T c_val = w_val;
+        const V c_val = w_val;
         if constexpr (!WS) {
             __syncthreads();
 #pragma unroll
@@ -860,18 +867,18 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
         for (int u = 0; u < U; ++u) {
             const int64_t e0 = base + r0 + rin[u];
             // gradient of the staged entries of this row: (lam1 + cg v_j)[row] * v_j[col]
-            const T rf = fmac((T)a.cg, vo[u], l1[u]);
+            const T rf = fmac((V)a.cg, vo[u], l1[u]);
 #pragma unroll
             for (int k4 = 0; k4 < KMAX; k4 += 4) {
-                const double r = reduce4<VW>((double)(rf * vg[u][k4]), (double)(rf * vg[u][k4 + 1]),
-                                             (double)(rf * vg[u][k4 + 2]), (double)(rf * vg[u][k4 + 3]), col);
+                const double r = reduce4<VW>(col_product(rf, vg[u][k4]), col_product(rf, vg[u][k4 + 1]),
+                                             col_product(rf, vg[u][k4 + 2]), col_product(rf, vg[u][k4 + 3]), col);
                 const int k = k4 + gk;
                 if (glane && ok[u] && k < kst) a.grad[e0 + (int64_t)k * WAVE] = gold[u][k4 / 4] + a.gscale * r;
             }
             T acc_l = zero(T{}), acc_v = zero(T{});
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
-                T av;
+                V av;
                 if constexpr (WS) av = shfl_any(c_val, k * RW + u * RPW + rsub);
                 else av = s_val[k * TR + rin[u]];
                 acc_l = fmac(av, lg[u][k], acc_l);
@@ -881,18 +888,18 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
             for (int k = KMAX; k < width; ++k) {
                 const int64_t off = e0 + (int64_t)k * WAVE;
                 const int32_t c = sidx[off];
-                const T av = sval[off];
+                const V av = sval[off];
                 const T xl = gather(lam1, c), xv = gather(vj, c);
                 acc_l = fmac(av, xl, acc_l);
                 acc_v = fmac(av, xv, acc_v);
-                double g = (double)(rf * xv);
+                double g = col_product(rf, xv);
                 for (int w = VW / 2; w >= 1; w >>= 1) g += __shfl_xor(g, w, WAVE);
                 if (col == 0 && ok[u]) a.grad[off] += a.gscale * g;
             }
             if (a.update && ok[u]) {
                 T lnew = sub(acc_l, l2[u]);
-                lnew = fmac((T)a.ca, vo[u], lnew);
-                lnew = fmac((T)a.chv, acc_v, lnew);
+                lnew = fmac((V)a.ca, vo[u], lnew);
+                lnew = fmac((V)a.chv, acc_v, lnew);
                 nt_store(lnew, &a.lam_out[row[u] * VW + col]);
             }
             if (a.v_prev && ok[u]) nt_store(sub(acc_v, vn[u]), &a.v_prev[row[u] * VW + col]);
@@ -901,9 +908,8 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_sellb(AdjArgs<T> a, const int
 }
 
 // sums[s][j] = sum over workgroups g (in order) of partials[s][g][j], for s in [0, nsteps).
-// Block (s, chunk) covers columns [64 chunk, 64 chunk + 64) of step s: 4 groups of 64 threads
-// sum interleaved workgroup subsets, then the 4 subtotals are combined in order through LDS.
-// Sums of one step's per-workgroup partials (width doubles each) in workgroup order.  With
+// Block (s, chunk) covers outputs [64 chunk, 64 chunk + 64) of step s: 4 groups of 64 threads
+// sum interleaved workgroup subsets, then the 4 subtotals are combined in order through LDS.  With
 // pair_cols > 0 the partials are column-pair sums ([4][pair_cols / 2]: A of the even columns, A of
 // the odd ones, B of the even, B of the odd) and come out in the standard [4][pair_cols] layout
 // (A, imaginary A = 0, B, imaginary B = 0).
@@ -1564,8 +1570,9 @@ static void adjoint_rw_launch(AdjArgs<T> a, const AdjointLaunch& A, hipStream_t 
         if (g >= 8) g -= g % 8;
         hipLaunchKernelGGL(kernel, dim3((unsigned)(g < 1 ? 1 : g)), dim3(BLOCK), 0, stream, a, A.slice_ptr, A.slice_width);
     };
-    // variants 15..29 stage the entries per wave where they fit its lanes
-    const bool ws = A.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) == 1;
+    // variants 15..44 stage the entries per wave where they fit its lanes (30..44 on column
+    // pairs for fp64 blocks, see adjoint_t)
+    const bool ws = A.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) >= 1;
     if (A.max_width <= 4) {
         if constexpr (4 * RW <= WAVE) {
             if (ws) {
@@ -1591,9 +1598,16 @@ static void adjoint_rw_launch(AdjArgs<T> a, const AdjointLaunch& A, hipStream_t 
 
 template <class T, int VW>
 static void adjoint_t(const AdjointLaunch& A, hipStream_t stream) {
+    // fp64 blocks of 8 or more vectors as column pairs (16-byte lanes)
+    if constexpr (std::is_same<T, double>::value && VW >= 8) {
+        if (A.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) == 2) {
+            adjoint_t<dpair, VW / 2>(A, stream);
+            return;
+        }
+    }
     AdjArgs<T> a{};
     a.indices = A.indices;
-    a.vals = static_cast<const T*>(A.vals);
+    a.vals = static_cast<const typename traits<T>::value*>(A.vals);
     a.lam1 = static_cast<const T*>(A.lam1);
     a.lam2 = static_cast<const T*>(A.lam2);
     a.lam_out = static_cast<T*>(A.lam_out);

@@ -140,3 +140,36 @@ def test_site_order_does_not_change_gradient():
     np.testing.assert_allclose(out[1][0], out[0][0], rtol=0, atol=1e-12 * abs(out[0][0][0]))
     for k in (1, 2):
         np.testing.assert_allclose(out[1][k], out[0][k], rtol=1e-10, atol=1e-12 * np.abs(out[0][k]).max())
+
+
+@pytest.mark.parametrize("nvec", [8, 16, 64])
+def test_adjoint_variants_agree(oracle, nvec):
+    """The adjoint step's kernel variants (LDS staging, wave staging, fp64 column pairs; several
+    grids and tile shapes) give the same gradient, checkpointed and stored, and the forward moments
+    of the oracle."""
+    n, bonds, onsite, hop = strained_system(7)
+    M = 42
+    hm = sp.coo_matrix((np.r_[onsite, hop, hop], (np.r_[np.arange(n), bonds[:, 0], bonds[:, 1]],
+                                                   np.r_[np.arange(n), bonds[:, 1], bonds[:, 0]])), shape=(n, n)).tocsr()
+    a, b = oracle.scale_factors(-9, 9)
+    h2 = oracle.scale_hamiltonian(hm, a, b, np.float64)
+    h2.sort_indices()
+    r0 = oracle.random_starter(n, nvec, np.float64)
+    w = np.random.default_rng(11).standard_normal(M)
+    ctx = abi.Ctx(0)
+    mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
+    try:
+        ref_g, mu = abi.adjoint_diagonal(mat, blk, M, w, checkpoint=0)
+        ref = oracle.diag_moments(h2, r0, M)
+        assert np.max(np.abs(mu - ref) / np.abs(ref[:, :1])) <= 1e-10
+        scale = np.abs(ref_g).max()
+        for variant in (0, 1, 2, 15, 24, 30, 31, 32, 39, 44):
+            ctx.set_adjoint_variant(variant)
+            for ck in (0, 6):
+                g, _ = abi.adjoint_diagonal(mat, blk, M, w, checkpoint=ck)
+                assert np.abs(g - ref_g).max() <= 1e-11 * scale, (variant, ck)
+    finally:
+        ctx.set_adjoint_variant(-1)
+        blk.close()
+        mat.close()
+        ctx.close()
