@@ -739,6 +739,12 @@ int kpmx_ctx_create(int device, kpmx_ctx** out) {
             ctx->num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
             KPMX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
             ctx->scratch = static_cast<char*>(scratch_address(device));
+            // default step variants: fp64 blocks of 16 or more vectors as column pairs (the tuned
+            // choice on the 4.19 M-site 64-vector and 1 M-site 16-vector workloads)
+            const int pairs = 2 * NUM_TILE_SHAPES * NUM_GRID_VARIANTS;
+            ctx->tune[KPMX_F64][4] = pairs + 1;                     // 16 vectors: tile shape 1, resident grid
+            ctx->tune[KPMX_F64][5] = pairs + 1;                     // 32 vectors
+            ctx->tune[KPMX_F64][6] = pairs + 1 + NUM_TILE_SHAPES;   // 64 vectors: tile shape 1, half grid
         } catch (...) {
             delete ctx;
             throw;
