@@ -183,11 +183,13 @@ def cpu_baseline(cells, dtype, nvec, target_seconds=15.0):
                       % (nvec, m, t)}
 
 
-def batched_realizations(steps=2):
+def batched_realizations(steps=2, site_order="hilbert", partition=None):
     """256 square-lattice flakes (one orbital, t = -1), sizes log-uniform in [1e4, 5e5] sites
     (seed 3), Anderson W_f uniform in [0, 4] (seed 4), bounds +-(4 + W_f/2), fp64, 1024 moments,
     4 reference random vectors per flake, all flakes in one block-diagonal recurrence replayed from
-    one HIP graph.  Returns the GNNZ*moments/s figure of that run."""
+    one HIP graph; each flake's sites in the Hilbert order of their positions ("hilbert") or in
+    row-major order ("given").  partition: (workgroups, walk_chunk) to use instead of the
+    autotuner's choice.  Returns the GNNZ*moments/s figure of that run."""
     from pybinding_amd import generators
     from pybinding_amd.batch import kpm_batch
     t0 = time.time()
@@ -195,19 +197,26 @@ def batched_realizations(steps=2):
     sizes = np.exp(rs.uniform(np.log(1e4), np.log(5e5), 256))
     rw = np.random.default_rng(4)
     Ws = rw.uniform(0, 4, 256)
-    hs, ranges = [], []
+    hs, ranges, pos = [], [], []
     for sz, W in zip(sizes, Ws):
         side = max(2, int(round(np.sqrt(sz))))
         hs.append(generators.square_flake(side, side, t=-1.0, disorder=W, rng=rw))
         ranges.append((-(4 + W / 2), 4 + W / 2))
-    kb = kpm_batch(hs, ranges)
+        pos.append(generators.square_positions(side, side))
+    kb = kpm_batch(hs, ranges, positions=pos if site_order == "hilbert" else None)
     kb.set_random_starters(4)
     n = sum(kb.sizes)
     nnz = sum(h.nnz for h in hs)
     log("batched: %d flakes, %d sites, nnz %d (%.1fs setup)" % (len(hs), n, nnz, time.time() - t0))
     M = 1024
-    wg = kb.autotune(4)
-    log("batched: autotuned partition over %d workgroups" % wg)
+    if partition is None:
+        wg, chunk = kb.autotune(4)
+    else:
+        wg, chunk = partition
+        kb.set_partition(wg, chunk)
+    log("batched: %s partition over %d workgroups (%s)"
+        % ("autotuned" if partition is None else "given", wg,
+           "XCD-aware walk, runs of %d slices" % chunk if chunk else "contiguous ranges"))
     kb.batch.run(M, True)  # capture + warm-up
     kb.ctx.synchronize()
     kb.ctx.timer_start()
@@ -223,6 +232,7 @@ def batched_realizations(steps=2):
     gbps = bytes_step * (M // 2) / (ms / 1000) / 1e9
     log("batched: %.1f ms per run (one graph launch), %.1f GNNZ*moments/s, %.0f GB/s algorithmic" % (ms, value, gbps))
     return {"value": value, "unit": "GNNZ*moments/s", "ms_per_run": ms, "achieved_GBps": gbps, "workgroups": wg,
+            "walk_chunk": chunk, "site_order": site_order,
             "workload": "256 square flakes, %d sites, 1024 moments, 4 vectors each, one HIP graph" % n}
 
 

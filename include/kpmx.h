@@ -222,12 +222,20 @@ int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int6
                       const int32_t* indptr, const int32_t* indices, const void* values, kpmx_batch** out);
 int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0);
 int kpmx_batch_run(kpmx_batch* b, int64_t num_moments, int use_graph);
-/* Autotuner of the batched step (no counterpart in the reference): times the step with the
- * slices dealt as contiguous ranges to 1, 2, 4, ..., 64 workgroups per CU (shorter ranges, more
- * per-system partial sums) on this batch and its starters, and keeps the fastest partition.
- * num_workgroups (optional): the chosen workgroup count.  Results stay within rounding of any
- * other partition (the per-system sums are grouped differently). */
-int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups);
+/* Partition of the batched step's 64-row slices over the walkers of num_workgroups workgroups
+ * (four per workgroup, one per wave; no counterpart in the reference).  walk_chunk == 0:
+ * contiguous ranges.  walk_chunk > 0 (num_workgroups a multiple of 8): XCD-aware walk -- 8
+ * contiguous regions, one per XCD, whose runs of walk_chunk slices are dealt round-robin to that
+ * XCD's walkers, so the gathered rows of the band they sweep together stay in the XCD's L2.
+ * Results stay within rounding of any other partition (only the grouping of each system's sums
+ * changes). */
+int kpmx_batch_set_partition(kpmx_batch* b, int num_workgroups, int walk_chunk);
+/* Autotuner of the batched step (no counterpart in the reference): times the step on this batch
+ * and its starters over contiguous ranges for 1 to 64 workgroups per CU (four walkers, one per
+ * wave, each) and over the XCD-aware walk (runs of 1, 2 or 4 slices on the resident grid and half
+ * of it), and keeps the fastest partition.  num_workgroups, walk_chunk (optional): the chosen
+ * partition. */
+int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups, int* walk_chunk);
 int kpmx_batch_fetch(kpmx_batch* b, int64_t num_moments, void* mu);
 int kpmx_batch_destroy(kpmx_batch* b);
 

@@ -272,11 +272,16 @@ class Batch(_Handle):
         _check(lib().kpmx_batch_run(self.h, ctypes.c_int64(num_moments), ctypes.c_int(1 if use_graph else 0)))
 
     def autotune(self):
-        """Pick the fastest slice partition over workgroups (kpmx_batch_autotune); returns the
-        workgroup count."""
-        wg = ctypes.c_int()
-        _check(lib().kpmx_batch_autotune(self.h, ctypes.byref(wg)))
-        return wg.value
+        """Pick the fastest slice partition (kpmx_batch_autotune); returns (workgroups, walk_chunk),
+        walk_chunk 0 for contiguous ranges."""
+        wg, chunk = ctypes.c_int(), ctypes.c_int()
+        _check(lib().kpmx_batch_autotune(self.h, ctypes.byref(wg), ctypes.byref(chunk)))
+        return wg.value, chunk.value
+
+    def set_partition(self, num_workgroups, walk_chunk=0):
+        """Deal the slices to `num_workgroups` workgroups: contiguous ranges (walk_chunk 0) or the
+        XCD-aware walk with runs of `walk_chunk` slices (kpmx_batch_set_partition)."""
+        _check(lib().kpmx_batch_set_partition(self.h, ctypes.c_int(num_workgroups), ctypes.c_int(walk_chunk)))
 
     def fetch(self, num_moments):
         out = np.empty((len(self.sizes), self.nvec, num_moments), dtype=self.dtype)

@@ -229,9 +229,11 @@ struct kpmx_batch {
     int32_t* sidx = nullptr;
     void* svals = nullptr;
     int num_wg = 0;
+    int walk_chunk = 0;  // BatchWalk of the step kernel
     int64_t num_slots = 0;
     int64_t* tile_begin = nullptr;
     int64_t* slot_begin = nullptr;
+    int64_t* slot_index = nullptr;
     int64_t* slot_first = nullptr;
     // starters and workspaces
     int64_t nvec = 0;
@@ -255,7 +257,7 @@ struct kpmx_batch {
     ~kpmx_batch() {
         drop_graph();
         for (void* p : {(void*)slice_ptr, (void*)slice_width, (void*)slice_sys, (void*)sidx, svals, (void*)tile_begin,
-                        (void*)slot_begin, (void*)slot_first, r0, w0, w1, (void*)partials, (void*)sums})
+                        (void*)slot_begin, (void*)slot_index, (void*)slot_first, r0, w0, w1, (void*)partials, (void*)sums})
             dev_free(p);
     }
 };
@@ -631,40 +633,62 @@ void tridiagonal_minmax(const std::vector<double>& a, const std::vector<double>&
 
 // ==================================================================================== C-ABI
 namespace {
-// Workgroup w of the batched step owns the contiguous slices [tile_begin[w], tile_begin[w+1]);
-// its per-system partial sums go to slots slot_begin[w] ... (one per system it meets), and
-// slot_first[s] is the first slot of system s.  Rebuilding the partition drops the captured
-// graph and the partial-sum buffer (both depend on it).
-void batch_partition(kpmx_batch* b, int num_wg) {
+// Walker w of the batched step (one per wave, BATCH_WALKERS_PER_WG per workgroup) walks its
+// slices along BatchWalk (chunk 0: the contiguous slices [tile_begin[w], tile_begin[w+1]);
+// chunk > 0: the XCD-aware round-robin runs) and meets each system in one stretch.  Replaying the
+// walks numbers the stretches (walker w's first is slot_begin[w]) and gives each its partial
+// slot, slot_index[stretch]: the slots of one system are consecutive, in walker order, from
+// slot_first[s].  Rebuilding the partition drops the captured graph and the partial-sum buffer
+// (both depend on it).
+void batch_partition(kpmx_batch* b, int num_wg, int chunk) {
     kpmx_ctx* ctx = b->ctx;
     const int64_t ns_tot = b->nslices;
-    num_wg = (int)std::max<int64_t>(1, std::min<int64_t>(ns_tot, num_wg));
-    std::vector<int64_t> tb(num_wg + 1), sbeg(num_wg);
-    for (int w = 0; w <= num_wg; ++w) tb[w] = ns_tot * w / num_wg;
-    std::vector<int64_t> slot_sys;
-    const auto& ssys = b->slice_sys_host;
-    for (int w = 0; w < num_wg; ++w) {
-        sbeg[w] = (int64_t)slot_sys.size();
-        if (tb[w] == tb[w + 1]) continue;
-        for (int sy = ssys[tb[w]]; sy <= ssys[tb[w + 1] - 1]; ++sy) slot_sys.push_back(sy);
+    constexpr int PER = BATCH_WALKERS_PER_WG;
+    if (chunk > 0) {
+        num_wg = std::max(8, num_wg / 8 * 8);
+    } else {
+        num_wg = (int)std::max<int64_t>(1, std::min<int64_t>((ns_tot + PER - 1) / PER, num_wg));
     }
-    const int64_t num_slots = (int64_t)slot_sys.size();
-    std::vector<int64_t> sfirst(b->num_sys + 1, num_slots);
-    for (int64_t p = num_slots - 1; p >= 0; --p) sfirst[slot_sys[p]] = p;
-    for (int64_t sy = b->num_sys - 1; sy >= 0; --sy) sfirst[sy] = std::min(sfirst[sy], sfirst[sy + 1]);
+    const int walkers = num_wg * PER;
+    std::vector<int64_t> tb(walkers + 1, 0), sbeg(walkers);
+    if (chunk <= 0)
+        for (int w = 0; w <= walkers; ++w) tb[w] = ns_tot * w / walkers;
+    std::vector<int32_t> stretch_sys;  // system of every stretch, in walker order
+    const auto& ssys = b->slice_sys_host;
+    for (int w = 0; w < walkers; ++w) {
+        sbeg[w] = (int64_t)stretch_sys.size();
+        int prev = -1;
+        for (BatchWalk walk(ns_tot, walkers, w, chunk, tb.data()); walk.valid(); walk.next()) {
+            if (ssys[walk.s] != prev) stretch_sys.push_back(ssys[walk.s]);
+            prev = ssys[walk.s];
+        }
+    }
+    const int64_t num_slots = (int64_t)stretch_sys.size();
+    // counting sort of the stretches by system (stable: walker order inside a system)
+    std::vector<int64_t> sfirst(b->num_sys + 1, 0), sidx(std::max<int64_t>(num_slots, 1));
+    for (int32_t sy : stretch_sys) ++sfirst[sy + 1];
+    for (int64_t sy = 0; sy < b->num_sys; ++sy) sfirst[sy + 1] += sfirst[sy];
+    {
+        std::vector<int64_t> fill(sfirst.begin(), sfirst.end() - 1);
+        for (int64_t p = 0; p < num_slots; ++p) sidx[p] = fill[stretch_sys[p]]++;
+    }
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
     b->drop_graph();
-    for (void** p : {(void**)&b->tile_begin, (void**)&b->slot_begin, (void**)&b->slot_first, (void**)&b->partials}) {
+    for (void** p : {(void**)&b->tile_begin, (void**)&b->slot_begin, (void**)&b->slot_index, (void**)&b->slot_first,
+                     (void**)&b->partials}) {
         dev_free(*p);
         *p = nullptr;
     }
     b->num_wg = num_wg;
+    b->walk_chunk = chunk;
     b->num_slots = num_slots;
     b->tile_begin = static_cast<int64_t*>(dev_alloc(tb.size() * sizeof(int64_t)));
     b->slot_begin = static_cast<int64_t*>(dev_alloc(sbeg.size() * sizeof(int64_t)));
+    b->slot_index = static_cast<int64_t*>(dev_alloc(sidx.size() * sizeof(int64_t)));
     b->slot_first = static_cast<int64_t*>(dev_alloc(sfirst.size() * sizeof(int64_t)));
     KPMX_HIP(hipMemcpyAsync(b->tile_begin, tb.data(), tb.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     KPMX_HIP(hipMemcpyAsync(b->slot_begin, sbeg.data(), sbeg.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(b->slot_index, sidx.data(), sidx.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
     KPMX_HIP(hipMemcpyAsync(b->slot_first, sfirst.data(), sfirst.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                             ctx->stream));
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
@@ -1594,7 +1618,7 @@ int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int6
         up(b->slice_sys, ssys.data(), ssys.size() * sizeof(int32_t));
         up(b->sidx, sidx.data(), sidx.size() * sizeof(int32_t));
         up(b->svals, svals.data(), svals.size());
-        batch_partition(b.get(), (int)std::min<int64_t>(ns_tot, (int64_t)ctx->num_cu * 4));
+        batch_partition(b.get(), ctx->num_cu * 4, 0);
         *out = b.release();
     });
 }
@@ -1637,13 +1661,13 @@ int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0) {
 }
 
 namespace {
-void batch_enqueue(kpmx_batch* b, int64_t steps) {
-    kpmx_ctx* ctx = b->ctx;
-    const int width = 4 * b->vw;
+BatchedLaunch batch_launch(const kpmx_batch* b) {
     BatchedLaunch B;
     B.scalar = b->scalar;
     B.vw = b->vw;
     B.num_wg = b->num_wg;
+    B.chunk = b->walk_chunk;
+    B.nslices = b->nslices;
     B.max_width = b->max_width;
     B.indices = b->sidx;
     B.vals = b->svals;
@@ -1652,8 +1676,16 @@ void batch_enqueue(kpmx_batch* b, int64_t steps) {
     B.slice_sys = b->slice_sys;
     B.tile_begin = b->tile_begin;
     B.slot_begin = b->slot_begin;
+    B.slot_index = b->slot_index;
     B.partials = b->partials;
     B.n = b->n_pad;
+    return B;
+}
+
+void batch_enqueue(kpmx_batch* b, int64_t steps) {
+    kpmx_ctx* ctx = b->ctx;
+    const int width = 4 * b->vw;
+    BatchedLaunch B = batch_launch(b);
     for (int64_t k = 0; k < steps; ++k) {
         if (k == 0) {
             B.x = b->r0;
@@ -1718,7 +1750,18 @@ int kpmx_batch_run(kpmx_batch* b, int64_t M, int use_graph) {
     });
 }
 
-int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups) {
+int kpmx_batch_set_partition(kpmx_batch* b, int num_workgroups, int walk_chunk) {
+    return guarded([&] {
+        KPMX_REQUIRE(b, "null batch");
+        KPMX_REQUIRE(num_workgroups >= 1 && walk_chunk >= 0, "invalid partition");
+        KPMX_REQUIRE(walk_chunk == 0 || (num_workgroups >= 8 && num_workgroups % 8 == 0),
+                     "the XCD-aware walk needs a multiple of 8 workgroups");
+        KPMX_HIP(hipSetDevice(b->ctx->device));
+        batch_partition(b, num_workgroups, walk_chunk);
+    });
+}
+
+int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups, int* walk_chunk) {
     return guarded([&] {
         KPMX_REQUIRE(b, "null batch");
         KPMX_REQUIRE(b->r0, "no starters uploaded");
@@ -1733,12 +1776,24 @@ int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups) {
         dev_free(b->sums);
         b->sums = static_cast<double*>(dev_alloc((size_t)steps * b->num_sys * 64 * 4 * sizeof(double)));
         b->sums_steps = -1;  // the timing runs leave no result to fetch
-        int best = 0;
+        // candidates: contiguous ranges over 1 ... 64 workgroups per CU (in steps of half a
+        // workgroup per CU up to 4), and the XCD-aware walk on the resident grid (and half of it)
+        // with runs of 1, 2 or 4 slices
+        std::vector<std::pair<int, int>> cand;  // (workgroups, chunk)
+        for (int half_per_cu : {2, 3, 4, 5, 6, 7, 8, 12, 16, 32, 64, 128}) {
+            const int wg = (int)std::min<int64_t>((b->nslices + BATCH_WALKERS_PER_WG - 1) / BATCH_WALKERS_PER_WG,
+                                                  (int64_t)ctx->num_cu * half_per_cu / 2);
+            if (!cand.empty() && wg == cand.back().first) break;  // no more slices to spread
+            cand.push_back({wg, 0});
+        }
+        const int resident = ctx->num_cu * batched_blocks_per_cu(batch_launch(b));
+        for (int chunk : {1, 2, 4})
+            for (int wg : {resident, resident / 2})
+                if (wg >= 8) cand.push_back({wg, chunk});
+        int best = -1;
         float best_ms = 0;
-        for (int per_cu : {1, 2, 4, 8, 16, 32, 64}) {
-            const int wg = (int)std::min<int64_t>(b->nslices, (int64_t)ctx->num_cu * per_cu);
-            if (per_cu > 1 && wg == b->num_wg && wg < ctx->num_cu * per_cu) break;  // no more slices to spread
-            batch_partition(b, wg);
+        for (size_t c = 0; c < cand.size(); ++c) {
+            batch_partition(b, cand[c].first, cand[c].second);
             b->partials = static_cast<double*>(dev_alloc((size_t)b->num_slots * 64 * 4 * sizeof(double)));
             float ms_w = 0;
             for (int rep = 0; rep < 3; ++rep) {  // the first repetition warms caches and code
@@ -1751,12 +1806,14 @@ int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups) {
                 KPMX_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
                 if (rep == 1 || (rep > 1 && ms < ms_w)) ms_w = ms;
             }
-            if (best == 0 || ms_w < best_ms) {
+            if (best < 0 || ms_w < best_ms) {
                 best_ms = ms_w;
-                best = b->num_wg;
+                best = (int)c;
             }
         }
-        if (best != b->num_wg) batch_partition(b, best);
+        if (b->num_wg != cand[best].first || b->walk_chunk != cand[best].second)
+            batch_partition(b, cand[best].first, cand[best].second);
+        if (walk_chunk) *walk_chunk = b->walk_chunk;
         if (num_workgroups) *num_workgroups = b->num_wg;
     });
 }

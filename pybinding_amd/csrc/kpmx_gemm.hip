@@ -2,9 +2,10 @@
 // (the reference's MomentMultiplication::matrix_mul_add, cppcore/src/kpm/Moments.cpp:92-101).
 //
 // Three implementations of the same M x M contraction over a long k (= sites x vectors):
-//   mode 0 (FP64): 64x64 output tiles, 16-deep k panels staged in LDS, every product and sum in
-//                  double on the vector ALUs (FP64 vector peak equals the FP64 matrix peak on
-//                  gfx950), accumulation exact to double rounding.
+//   mode 0 (FP64 MFMA): 128x64 output tiles on v_mfma_f64_16x16x4f64, 16-deep k panels staged
+//                  in LDS with the next panel prefetched into registers, every product and sum in
+//                  double (accumulation exact to double rounding); persistent XCD-aware grid
+//                  (x3_plan) like mode 2.
 //   mode 1 (FP32 split MFMA): each operand is split into a float "hi" part and a float "lo"
 //                  remainder (x = hi + lo), and L R^H ~ Lhi Rhi^H + Lhi Rlo^H + Llo Rhi^H is
 //                  formed with v_mfma_f32_32x32x2_f32; the float accumulators are flushed into
@@ -25,83 +26,10 @@ namespace kpmx {
 namespace {
 
 constexpr int TM = 64;   // output tile edge
-constexpr int TK = 16;   // k panel depth (FP64 path)
 
 __device__ __forceinline__ d2 to_d2(float v) { return {double(v), 0.0}; }
 __device__ __forceinline__ d2 to_d2(double v) { return {v, 0.0}; }
 template <class R> __device__ __forceinline__ d2 to_d2(cplx<R> v) { return {double(v.re), double(v.im)}; }
-
-template <class T>
-__global__ __launch_bounds__(256) void k_gemm_nc_fp64(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
-                                                      int64_t ldb, int64_t M, int64_t kdim, int64_t kslice,
-                                                      double* __restrict__ P) {
-    constexpr bool CPLX = traits<T>::is_complex;
-    __shared__ d2 As[TK][TM + 1];
-    __shared__ d2 Bs[TK][TM + 1];
-    const int t = threadIdx.x;
-    const int tx = t % 16, ty = t / 16;
-    const int64_t tiles = (M + TM - 1) / TM;
-    const int64_t tile = blockIdx.x % (tiles * tiles);
-    const int64_t s = blockIdx.x / (tiles * tiles);
-    const int64_t n0 = (tile / tiles) * TM, m0 = (tile % tiles) * TM;
-    const int64_t kbeg = s * kslice;
-    const int64_t kend = (kbeg + kslice < kdim) ? kbeg + kslice : kdim;
-
-    double cr[4][4], ci[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) cr[i][j] = ci[i][j] = 0.0;
-
-    for (int64_t k0 = kbeg; k0 < kend; k0 += TK) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = t + 256 * q;
-            const int r = e / TK, kk = e % TK;
-            const int64_t k = k0 + kk;
-            const bool kin = k < kend;
-            As[kk][r] = (kin && n0 + r < M) ? to_d2(A[(n0 + r) * lda + k]) : d2{0.0, 0.0};
-            Bs[kk][r] = (kin && m0 + r < M) ? to_d2(B[(m0 + r) * ldb + k]) : d2{0.0, 0.0};
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < TK; ++kk) {
-            d2 a[4], b[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    // a * conj(b)
-                    cr[i][j] += a[i].re * b[j].re;
-                    if (CPLX) {
-                        cr[i][j] += a[i].im * b[j].im;
-                        ci[i][j] += a[i].im * b[j].re - a[i].re * b[j].im;
-                    }
-                }
-        }
-        __syncthreads();
-    }
-    const int width = CPLX ? 2 : 1;
-    double* out = P + s * M * M * width;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t n = n0 + ty + 16 * i, m = m0 + tx + 16 * j;
-            if (n < M && m < M) {
-                if (CPLX) {
-                    out[(n * M + m) * 2] = cr[i][j];
-                    out[(n * M + m) * 2 + 1] = ci[i][j];
-                } else {
-                    out[n * M + m] = cr[i][j];
-                }
-            }
-        }
-}
 
 // ---------------------------------------------------------------------------- FP32 split MFMA
 // One 64-lane wave computes a 32x32 output block with v_mfma_f32_32x32x2_f32:
@@ -581,16 +509,11 @@ static void gemm_t(int mode, const void* A, int64_t lda, const void* B, int64_t 
     const int64_t S = num_slices(M, kdim);
     {
         int64_t kslice = (kdim + S - 1) / S;
-        const int64_t align = mode == 1 ? KP : TK;
-        kslice = (kslice + align - 1) / align * align;
+        kslice = (kslice + KP - 1) / KP * KP;
         const int64_t tiles = (M + TM - 1) / TM;
         const dim3 grid((unsigned)(tiles * tiles * S)), block(256);
-        if (mode == 1)
-            hipLaunchKernelGGL(k_gemm_nc_split<T>, grid, block, 0, s, static_cast<const T*>(A), lda,
-                               static_cast<const T*>(B), ldb, M, kdim, kslice, work);
-        else
-            hipLaunchKernelGGL(k_gemm_nc_fp64<T>, grid, block, 0, s, static_cast<const T*>(A), lda,
-                               static_cast<const T*>(B), ldb, M, kdim, kslice, work);
+        hipLaunchKernelGGL(k_gemm_nc_split<T>, grid, block, 0, s, static_cast<const T*>(A), lda,
+                           static_cast<const T*>(B), ldb, M, kdim, kslice, work);
     }
     int64_t g = (count + 255) / 256;
     if (g > 65536) g = 65536;

@@ -39,15 +39,56 @@ struct StepLaunch {
     int tune = 0;             // tile-shape variant of the SELL block kernel (0 default, 1 smaller, 2 larger)
 };
 
+// Slice sequence of walker b of nb of the batched step (the device kernel walks it and the host
+// replays it to number the partial-sum slots).
+//   chunk == 0: the contiguous slices [tile_begin[b], tile_begin[b+1]).
+//   chunk > 0 (nb a multiple of 8): the slices are cut into 8 contiguous regions, region r
+//     belonging to the walkers with b % 8 == r (on the XCD r of their workgroup), and inside a
+//     region runs of `chunk` consecutive slices are dealt round-robin to its walkers, so the
+//     resident walkers of one XCD sweep one narrow band of rows together.
+// Slices only increase along a walk, so a walker meets every system in one stretch.
+struct BatchWalk {
+    int64_t s = 0, end = 0, jump = 0;
+    int64_t left = 0, chunk = 0;
+    __host__ __device__ BatchWalk(int64_t nslices, int nb, int b, int chunk_, const int64_t* tile_begin) {
+        if (chunk_ <= 0) {
+            s = tile_begin[b];
+            end = tile_begin[b + 1];
+            chunk = end - s + 1;  // never runs out
+            jump = 0;
+        } else {
+            const int region = b % 8, local = b / 8, per_region_wg = nb / 8;
+            const int64_t per_region = (nslices + 7) / 8;
+            const int64_t start = region * per_region;
+            end = start + per_region < nslices ? start + per_region : nslices;
+            chunk = chunk_;
+            s = start + (int64_t)local * chunk;
+            jump = (int64_t)(per_region_wg - 1) * chunk;
+        }
+        left = chunk;
+    }
+    __host__ __device__ bool valid() const { return s < end; }
+    __host__ __device__ void next() {
+        ++s;
+        if (--left == 0) {
+            s += jump;
+            left = chunk;
+        }
+    }
+};
+
 struct BatchedLaunch {
     int scalar = 1, vw = 4, num_wg = 0, max_width = 0;
+    int chunk = 0;                          // walk of the slices (see BatchWalk)
+    int64_t nslices = 0;
     const int32_t* indices = nullptr;
     const void* vals = nullptr;
     const int64_t* slice_ptr = nullptr;
     const int32_t* slice_width = nullptr;
     const int32_t* slice_sys = nullptr;
-    const int64_t* tile_begin = nullptr;  // num_wg + 1
-    const int64_t* slot_begin = nullptr;  // num_wg
+    const int64_t* tile_begin = nullptr;  // walkers + 1 (chunk == 0)
+    const int64_t* slot_begin = nullptr;  // walkers: first stretch of every walker
+    const int64_t* slot_index = nullptr;  // partial slot of every stretch
     const void* x = nullptr;
     const void* yin = nullptr;
     void* yout = nullptr;
@@ -56,7 +97,11 @@ struct BatchedLaunch {
     double hscale = 1.0;
     int sub_y = 1;
 };
+// The batched step kernel runs one walker per wave.
+constexpr int BATCH_WALKERS_PER_WG = 4;
 int launch_batched_step(const BatchedLaunch& B, hipStream_t stream);
+// Workgroups of the batched step kernel that B selects that fit on one CU at once.
+int batched_blocks_per_cu(const BatchedLaunch& B);
 void launch_reduce_segments(const double* partials, const int64_t* slot_first, int64_t num_sys, int width, double* sums,
                             hipStream_t stream);
 

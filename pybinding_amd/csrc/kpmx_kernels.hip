@@ -538,56 +538,96 @@ constexpr bool pairs_fit(int vw, int kmax) {
            kmax * rw_of(vw / 2, 2) <= WAVE;
 }
 
+// Write one wave's diagonal sums (reduced over its row groups) as partial slot `slot` in the
+// standard [4][vw] layout; a column-pair block (dpair, VW pairs) holds columns 2p and 2p + 1 of
+// pair p in the (re, im) halves of its sums.
+template <class T, int VW>
+__device__ __forceinline__ void wave_diag_store(double a_re, double a_im, double b_re, double b_im, double* partials,
+                                                int64_t slot, int lane) {
+    for (int off = WAVE / 2; off >= VW; off >>= 1) {
+        a_re += __shfl_xor(a_re, off, WAVE);
+        a_im += __shfl_xor(a_im, off, WAVE);
+        b_re += __shfl_xor(b_re, off, WAVE);
+        b_im += __shfl_xor(b_im, off, WAVE);
+    }
+    if (lane < VW) {
+        if constexpr (std::is_same<T, dpair>::value) {
+            constexpr int W = 2 * VW;
+            double* p = partials + slot * 4 * W + 2 * lane;
+            p[0] = a_re;
+            p[1] = a_im;
+            p[W] = 0;
+            p[W + 1] = 0;
+            p[2 * W] = b_re;
+            p[2 * W + 1] = b_im;
+            p[3 * W] = 0;
+            p[3 * W + 1] = 0;
+        } else {
+            double* p = partials + slot * 4 * VW + lane;
+            p[0] = a_re;
+            p[VW] = a_im;
+            p[2 * VW] = b_re;
+            p[3 * VW] = b_im;
+        }
+    }
+}
+
 // Batched independent systems: a block-diagonal SELL-64 matrix whose diagonal blocks (one per
-// system) start on slice boundaries, with VW >= 4 vectors per system.  Workgroup b owns the
-// contiguous tiles [tile_begin[b], tile_begin[b+1]) (one 64-row slice per tile), so it meets each
-// system in one stretch; the diagonal-moment sums are flushed to a new partial slot whenever the
-// system changes (slot numbering precomputed on the host: slot_begin[b] is the first slot of
-// workgroup b), and k_reduce_segments sums the slots of every system in order.
+// system) start on slice boundaries, VW lanes per row (VW >= 2 pairs for fp64, VW >= 4 columns
+// otherwise).  Every wave is an independent walker: walker w = wave * gridDim.x + blockIdx.x
+// (so w % 8 == blockIdx.x % 8, the XCD of its workgroup) takes whole 64-row slices in increasing
+// order along BatchWalk -- contiguous ranges, or the XCD-aware round-robin runs -- and meets each
+// system in one stretch.  Lane l loads the first KMAX entries of row l of the slice (coalesced,
+// the next slice's entries prefetched into registers while the current gathers are in flight) and
+// hands them to the gathering lanes by shuffles, so there is no LDS staging and no barrier.  The
+// slice's VW row groups are processed UC at a time, all their loads issued before the first
+// multiply.  The diagonal sums are flushed to a new partial slot whenever the system changes:
+// the host numbers the stretches by replaying the walks (walker w's j-th stretch is stretch
+// slot_begin[w] + j, whose sums go to partial slot_index[stretch]); the partials of one system
+// are consecutive, in walker order, and k_reduce_segments sums them in order.
 template <class T, int VW, int KMAX, bool OFF32>
 __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
                                                         const int32_t* __restrict__ slice_width,
                                                         const int32_t* __restrict__ slice_sys,
                                                         const int64_t* __restrict__ tile_begin,
-                                                        const int64_t* __restrict__ slot_begin) {
-    constexpr int RPW = WAVE / VW;       // rows per row group
-    constexpr int RW = 16;               // rows per wave (the tile is one 64-row slice)
-    constexpr int U = RW / RPW;
-    constexpr int TR = 64;
-    static_assert(VW >= 4 && RW % RPW == 0, "bad block configuration");
-    __shared__ int32_t s_idx[KMAX * TR];
-    __shared__ T s_val[KMAX * TR];
+                                                        const int64_t* __restrict__ slot_begin,
+                                                        const int64_t* __restrict__ slot_index, int64_t nslices,
+                                                        int walk_chunk) {
+    constexpr int RPW = WAVE / VW;  // rows per row group
+    constexpr int U = VW;           // row groups per slice
+    constexpr int UCM = sizeof(T) <= 4 ? 8 : sizeof(T) <= 8 ? 4 : 2;
+    constexpr int UC = U < UCM ? U : UCM;  // row groups per pass
+    static_assert(VW >= 2 && VW <= WAVE && U % UC == 0, "bad batched configuration");
+    using V = typename traits<T>::value;
     const int lane = threadIdx.x % WAVE;
-    const int wave = threadIdx.x / WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const int rsub = lane / VW;
     const int col = lane % VW;
     const int32_t* __restrict__ sidx = a.indices;
-    const T* __restrict__ sval = a.vals;
+    const V* __restrict__ sval = a.vals;
     const T* __restrict__ x = a.x;
-    const int64_t t0 = tile_begin[blockIdx.x], t1 = tile_begin[blockIdx.x + 1];
-    if (t0 >= t1) return;  // uniform: the whole workgroup leaves
-    int64_t slot = slot_begin[blockIdx.x];
-    int cur = slice_sys[t0];
+    const int walker = wave * (int)gridDim.x + (int)blockIdx.x;
+    BatchWalk walk(nslices, WAVES_PER_BLOCK * (int)gridDim.x, walker, walk_chunk, tile_begin);
+    if (!walk.valid()) return;  // uniform over the wave; the kernel has no barrier
+    int64_t slot = slot_begin[walker];
+    int cur = slice_sys[walk.s];
     double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
-    // software pipeline: the next slice's indices and values are loaded into registers while the
-    // current slice's gathers are in flight (as in k_step_sellb)
-    constexpr int E = (KMAX * TR + BLOCK - 1) / BLOCK;
-    int32_t p_idx[E];
-    T p_val[E];
-    auto prefetch = [&](int64_t slc, int64_t& pbase, int& pwidth) {
-        pbase = slice_ptr[slc];
-        pwidth = slice_width[slc];
-        const int kk = pwidth < KMAX ? pwidth : KMAX;
+    int32_t p_idx[KMAX];
+    V p_val[KMAX];
+    int64_t next_base = 0;
+    int next_width = 0, next_sys = 0;
+    auto prefetch = [&](int64_t slc) {
+        next_base = slice_ptr[slc];
+        next_width = slice_width[slc];
+        next_sys = slice_sys[slc];
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int e = threadIdx.x + q * BLOCK;
-            const int k = e / TR, r = e % TR;
-            if (e < KMAX * TR && k < kk) {
-                p_idx[q] = sidx[pbase + (int64_t)k * WAVE + r];
-                p_val[q] = sval[pbase + (int64_t)k * WAVE + r];
+        for (int k = 0; k < KMAX; ++k) {
+            if (k < next_width) {
+                p_idx[k] = sidx[next_base + (int64_t)k * WAVE + lane];
+                p_val[k] = sval[next_base + (int64_t)k * WAVE + lane];
             } else {
-                p_idx[q] = (int32_t)(slc * WAVE + r);
-                p_val[q] = zero(T{});
+                p_idx[k] = (int32_t)(slc * WAVE + lane);
+                p_val[k] = zero(V{});
             }
         }
     };
@@ -599,77 +639,98 @@ __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int
             return x[(int64_t)c * VW + col];
         }
     };
-    int64_t next_base = 0;
-    int next_width = 0;
-    prefetch(t0, next_base, next_width);
-    for (int64_t slice = t0; slice < t1; ++slice) {
-        const int sys = slice_sys[slice];
+    prefetch(walk.s);
+    while (walk.valid()) {
+        const int64_t slice = walk.s;
+        walk.next();
+        const int sys = next_sys;
         if (sys != cur) {
-            block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials, slot);
+            wave_diag_store<T, VW>(a_re, a_im, b_re, b_im, a.partials, slot_index[slot], lane);
             ++slot;
             cur = sys;
             a_re = a_im = b_re = b_im = 0;
         }
         const int64_t base = next_base;
         const int width = next_width;
-        __syncthreads();
+        int32_t c_idx[KMAX];
+        V c_val[KMAX];
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int e = threadIdx.x + q * BLOCK;
-            if (e < KMAX * TR) {
-                s_idx[e] = p_idx[q];
-                s_val[e] = p_val[q];
-            }
-        }
-        __syncthreads();
-        int rin[U];
-        int64_t row[U];
-        T yv[U], xo[U], xg[U][KMAX];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            rin[u] = wave * RW + u * RPW + rsub;
-            row[u] = slice * WAVE + rin[u];
-            const int64_t e = row[u] * VW + col;
-            yv[u] = a.sub_y ? nt_load(&a.yin[e]) : zero(T{});
-            xo[u] = x[e];
+        for (int k = 0; k < KMAX; ++k) {
+            c_idx[k] = p_idx[k];
+            c_val[k] = p_val[k];
         }
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k)
+        for (int u0 = 0; u0 < U; u0 += UC) {
+            int rin[UC];
+            int64_t e[UC];
+            T yv[UC], xo[UC], xg[UC][KMAX];
 #pragma unroll
-            for (int u = 0; u < U; ++u) xg[u][k] = gather(s_idx[k * TR + rin[u]]);
-        if (slice + 1 < t1) prefetch(slice + 1, next_base, next_width);
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            T acc = zero(T{});
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) acc = fmac(s_val[k * TR + rin[u]], xg[u][k], acc);
-            for (int k = KMAX; k < width; ++k) {
-                const int64_t off = base + (int64_t)k * WAVE + rin[u];
-                acc = fmac(sval[off], x[(int64_t)sidx[off] * VW + col], acc);
+            for (int u = 0; u < UC; ++u) {
+                rin[u] = (u0 + u) * RPW + rsub;
+                e[u] = (slice * WAVE + rin[u]) * VW + col;
+                yv[u] = a.sub_y ? nt_load(&a.yin[e[u]]) : zero(T{});
+                xo[u] = x[e[u]];
             }
-            const int64_t e = row[u] * VW + col;
-            T ynew = scal(acc, a.hscale);
-            if (a.sub_y) ynew = sub(ynew, yv[u]);
-            nt_store(ynew, &a.yout[e]);
-            a_re += sqabs(xo[u]);
-            const d2 p = cdot(ynew, xo[u]);
-            b_re += p.re;
-            b_im += p.im;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+                for (int u = 0; u < UC; ++u) xg[u][k] = gather(__shfl(c_idx[k], rin[u], WAVE));
+            // the next slice's entries go out while the first pass's gathers are in flight
+            if (u0 == 0 && walk.valid()) prefetch(walk.s);
+#pragma unroll
+            for (int u = 0; u < UC; ++u) {
+                T acc = zero(T{});
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) acc = fmac(shfl_any(c_val[k], rin[u]), xg[u][k], acc);
+                for (int k = KMAX; k < width; ++k) {
+                    const int64_t off = base + (int64_t)k * WAVE + rin[u];
+                    acc = fmac(sval[off], x[(int64_t)sidx[off] * VW + col], acc);
+                }
+                T ynew = scal(acc, a.hscale);
+                if (a.sub_y) ynew = sub(ynew, yv[u]);
+                nt_store(ynew, &a.yout[e[u]]);
+                add_diag_sums(ynew, xo[u], a_re, a_im, b_re, b_im);
+            }
         }
     }
-    block_reduce_store<VW>(a_re, a_im, b_re, b_im, a.partials, slot);
+    wave_diag_store<T, VW>(a_re, a_im, b_re, b_im, a.partials, slot_index[slot], lane);
 }
 
-// sums[s][j] = sum over the slots [slot_first[s], slot_first[s+1]) of partials[slot][j], in order.
-__global__ void k_reduce_segments(const double* partials, const int64_t* slot_first, int64_t num_sys, int width,
-                                  double* sums) {
-    const int64_t total = num_sys * width;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t sy = e / width;
-        const int j = (int)(e % width);
+// sums[s][j] = sum over the slots [slot_first[s], slot_first[s+1]) of partials[slot][j]: one
+// workgroup of RS_THREADS per system; thread t sums column j = t % width of the slots g, g + G,
+// g + 2G, ... (g = t / width, G = RS_THREADS / width groups, loads unrolled four deep), then the
+// group sums are added by a fixed pairwise tree over g, so the result is deterministic.
+constexpr int RS_THREADS = 1024;
+__global__ __launch_bounds__(RS_THREADS) void k_reduce_segments(const double* __restrict__ partials,
+                                                                const int64_t* __restrict__ slot_first,
+                                                                int64_t num_sys, int width, double* __restrict__ sums) {
+    __shared__ double part[RS_THREADS];
+    const int groups = RS_THREADS / width;
+    const int t = threadIdx.x;
+    const int g = t / width, j = t % width;
+    for (int64_t sy = blockIdx.x; sy < num_sys; sy += gridDim.x) {
+        const int64_t p0 = slot_first[sy], p1 = slot_first[sy + 1];
         double acc = 0;
-        for (int64_t p = slot_first[sy]; p < slot_first[sy + 1]; ++p) acc += partials[p * width + j];
-        sums[e] = acc;
+        int64_t p = p0 + g;
+        for (; p + 3 * groups < p1; p += 4 * groups) {
+            const double v0 = partials[p * width + j];
+            const double v1 = partials[(p + groups) * width + j];
+            const double v2 = partials[(p + 2 * groups) * width + j];
+            const double v3 = partials[(p + 3 * groups) * width + j];
+            acc += v0;
+            acc += v1;
+            acc += v2;
+            acc += v3;
+        }
+        for (; p < p1; p += groups) acc += partials[p * width + j];
+        part[t] = acc;
+        __syncthreads();
+        for (int half = groups / 2; half >= 1; half /= 2) {
+            if (g < half) part[t] += part[t + half * width];
+            __syncthreads();
+        }
+        if (t < width) sums[sy * width + t] = part[t];
+        __syncthreads();
     }
 }
 
@@ -1490,10 +1551,11 @@ void launch_lanczos_axpy(int scalar, const void* v1, void* v0, int64_t n, double
 void launch_scale(int scalar, void* v, int64_t n, double f, hipStream_t s) { KPMX_DISPATCH(scalar, scale_t, v, n, f, s) }
 
 template <class T, int VW>
-static void batched_t(const BatchedLaunch& B, hipStream_t stream) {
+static void batched_t(const BatchedLaunch& B, hipStream_t stream, int* occupancy) {
+    // VW lanes of type T per row: fp64 blocks run as vw / 2 column pairs (16-byte lanes)
     StepArgs<T> a{};
     a.indices = B.indices;
-    a.vals = static_cast<const T*>(B.vals);
+    a.vals = static_cast<const typename traits<T>::value*>(B.vals);
     a.x = static_cast<const T*>(B.x);
     a.yin = static_cast<const T*>(B.yin);
     a.yout = static_cast<T*>(B.yout);
@@ -1503,8 +1565,12 @@ static void batched_t(const BatchedLaunch& B, hipStream_t stream) {
     a.sub_y = B.sub_y;
     const bool off32 = (uint64_t)B.n * VW * sizeof(T) < ((uint64_t)1 << 32);
     auto go = [&](auto kernel) {
+        if (occupancy) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(occupancy, kernel, BLOCK, 0) != hipSuccess) *occupancy = 1;
+            return;
+        }
         hipLaunchKernelGGL(kernel, dim3(B.num_wg), dim3(BLOCK), 0, stream, a, B.slice_ptr, B.slice_width, B.slice_sys,
-                           B.tile_begin, B.slot_begin);
+                           B.tile_begin, B.slot_begin, B.slot_index, B.nslices, B.chunk);
     };
     if (B.max_width <= 4) {
         if (off32) go(k_step_batched<T, VW, 4, true>);
@@ -1519,33 +1585,51 @@ static void batched_t(const BatchedLaunch& B, hipStream_t stream) {
 }
 
 template <class T>
-static int batched_vw(const BatchedLaunch& B, hipStream_t stream) {
+static int batched_vw(const BatchedLaunch& B, hipStream_t stream, int* occupancy) {
     switch (B.vw) {
-        case 4: batched_t<T, 4>(B, stream); return 0;
-        case 8: batched_t<T, 8>(B, stream); return 0;
-        case 16: batched_t<T, 16>(B, stream); return 0;
-        case 32: batched_t<T, 32>(B, stream); return 0;
-        case 64: batched_t<T, 64>(B, stream); return 0;
+        case 4: batched_t<T, 4>(B, stream, occupancy); return 0;
+        case 8: batched_t<T, 8>(B, stream, occupancy); return 0;
+        case 16: batched_t<T, 16>(B, stream, occupancy); return 0;
+        case 32: batched_t<T, 32>(B, stream, occupancy); return 0;
+        case 64: batched_t<T, 64>(B, stream, occupancy); return 0;
+        default: return -1;
+    }
+}
+
+static int batched_dispatch(const BatchedLaunch& B, hipStream_t stream, int* occupancy) {
+    switch (B.scalar) {
+        case 0: return batched_vw<float>(B, stream, occupancy);
+        case 1:
+            switch (B.vw) {  // column pairs
+                case 4: batched_t<dpair, 2>(B, stream, occupancy); return 0;
+                case 8: batched_t<dpair, 4>(B, stream, occupancy); return 0;
+                case 16: batched_t<dpair, 8>(B, stream, occupancy); return 0;
+                case 32: batched_t<dpair, 16>(B, stream, occupancy); return 0;
+                case 64: batched_t<dpair, 32>(B, stream, occupancy); return 0;
+                default: return -1;
+            }
+        case 2: return batched_vw<cf32>(B, stream, occupancy);
+        case 3: return batched_vw<cf64>(B, stream, occupancy);
         default: return -1;
     }
 }
 
 int launch_batched_step(const BatchedLaunch& B, hipStream_t stream) {
-    switch (B.scalar) {
-        case 0: return batched_vw<float>(B, stream);
-        case 1: return batched_vw<double>(B, stream);
-        case 2: return batched_vw<cf32>(B, stream);
-        case 3: return batched_vw<cf64>(B, stream);
-        default: return -1;
-    }
+    if (B.chunk > 0 && (B.num_wg < 8 || B.num_wg % 8 != 0)) return -1;  // the XCD walk needs 8 | grid
+    return batched_dispatch(B, stream, nullptr);
+}
+
+int batched_blocks_per_cu(const BatchedLaunch& B) {
+    int per_cu = 0;
+    if (batched_dispatch(B, nullptr, &per_cu) != 0 || per_cu < 1) return 1;
+    return per_cu;
 }
 
 void launch_reduce_segments(const double* partials, const int64_t* slot_first, int64_t num_sys, int width, double* sums,
                             hipStream_t stream) {
-    int64_t g = (num_sys * width + 255) / 256;
-    if (g < 1) g = 1;
-    if (g > 65536) g = 65536;
-    hipLaunchKernelGGL(k_reduce_segments, dim3((unsigned)g), dim3(256), 0, stream, partials, slot_first, num_sys, width, sums);
+    const int64_t g = num_sys < 65536 ? num_sys : 65536;  // width (16 ... 256) divides RS_THREADS
+    hipLaunchKernelGGL(k_reduce_segments, dim3((unsigned)g), dim3(RS_THREADS), 0, stream, partials, slot_first, num_sys,
+                       width, sums);
 }
 
 // Rows per wave per adjoint tile: 8 (two or more row groups in flight) up to 16 vectors, 4 above;

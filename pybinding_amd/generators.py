@@ -99,6 +99,12 @@ def square_flake(l1, l2, t=-1.0, disorder=0.0, seed=None, dtype=np.float64, rng=
     return _assemble(n, rows, cols, vals, onsite, dtype)
 
 
+def square_positions(l1, l2):
+    """Positions (x, y) of the sites of `square_flake(l1, l2)` (lattice constant 1)."""
+    i = np.arange(l1 * l2)
+    return (i % l1).astype(np.float64), (i // l1).astype(np.float64)
+
+
 def honeycomb_flake(width, height, t=T_GRAPHENE, dtype=np.float64, return_positions=False):
     """Graphene flake: sites of the monolayer lattice (a1 = [a, 0], a2 = [a/2, a sqrt(3)/2],
     A at [0, -a_cc/2], B at [0, a_cc/2]) inside the centred `width` x `height` rectangle, dangling

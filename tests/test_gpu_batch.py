@@ -58,18 +58,18 @@ def test_batched_rejects_coupled_systems():
 
 
 def test_batched_autotune_keeps_results(oracle):
-    """The autotuner's partitions (256, 512 and 523 workgroups over 523 slices) change only how
-    each system's sums are grouped: every system still matches the oracle, and the tuned run agrees
-    with the default partition to rounding."""
+    """The autotuner's partitions (contiguous slice ranges over 256 ... 16384 workgroups, the
+    XCD-aware walk) change only how each system's sums are grouped: every system still matches the
+    oracle, and the tuned run agrees with the default partition to rounding."""
     hs, ranges = systems(np.float64, count=40, seed=8)
     kb = kpm_batch(hs, ranges)
     M = 66
     before = kb.moments(M, 4)
-    wg = kb.autotune(4)
+    wg, chunk = kb.autotune(4)
     after = kb.moments(M, 4)
     again = kb.moments(M, 4, use_graph=False)
     kb.close()
-    assert wg >= 1
+    assert wg >= 1 and chunk >= 0 and (chunk == 0 or wg % 8 == 0)
     np.testing.assert_array_equal(after, again)
     assert np.max(np.abs(after - before) / np.abs(before[:, :, :1])) <= 1e-13
     for s, (h, (lo, hi)) in enumerate(zip(hs, ranges)):
@@ -77,3 +77,96 @@ def test_batched_autotune_keeps_results(oracle):
         h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
         ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], 4, np.float64), M)
         assert np.max(np.abs(after[s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
+
+
+@pytest.mark.parametrize("nvec", [4, 8])
+def test_batched_partitions_agree(oracle, nvec):
+    """Every slice partition -- contiguous ranges, and the XCD-aware walk with runs of 1, 2 and 5
+    slices over grids with fewer, about as many and more workgroups than slices per region --
+    visits every slice once: each system matches the oracle and the partitions agree to rounding
+    (the grouping of each system's sums is all that changes)."""
+    hs, ranges = systems(np.float64, count=30, seed=11)
+    kb = kpm_batch(hs, ranges)
+    nslices = sum((h.shape[0] + 63) // 64 for h in hs)
+    M = 40
+    res = {}
+    try:
+        for wg, chunk in [(1, 0), (7, 0), (nslices, 0), (8, 1), (16, 2), (64, 1), (64, 5),
+                          (8 * (nslices // 8 + 3), 1), (256, 2)]:
+            kb.set_partition(wg, chunk)
+            res[wg, chunk] = kb.moments(M, nvec, use_graph=(chunk % 2 == 1))
+    finally:
+        kb.close()
+    base = res[1, 0]
+    for key, mu in res.items():
+        assert np.max(np.abs(mu - base) / np.abs(base[:, :, :1])) <= 1e-13, key
+    for s, (h, (lo, hi)) in enumerate(zip(hs, ranges)):
+        a, b = oracle.scale_factors(lo, hi)
+        h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
+        ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], nvec, np.float64), M)
+        assert np.max(np.abs(res[64, 1][s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
+
+
+def test_batched_partition_rejects_bad_walk():
+    hs, ranges = systems(np.float64, count=3, seed=2)
+    kb = kpm_batch(hs, ranges)
+    try:
+        with pytest.raises(RuntimeError, match="multiple of 8"):
+            kb.set_partition(12, 1)
+        with pytest.raises(RuntimeError, match="invalid partition"):
+            kb.set_partition(0, 0)
+    finally:
+        kb.close()
+
+
+def test_batched_hilbert_order_matches_oracle(oracle):
+    """Systems with site positions run in the Hilbert order of their sites (starters drawn in the
+    original order and moved along): every system still matches the oracle on its unpermuted
+    Hamiltonian, and Models given directly are ordered the same way."""
+    rng = np.random.default_rng(21)
+    hs, ranges, pos = [], [], []
+    for _ in range(9):
+        l1, l2 = int(rng.integers(9, 70)), int(rng.integers(5, 40))
+        W = float(rng.uniform(0, 3))
+        hs.append(generators.square_flake(l1, l2, t=-1.0, disorder=W, rng=rng))
+        ranges.append((-(4 + W / 2), 4 + W / 2))
+        pos.append(generators.square_positions(l1, l2))
+    M = 98
+    kb = kpm_batch(hs, ranges, positions=pos)
+    try:
+        assert all(q is not None for q in kb.orders)
+        assert any(np.any(q != np.arange(q.size)) for q in kb.orders)
+        mu = kb.moments(M, 4)
+    finally:
+        kb.close()
+    given = kpm_batch(hs, ranges, site_order=None)
+    try:
+        mu_given = given.moments(M, 4)
+    finally:
+        given.close()
+    assert np.max(np.abs(mu - mu_given) / np.abs(mu_given[:, :, :1])) <= 1e-12
+    for s, (h, (lo, hi)) in enumerate(zip(hs, ranges)):
+        a, b = oracle.scale_factors(lo, hi)
+        h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
+        ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], 4, np.float64), M)
+        assert np.max(np.abs(mu[s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
+
+
+def test_batched_models(oracle):
+    """Models in the batch: their Hamiltonians (graphene flakes of two sizes) in the Hilbert order
+    of their site positions, against the oracle."""
+    import pybinding_amd as pb
+    from pybinding_amd.repository import graphene
+    models = [pb.Model(graphene.monolayer(), pb.rectangle(w), pb.force_double_precision()) for w in (2.0, 3.5)]
+    ranges = [(-9, 9), (-9.5, 9.5)]
+    kb = kpm_batch(models, ranges)
+    try:
+        mu = kb.moments(64, 2)
+    finally:
+        kb.close()
+    for s, (m, (lo, hi)) in enumerate(zip(models, ranges)):
+        h = m.hamiltonian
+        a, b = oracle.scale_factors(lo, hi)
+        h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
+        ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], 2, np.float64), 64)
+        assert np.max(np.abs(mu[s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
