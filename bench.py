@@ -75,6 +75,21 @@ def step_bytes(n, nnz, vw, s):
     return nnz * (s + 4) + 4 * (n + 1) + 3 * n * vw * s
 
 
+def launch_widths(R, tile_width):
+    """Vector widths of the step launches of one recurrence step over R vectors: starter tiles of
+    up to 64 vectors (rounded up to a power of two), each run as sub-tiles of `tile_width`
+    columns when wider (kpmx_ctx_set_tile_width)."""
+    out = []
+    for c0 in range(0, R, 64):
+        cols = min(64, R - c0)
+        vw = 1 << (cols - 1).bit_length()
+        if vw <= tile_width:
+            out.append(vw)
+        else:
+            out += [tile_width] * ((cols + tile_width - 1) // tile_width)
+    return out
+
+
 def periodic_graphene_model(n1, n2, disorder, seed, field=None):
     """The periodic honeycomb supercell through the model builder: graphene.monolayer() on a
     primitive n1 x n2 block with translational symmetry of the whole block (a complex128
@@ -488,6 +503,7 @@ def fp32_variant(cells, M, R, fmt, order):
     n, nnz = h2.shape[0], h2.nnz
     r0 = starters(native, n, R, dtype, rmap)
     ctx = abi.Ctx(0)
+    ctx_tw = ctx.tile_width(dtype)
     mat = ctx.mat(h2, fmt=1 if fmt == "SELL" else 0)
     blk = ctx.block(r0)
     ctx.synchronize()
@@ -502,10 +518,11 @@ def fp32_variant(cells, M, R, fmt, order):
     blk.close()
     mat.close()
     ctx.close()
+    widths = launch_widths(R, ctx_tw)
     step_ms = ms / (M // 2)
-    vw = min(64, 1 << (max(R, 1) - 1).bit_length())
     out = {"value": nnz * M * R / (ms / 1000) / 1e9, "unit": "GNNZ*moments/s", "ms_per_computation": ms,
-           "step_ms": step_ms, "achieved_GBps": step_bytes(n, nnz, vw, 4) / (step_ms / 1000) / 1e9,
+           "step_ms": step_ms, "launch_widths": widths,
+           "achieved_GBps": sum(step_bytes(n, nnz, w, 4) for w in widths) / (step_ms / 1000) / 1e9,
            "kernel_variant": variant}
     log("fp32 variant: %.1f ms per moment computation, %.4f ms per step, %.0f GB/s algorithmic"
         % (ms, step_ms, out["achieved_GBps"]))
@@ -589,11 +606,12 @@ def main():
     t_draw = time.time() - t_draw
 
     ctx = abi.Ctx(0)
+    tile_width = ctx.tile_width(dtype)
     mat = ctx.mat(h2, fmt=1 if args.format == "SELL" else 0)
     blk = ctx.block(r0)
     ctx.synchronize()
     variant = abi.autotune_diagonal(mat, blk)
-    log("autotuned step kernel variant: %d" % variant)
+    log("autotuned step kernel variant: %d (column tiles of %d vectors)" % (variant, tile_width))
 
     for _ in range(args.warmup):
         abi.moments_diagonal_run(mat, blk, M)
@@ -612,17 +630,27 @@ def main():
     ms_per_step = wall * 1000 / args.steps
     value = nnz * M * R / (wall / args.steps) / 1e9
 
-    launches = M // 2  # fused step launches per moment computation (one per two moments)
-    tiles = (R + 63) // 64
-    vw = min(64, 1 << (max(R, 1) - 1).bit_length())
-    step_ms = gpu_ms / (args.steps * launches * tiles)
-    bytes_per_launch = step_bytes(n, nnz, vw, s)
+    # fused step launches per moment computation: one per two moments and column tile
+    widths = launch_widths(R, tile_width)
+    steps_per_run = M // 2
+    step_ms = gpu_ms / (args.steps * steps_per_run * len(widths))      # one launch
+    bytes_per_launch = sum(step_bytes(n, nnz, w, s) for w in widths) / len(widths)
     achieved = bytes_per_launch / (step_ms / 1000)
-    log("GPU: %.1f ms per moment computation, %.4f ms per step launch, %.1f GB/s algorithmic"
-        % (gpu_ms / args.steps, step_ms, achieved / 1e9))
+    # the same step over all R vectors as one pass (the matrix read once): the algorithmic bytes of
+    # the whole step over the time of its launches
+    pass_ms = gpu_ms / (args.steps * steps_per_run)
+    vw_all = min(64, 1 << (max(R, 1) - 1).bit_length())
+    pass_gbps = step_bytes(n, nnz, vw_all, s) * ((R + 63) // 64) / (pass_ms / 1000) / 1e9
+    log("GPU: %.1f ms per moment computation, %d launches of %s vectors per step, %.4f ms per launch, "
+        "%.1f GB/s algorithmic per launch; %.4f ms per step over all %d vectors = %.1f GB/s of one-pass bytes"
+        % (gpu_ms / args.steps, len(widths), "/".join(str(w) for w in sorted(set(widths))), step_ms,
+           achieved / 1e9, pass_ms, R, pass_gbps))
 
-    traffic_key = "%dx%d %s v%d %s %s" % (args.cells[0], args.cells[1], dtype.name, R, args.format, args.order)
-    extra = {"starter_draw_s": t_draw}
+    traffic_key = "%dx%d %s v%d launch%d %s %s" % (args.cells[0], args.cells[1], dtype.name, R, max(widths), args.format,
+                                                   args.order)
+    extra = {"starter_draw_s": t_draw,
+             "one_pass_equivalent": {"ms_per_step": pass_ms, "achieved_GBps": pass_gbps,
+                                     "frac": pass_gbps * 1e9 / HBM_PEAK, "launch_widths": widths}}
     cg, rg, tg = ctx.stream_probe(1 << 31)
     step_traffic_gbps = (counted_traffic(traffic_key) or float("nan")) / (step_ms / 1000) / 1e9
     extra["stream_probe"] = {"copy_GBps": cg, "read_GBps": rg, "triad_GBps": tg,
@@ -691,9 +719,10 @@ def main():
         "dtype": "f64" if dtype == np.float64 else "f32",
         "data": "synthetic (seeded disorder, reference mt19937 starters)",
         "config": {"workload": "periodic honeycomb %dx%d cells (N=%d, nnz=%d), Anderson W=1 eV, %d moments, "
-                               "%d random vectors as one block SpMM" % (args.cells[0], args.cells[1], n, nnz, M, R),
+                               "%d random vectors as one starter block (run in column tiles of %d)"
+                               % (args.cells[0], args.cells[1], n, nnz, M, R, tile_width),
                    "matrix_format": args.format, "site_order": args.order, "kernel_variant": variant,
-                   "moments": M, "vectors": R},
+                   "moments": M, "vectors": R, "column_tile": tile_width},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": counted_traffic(traffic_key)},
         "cpu_baseline": cpu,

@@ -134,6 +134,15 @@ int kpmx_ctx_set_step_variant(kpmx_ctx* ctx, int scalar, int vector_width, int v
  * twice the resident grid, fp64 blocks of 8 or more vectors as column pairs). */
 int kpmx_ctx_set_adjoint_variant(kpmx_ctx* ctx, int variant);
 
+/* Widest column tile of the diagonal recurrence for one scalar type (1, 2, 4, ..., 64; no
+ * counterpart in the reference).  A starter tile wider than this runs as several recurrences over
+ * sub-tiles of this many columns (each copied out of the tile first); narrower tiles move fewer
+ * bytes per gathered row, so one XCD's L2 holds the rows of a wider band, at the price of one
+ * matrix read per sub-tile and step.  The moments are the same up to rounding.  Defaults: 16 for
+ * fp64, 64 for the others.  get: the current width. */
+int kpmx_ctx_set_tile_width(kpmx_ctx* ctx, int scalar, int width);
+int kpmx_ctx_get_tile_width(kpmx_ctx* ctx, int scalar, int* width);
+
 /* Graph policy (no counterpart in the reference, whose CPU loop has no launch cost): with
  * KPMX_GRAPH_ALWAYS the diagonal recurrence of a starter block is captured into a HIP graph on its
  * first run (KPMX_GRAPH_AUTO: on its second run) and replayed by later runs with the same matrix,
@@ -152,10 +161,10 @@ int kpmx_moments_diagonal_fetch(const kpmx_mat* h2, const kpmx_block* r0, int64_
  * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
  * reference.  Same inputs, output and kernels as kpmx_moments_diagonal, but the recurrence's work
  * vectors, partial and raw sums and the collected moments live in a static device array of
- * 64 MiB per GPU (held by the process until it ends): 2 x n x (vector width) scalars, plus
- * (num_moments / 2) x 4 x (vector width) doubles of sums, plus num_vectors x num_moments
- * scalars, plus one step's partial sums, must fit, otherwise KPMX_EINVAL with the sizes in the
- * message.  No device memory is allocated or freed (h2 and r0 are uploaded beforehand).  Calls
+ * 64 MiB per GPU (held by the process until it ends): 2 x n x (vector width) scalars (3 x when
+ * the block is run as column sub-tiles, see kpmx_ctx_set_tile_width), plus (num_moments / 2) x
+ * 4 x (vector width) doubles of sums, plus num_vectors x num_moments scalars, plus one step's
+ * partial sums, must fit, otherwise KPMX_EINVAL with the sizes in the message.  No device memory is allocated or freed (h2 and r0 are uploaded beforehand).  Calls
  * that overlap on one GPU share the array and race. */
 int kpmx_moments_diagonal_noalloc(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu);
 

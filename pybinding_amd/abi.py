@@ -116,6 +116,16 @@ class Ctx(_Handle):
         """Pin the adjoint step kernel's variant (-1: the default)."""
         _check(lib().kpmx_ctx_set_adjoint_variant(self.h, ctypes.c_int(variant)))
 
+    def set_tile_width(self, dtype, width):
+        """Widest column tile the diagonal recurrence runs for one scalar type (wider starter tiles
+        run as sub-tiles of this many columns)."""
+        _check(lib().kpmx_ctx_set_tile_width(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.c_int(width)))
+
+    def tile_width(self, dtype):
+        w = ctypes.c_int()
+        _check(lib().kpmx_ctx_get_tile_width(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.byref(w)))
+        return w.value
+
     def set_graph_policy(self, policy):
         """Graph policy of the diagonal recurrence: 0 never (default), 1 auto (small starter blocks,
         captured on the second identical run), 2 always."""

@@ -132,9 +132,12 @@ struct kpmx_ctx {
     int num_cu = 256;
     char* scratch = nullptr;  // this device's `scratch` array (device address)
     hipStream_t stream = nullptr;
-    Workspace partials, sums, work0, work1, aux0, aux1, aux2, panel_l, panel_r;
+    Workspace partials, sums, work0, work1, work2, aux0, aux1, aux2, panel_l, panel_r;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int tune[4][7] = {};  // tile-shape variant per (scalar, log2 vw), chosen by kpmx_autotune_diagonal
+    // widest column tile the diagonal recurrence runs per scalar type (kpmx_ctx_set_tile_width):
+    // wider starter tiles are run as several recurrences over column sub-tiles
+    int tile_cap[4] = {64, 64, 64, 64};
     int graph_policy = KPMX_GRAPH_NEVER;
     int adjoint_variant = -1;  // adjoint step variant (kpmx_ctx_set_adjoint_variant); -1: the default
     // device memory of the largest finished adjoint tape, reused by the next tape that fits: a
@@ -192,9 +195,10 @@ struct kpmx_block {
     int scalar = 1;
     int64_t n = 0, nvec = 0;
     std::vector<Tile> tiles;
-    // raw sums of the last kpmx_moments_diagonal_run, per tile: (M/2) x (4 vw) doubles
+    // raw sums of the last kpmx_moments_diagonal_run, per run tile: (M/2) x (4 vw) doubles
     std::vector<void*> sums;
     int64_t sums_moments = 0;
+    int sums_cap = 0;  // column tile cap the sums were run with
     // captured diagonal recurrence and the launch state it was captured with
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -466,9 +470,56 @@ void check_pair(const kpmx_mat* h2, const kpmx_block* r0) {
     KPMX_REQUIRE(h2->rows == r0->n, "Size mismatch between the matrix and the starter vectors");
 }
 
-void diag_enqueue(const kpmx_mat* h2, const kpmx_block* r0, int64_t steps) {
+// The column tiles the diagonal recurrence runs over: every starter tile no wider than `cap`
+// as it is, wider ones as sub-tiles of `cap` columns, each copied out of its tile (into the
+// context's work2 buffer) right before its own recurrence.  Narrower column tiles keep the
+// gathered rows of one XCD's band of rows in its L2 and move less per row; the matrix is read
+// once per sub-tile and step.
+struct RunTile {
+    size_t tile = 0;    // starter tile
+    int c = 0;          // first column inside it
+    bool copy = false;  // sub-tile: columns [c, c + sub.vw) copied to work2 first
+    Tile sub;
+};
+std::vector<RunTile> run_tiles(const kpmx_block* r0, int cap) {
+    std::vector<RunTile> out;
     for (size_t i = 0; i < r0->tiles.size(); ++i) {
-        run_recurrence(h2, r0->tiles[i], MODE_DIAG, steps, nullptr, static_cast<double*>(r0->sums[i]),
+        const Tile& t = r0->tiles[i];
+        if (t.vw <= cap) {
+            out.push_back({i, 0, false, t});
+            continue;
+        }
+        for (int c = 0; c < t.cols; c += cap) {
+            RunTile rt{i, c, true, t};
+            rt.sub.c0 = t.c0 + c;
+            rt.sub.cols = std::min<int64_t>(cap, t.cols - c);
+            rt.sub.vw = cap;
+            rt.sub.data = nullptr;  // work2, set when enqueued
+            out.push_back(rt);
+        }
+    }
+    return out;
+}
+int diag_cap(const kpmx_mat* h2) { return h2->ctx->tile_cap[h2->scalar]; }
+
+// copies a sub-tile's columns into `dst` (default: work2, sized beforehand) and returns the tile
+// to run
+Tile stage_run_tile(const kpmx_mat* h2, const kpmx_block* r0, const RunTile& rt, void* dst = nullptr) {
+    if (!rt.copy) return rt.sub;
+    kpmx_ctx* ctx = h2->ctx;
+    const Tile& t = r0->tiles[rt.tile];
+    Tile sub = rt.sub;
+    sub.data = dst ? dst : ctx->work2.get((size_t)h2->rows * sub.vw * scalar_bytes(h2->scalar));
+    launch_copy_cols(h2->scalar, t.data, h2->rows, t.vw, rt.c, sub.vw, sub.data, ctx->stream);
+    KPMX_HIP(hipGetLastError());
+    return sub;
+}
+
+void diag_enqueue(const kpmx_mat* h2, const kpmx_block* r0, int64_t steps) {
+    const auto rts = run_tiles(r0, r0->sums_cap);
+    for (size_t i = 0; i < rts.size(); ++i) {
+        const Tile sub = stage_run_tile(h2, r0, rts[i]);
+        run_recurrence(h2, sub, MODE_DIAG, steps, nullptr, static_cast<double*>(r0->sums[i]),
                        [](StepLaunch&, int64_t) {}, [](StepLaunch const&, int64_t) {});
     }
 }
@@ -489,13 +540,16 @@ void diag_run(const kpmx_mat* h2, const kpmx_block* r0c, int64_t M) {
     KPMX_REQUIRE(M >= 2 && M % 2 == 0, "num_moments must be even and >= 2");
     kpmx_ctx* ctx = h2->ctx;
     const int64_t steps = M / 2;
-    if (r0->sums_moments != M) {
+    const int cap = diag_cap(h2);
+    const auto rts = run_tiles(r0, cap);
+    if (r0->sums_moments != M || r0->sums_cap != cap) {
         if (!r0->sums.empty()) KPMX_HIP(hipStreamSynchronize(ctx->stream));
         r0->drop_graph();
         for (auto* s : r0->sums) dev_free(s);
         r0->sums.clear();
-        for (auto& t : r0->tiles) r0->sums.push_back(dev_alloc((size_t)steps * 4 * t.vw * sizeof(double)));
+        for (auto& rt : rts) r0->sums.push_back(dev_alloc((size_t)steps * 4 * rt.sub.vw * sizeof(double)));
         r0->sums_moments = M;
+        r0->sums_cap = cap;
     }
     if (!diag_graph_candidate(h2, r0)) {
         diag_enqueue(h2, r0, steps);
@@ -504,10 +558,12 @@ void diag_run(const kpmx_mat* h2, const kpmx_block* r0c, int64_t M) {
     // Size the shared workspaces for every tile before capturing (nothing is allocated while the
     // stream captures) and key the graph on everything its launches read.
     const size_t sb = scalar_bytes(h2->scalar);
-    size_t work = 0, part = 0;
-    std::vector<uint64_t> key = {h2->serial, (uint64_t)M};
-    for (const Tile& t : r0->tiles) {
+    size_t work = 0, part = 0, copy = 0;
+    std::vector<uint64_t> key = {h2->serial, (uint64_t)M, (uint64_t)cap};
+    for (const RunTile& rt : rts) {
+        const Tile& t = rt.sub;
         work = std::max(work, (size_t)h2->rows * t.vw * sb);
+        if (rt.copy) copy = std::max(copy, (size_t)h2->rows * t.vw * sb);
         StepLaunch L = base_launch(h2, t.vw, true);
         L.mode = MODE_DIAG;
         L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
@@ -519,6 +575,7 @@ void diag_run(const kpmx_mat* h2, const kpmx_block* r0c, int64_t M) {
     key.push_back((uint64_t)(uintptr_t)ctx->work0.get(work));
     key.push_back((uint64_t)(uintptr_t)ctx->work1.get(work));
     key.push_back((uint64_t)(uintptr_t)ctx->partials.get(part));
+    if (copy) key.push_back((uint64_t)(uintptr_t)ctx->work2.get(copy));
     if (ctx->graph_policy == KPMX_GRAPH_AUTO && r0->graph_key != key && r0->last_key != key) {
         // automatic mode captures on the second run of the same recurrence: a one-off run
         // does not pay for capture and instantiation
@@ -550,8 +607,9 @@ void diag_fetch(const kpmx_mat* h2, const kpmx_block* r0, int64_t M, void* mu) {
     kpmx_ctx* ctx = h2->ctx;
     const bool cplx = is_complex(h2->scalar);
     const int64_t steps = M / 2;
-    for (size_t i = 0; i < r0->tiles.size(); ++i) {
-        const Tile& t = r0->tiles[i];
+    const auto rts = run_tiles(r0, r0->sums_cap);
+    for (size_t i = 0; i < rts.size(); ++i) {
+        const Tile& t = rts[i].sub;
         std::vector<double> s((size_t)steps * 4 * t.vw);
         KPMX_HIP(hipMemcpyAsync(s.data(), r0->sums[i], s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         KPMX_HIP(hipStreamSynchronize(ctx->stream));
@@ -769,6 +827,10 @@ int kpmx_ctx_create(int device, kpmx_ctx** out) {
             ctx->tune[KPMX_F64][4] = pairs + 1;                     // 16 vectors: tile shape 1, resident grid
             ctx->tune[KPMX_F64][5] = pairs + 1;                     // 32 vectors
             ctx->tune[KPMX_F64][6] = pairs + 1 + NUM_TILE_SHAPES;   // 64 vectors: tile shape 1, half grid
+            // fp64 recurrences run in column tiles of at most 16 vectors: on the 4.19 M-site
+            // workload four 16-column recurrences take 1.25 ms per step of 64 vectors, one
+            // 64-column recurrence 1.29 ms
+            ctx->tile_cap[KPMX_F64] = 16;
         } catch (...) {
             delete ctx;
             throw;
@@ -942,6 +1004,23 @@ int kpmx_block_destroy(kpmx_block* b) {
     });
 }
 
+int kpmx_ctx_set_tile_width(kpmx_ctx* ctx, int scalar, int width) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        KPMX_REQUIRE(width >= 1 && width <= 64 && (width & (width - 1)) == 0, "tile width must be 1, 2, 4, ..., 64");
+        ctx->tile_cap[scalar] = width;
+    });
+}
+
+int kpmx_ctx_get_tile_width(kpmx_ctx* ctx, int scalar, int* width) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx && width, "null argument");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        *width = ctx->tile_cap[scalar];
+    });
+}
+
 int kpmx_ctx_set_graph_policy(kpmx_ctx* ctx, int policy) {
     return guarded([&] {
         KPMX_REQUIRE(ctx, "null context");
@@ -992,21 +1071,28 @@ int kpmx_moments_diagonal_noalloc(const kpmx_mat* h2, const kpmx_block* r0, int6
         const int64_t steps = M / 2;
         const size_t sb = scalar_bytes(h2->scalar);
         auto align = [](size_t b) { return (b + 255) / 256 * 256; };
+        // the same column tiles as kpmx_moments_diagonal (sub-tiles of wide starter tiles)
+        const auto rts = run_tiles(r0, diag_cap(h2));
         int vw_max = 1;
+        bool any_copy = false;
         size_t part_step = 0;
-        for (const Tile& t : r0->tiles) {
+        for (const RunTile& rt : rts) {
+            const Tile& t = rt.sub;
             vw_max = std::max(vw_max, t.vw);
+            any_copy = any_copy || rt.copy;
             StepLaunch L = base_launch(h2, t.vw, true);
             L.mode = MODE_DIAG;
             L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
             L.num_wg = step_grid(L);
             part_step = std::max(part_step, (size_t)L.num_wg * 4 * t.vw * sizeof(double));
         }
-        // layout: [W0 | W1 | raw sums (steps x 4 vw_max) | moments (nvec x M) | partial sums (rest)]
+        // layout: [W0 | W1 | sub-tile copy (if any) | raw sums (steps x 4 vw_max) | moments (nvec x M) |
+        //          partial sums (rest)]
         const size_t work = align((size_t)h2->rows * vw_max * sb);
+        const size_t copy = any_copy ? work : 0;
         const size_t sums = align((size_t)steps * 4 * vw_max * sizeof(double));
         const size_t out = align((size_t)r0->nvec * M * sb);
-        const size_t fixed_part = 2 * work + sums + out;
+        const size_t fixed_part = 2 * work + copy + sums + out;
         if (fixed_part + part_step > KPMX_SCRATCH_BYTES)
             throw Error{KPMX_EINVAL, "kpmx_moments_diagonal_noalloc: this matrix, block and moment count need " +
                                          std::to_string(fixed_part + part_step) + " bytes of device scratch; the limit is " +
@@ -1014,11 +1100,13 @@ int kpmx_moments_diagonal_noalloc(const kpmx_mat* h2, const kpmx_block* r0, int6
         FixedBuffers fb;
         fb.w0 = ctx->scratch;
         fb.w1 = ctx->scratch + work;
-        double* d_sums = reinterpret_cast<double*>(ctx->scratch + 2 * work);
-        char* d_mu = ctx->scratch + 2 * work + sums;
+        char* d_copy = ctx->scratch + 2 * work;
+        double* d_sums = reinterpret_cast<double*>(ctx->scratch + 2 * work + copy);
+        char* d_mu = ctx->scratch + 2 * work + copy + sums;
         fb.partials = reinterpret_cast<double*>(ctx->scratch + fixed_part);
         fb.partials_bytes = KPMX_SCRATCH_BYTES - fixed_part;
-        for (const Tile& t : r0->tiles) {
+        for (const RunTile& rt : rts) {
+            const Tile t = stage_run_tile(h2, r0, rt, d_copy);
             run_recurrence(h2, t, MODE_DIAG, steps, nullptr, d_sums, [](StepLaunch&, int64_t) {},
                            [](StepLaunch const&, int64_t) {}, -1, &fb);
             launch_diag_collect(h2->scalar, d_sums, t.vw, t.cols, M, t.c0, d_mu, ctx->stream);
@@ -1498,7 +1586,8 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
         check_pair(h2, r0);
         kpmx_ctx* ctx = h2->ctx;
         KPMX_HIP(hipSetDevice(ctx->device));
-        const Tile& t = r0->tiles.front();
+        // the first column tile the diagonal recurrence runs (a sub-tile of a wider starter tile)
+        const Tile t = stage_run_tile(h2, r0, run_tiles(r0, diag_cap(h2)).front());
         // About 2 GB of vector traffic per timed sample (at least 9 steps, at most 255): small
         // systems get enough steps that launch jitter does not decide the choice.
         const double step_bytes = 3.0 * (double)h2->rows * t.vw * scalar_bytes(h2->scalar);
@@ -1528,8 +1617,7 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
                 best = v;
             }
         }
-        for (const Tile& tt : r0->tiles)
-            if (tt.vw == t.vw) ctx->tune[h2->scalar][log2i(t.vw)] = best;
+        ctx->tune[h2->scalar][log2i(t.vw)] = best;
         if (variant) *variant = best;
     });
 }

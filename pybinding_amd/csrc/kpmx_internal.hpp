@@ -149,6 +149,8 @@ void launch_gather_block(int scalar, const void* src, int64_t n, int64_t c0, int
                          hipStream_t s);
 void launch_scatter_block(int scalar, const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst,
                           hipStream_t s);
+// columns [c0, c0 + w) of a row-major n x src_vw block into a row-major n x w block
+void launch_copy_cols(int scalar, const void* src, int64_t n, int src_vw, int c0, int w, void* dst, hipStream_t s);
 void launch_zero_tails(uint16_t* base, int64_t rows, int64_t ld, int64_t n, hipStream_t s);
 // the bf16 pieces of scale * src (columns 0..cols-1) into split planes (stride `plane` elements)
 void launch_scatter_split(int scalar, const void* src, int64_t n, int vw, int64_t cols, double scale, uint16_t* dst,

@@ -1010,6 +1010,17 @@ __global__ void k_gather_block(const T* src, int64_t n, int64_t c0, int64_t cols
     }
 }
 
+// Columns [c0, c0 + w) of a row-major n x src_vw block -> row-major n x w block.
+template <class T>
+__global__ void k_copy_cols(const T* __restrict__ src, int64_t n, int src_vw, int c0, int w, T* __restrict__ dst) {
+    const int64_t total = n * w;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / w;
+        const int c = (int)(e % w);
+        dst[e] = src[row * src_vw + c0 + c];
+    }
+}
+
 // Row-major n x vw block -> vector-major columns [c0, c0+cols) of dst (nvec x n).
 template <class T>
 __global__ void k_scatter_block(const T* src, int64_t n, int vw, int64_t c0, int64_t cols, T* dst) {
@@ -1453,6 +1464,11 @@ static void gather_t(const void* src, int64_t n, int64_t c0, int64_t cols, int v
                        static_cast<T*>(dst));
 }
 template <class T>
+static void copy_cols_t(const void* src, int64_t n, int src_vw, int c0, int w, void* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_cols<T>, grid_for(n * w), dim3(256), 0, s, static_cast<const T*>(src), n, src_vw, c0, w,
+                       static_cast<T*>(dst));
+}
+template <class T>
 static void scatter_t(const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst, hipStream_t s) {
     hipLaunchKernelGGL(k_scatter_block<T>, grid_for(n * cols), dim3(256), 0, s, static_cast<const T*>(src), n, vw, c0, cols,
                        static_cast<T*>(dst));
@@ -1484,6 +1500,9 @@ static void matvec_t(const int32_t* indptr, const int32_t* indices, const void* 
 
 void launch_gather_block(int scalar, const void* src, int64_t n, int64_t c0, int64_t cols, int vw, void* dst, hipStream_t s) {
     KPMX_DISPATCH(scalar, gather_t, src, n, c0, cols, vw, dst, s)
+}
+void launch_copy_cols(int scalar, const void* src, int64_t n, int src_vw, int c0, int w, void* dst, hipStream_t s) {
+    KPMX_DISPATCH(scalar, copy_cols_t, src, n, src_vw, c0, w, dst, s)
 }
 void launch_scatter_block(int scalar, const void* src, int64_t n, int vw, int64_t c0, int64_t cols, void* dst, hipStream_t s) {
     KPMX_DISPATCH(scalar, scatter_t, src, n, vw, c0, cols, dst, s)
