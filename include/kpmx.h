@@ -143,6 +143,19 @@ int kpmx_ctx_set_adjoint_variant(kpmx_ctx* ctx, int variant);
 int kpmx_ctx_set_tile_width(kpmx_ctx* ctx, int scalar, int width);
 int kpmx_ctx_get_tile_width(kpmx_ctx* ctx, int scalar, int* width);
 
+/* Two Chebyshev steps per launch for the diagonal recurrence of one scalar type (the GPU form of
+ * the reference's interleaved sweep, cppcore/include/kpm/calc_moments.hpp:61-89): tile_rows > 0
+ * (a multiple of 64) cuts the rows into tiles of that many; each workgroup computes the next
+ * vector on its tile and on the tile's halo (the outside rows its rows reference), then the one
+ * after on its tile, so a launch reads two vectors and writes two instead of three per step.
+ * Runs on SELL matrices for column tiles whose rows are whole 128-byte lines (fp64 >= 16
+ * vectors, complex128 >= 8, fp32 >= 32, complex64 >= 16) and without light-cone row limits;
+ * anything else keeps single steps.  0 (the default) turns it off.  workgroups_per_cu > 0 runs
+ * a persistent grid of that many workgroups per CU walking the tiles (0: one workgroup per tile).
+ * The moments are the same up to rounding.  get: the current tile rows. */
+int kpmx_ctx_set_two_step(kpmx_ctx* ctx, int scalar, int tile_rows, int workgroups_per_cu);
+int kpmx_ctx_get_two_step(kpmx_ctx* ctx, int scalar, int* tile_rows);
+
 /* Graph policy (no counterpart in the reference, whose CPU loop has no launch cost): with
  * KPMX_GRAPH_ALWAYS the diagonal recurrence of a starter block is captured into a HIP graph on its
  * first run (KPMX_GRAPH_AUTO: on its second run) and replayed by later runs with the same matrix,

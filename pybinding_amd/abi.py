@@ -126,6 +126,18 @@ class Ctx(_Handle):
         _check(lib().kpmx_ctx_get_tile_width(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.byref(w)))
         return w.value
 
+    def set_two_step(self, dtype, tile_rows, workgroups_per_cu=0):
+        """Two Chebyshev steps per launch over tiles of `tile_rows` rows for the diagonal
+        recurrence of one scalar type (0: single steps), on a persistent grid of
+        `workgroups_per_cu` workgroups per CU (0: one workgroup per tile)."""
+        _check(lib().kpmx_ctx_set_two_step(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.c_int(tile_rows),
+                                           ctypes.c_int(workgroups_per_cu)))
+
+    def two_step(self, dtype):
+        p = ctypes.c_int()
+        _check(lib().kpmx_ctx_get_two_step(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.byref(p)))
+        return p.value
+
     def set_graph_policy(self, policy):
         """Graph policy of the diagonal recurrence: 0 never (default), 1 auto (small starter blocks,
         captured on the second identical run), 2 always."""

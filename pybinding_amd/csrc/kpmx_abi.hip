@@ -138,6 +138,11 @@ struct kpmx_ctx {
     // widest column tile the diagonal recurrence runs per scalar type (kpmx_ctx_set_tile_width):
     // wider starter tiles are run as several recurrences over column sub-tiles
     int tile_cap[4] = {64, 64, 64, 64};
+    // rows per tile of the two-step diagonal recurrence per scalar type; 0: single steps
+    // (kpmx_ctx_set_two_step)
+    int two_step[4] = {0, 0, 0, 0};
+    int two_step_wg_per_cu[4] = {0, 0, 0, 0};  // persistent grid of the two-step kernel; 0: a workgroup per tile
+    Workspace work3, work4;  // the third and fourth vector of the two-step recurrence
     int graph_policy = KPMX_GRAPH_NEVER;
     int adjoint_variant = -1;  // adjoint step variant (kpmx_ctx_set_adjoint_variant); -1: the default
     // device memory of the largest finished adjoint tape, reused by the next tape that fits: a
@@ -172,7 +177,14 @@ struct kpmx_mat {
     // light cone (kpmx_mat_set_step_rows): rows computed by each step of the next recurrences
     std::vector<int64_t> step_rows;
     mutable int64_t* d_sell_pos = nullptr;  // device copy, uploaded by the first adjoint pass
+    // halo rows of the two-step recurrence's tiles of halo_tile rows (built on first use): the
+    // rows outside tile j that its rows reference are h1_rows[h1_ptr[j] .. h1_ptr[j+1])
+    mutable int64_t halo_tile = 0, halo_ntiles = 0, halo_rows = 0;
+    mutable int64_t* d_h1_ptr = nullptr;
+    mutable int32_t* d_h1_rows = nullptr;
     ~kpmx_mat() {
+        dev_free(d_h1_ptr);
+        dev_free(d_h1_rows);
         dev_free(d_sell_pos);
         dev_free(indptr);
         dev_free(indices);
@@ -463,6 +475,159 @@ void diag_collect(const std::vector<double>& s, int vw, int64_t cols, int64_t M,
     }
 }
 
+// Halo rows of every tile of P rows: for tile j the sorted distinct columns of its rows' SELL
+// entries that lie outside the tile (read back from the device copy once per tile size).
+void ensure_halo(const kpmx_mat* h2, int64_t P) {
+    if (h2->halo_tile == P && h2->d_h1_ptr) return;
+    KPMX_REQUIRE(h2->format == KPMX_FMT_SELL, "the two-step recurrence needs the SELL format");
+    kpmx_ctx* ctx = h2->ctx;
+    const int64_t n = h2->rows, ns = h2->nslices;
+    std::vector<int64_t> sptr(ns + 1);
+    std::vector<int32_t> swidth(ns), sidx((size_t)std::max<int64_t>(h2->sell_total, 1));
+    KPMX_HIP(hipMemcpyAsync(sptr.data(), h2->slice_ptr, (ns + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(swidth.data(), h2->slice_width, ns * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (h2->sell_total)
+        KPMX_HIP(hipMemcpyAsync(sidx.data(), h2->sell_indices, h2->sell_total * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t ntiles = (n + P - 1) / P;
+    std::vector<int64_t> hptr(ntiles + 1, 0);
+    std::vector<int32_t> hrows;
+    std::vector<int64_t> stamp((size_t)n, -1);
+    std::vector<int32_t> cur;
+    for (int64_t j = 0; j < ntiles; ++j) {
+        const int64_t t0 = j * P, t1 = std::min(n, t0 + P);
+        cur.clear();
+        for (int64_t r = t0; r < t1; ++r) {
+            const int64_t slc = r >> 6, lane = r & 63;
+            for (int32_t k = 0; k < swidth[slc]; ++k) {
+                const int32_t c = sidx[(size_t)(sptr[slc] + (int64_t)k * 64 + lane)];
+                if ((c < t0 || c >= t1) && stamp[(size_t)c] != j) {
+                    stamp[(size_t)c] = j;
+                    cur.push_back(c);
+                }
+            }
+        }
+        std::sort(cur.begin(), cur.end());
+        hrows.insert(hrows.end(), cur.begin(), cur.end());
+        hptr[j + 1] = (int64_t)hrows.size();
+    }
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    dev_free(h2->d_h1_ptr);
+    dev_free(h2->d_h1_rows);
+    h2->d_h1_ptr = nullptr;
+    h2->d_h1_rows = nullptr;
+    h2->halo_tile = 0;
+    h2->d_h1_ptr = static_cast<int64_t*>(dev_alloc(hptr.size() * sizeof(int64_t)));
+    h2->d_h1_rows = static_cast<int32_t*>(dev_alloc(std::max<size_t>(hrows.size(), 1) * sizeof(int32_t)));
+    KPMX_HIP(hipMemcpyAsync(h2->d_h1_ptr, hptr.data(), hptr.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    if (!hrows.empty())
+        KPMX_HIP(hipMemcpyAsync(h2->d_h1_rows, hrows.data(), hrows.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    h2->halo_tile = P;
+    h2->halo_ntiles = ntiles;
+    h2->halo_rows = (int64_t)hrows.size();
+}
+
+// Whether the diagonal recurrence of this column tile runs two steps per launch.
+bool use_two_step(const kpmx_mat* h2, const Tile& t) {
+    return h2->ctx->two_step[h2->scalar] > 0 && h2->format == KPMX_FMT_SELL && h2->step_rows.empty() &&
+           step2_supported(h2->scalar, t.vw) && h2->rows >= 2 * 64;
+}
+
+// Device memory the two-step recurrence of this tile uses (sized before a graph capture).
+void size_two_step(const kpmx_mat* h2, const Tile& t, int64_t steps) {
+    kpmx_ctx* ctx = h2->ctx;
+    const int64_t P = ctx->two_step[h2->scalar];
+    ensure_halo(h2, P);
+    const size_t blk = (size_t)h2->rows * t.vw * scalar_bytes(h2->scalar);
+    ctx->work0.get(blk);
+    ctx->work1.get(blk);
+    ctx->work3.get(blk);
+    ctx->work4.get(blk);
+    const int grid = step2_grid(h2->halo_ntiles, ctx->two_step_wg_per_cu[h2->scalar] * ctx->num_cu);
+    const size_t per_step = (size_t)grid * step2_partial_width(h2->scalar, t.vw) * sizeof(double);
+    int64_t chunk = std::max<int64_t>(2, partials_chunk(per_step, steps));
+    StepLaunch L = base_launch(h2, t.vw, true);
+    L.mode = MODE_DIAG;
+    L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
+    L.num_wg = step_grid(L);
+    const size_t single = (size_t)L.num_wg * 4 * t.vw * sizeof(double);
+    ctx->partials.get(std::max(per_step * (size_t)chunk, single));
+}
+
+// The diagonal recurrence with two steps per launch (k_step2_sell over tiles of
+// ctx->two_step[scalar] rows), the last step of an odd count by the single-step kernel.  r_j
+// (j >= 1) lives in the j % 4-th of four work vectors: a launch reads r_{k-1}, r_k and writes
+// r_{k+1}, r_{k+2}, so nothing it reads is overwritten.  Same raw sums as run_recurrence.
+void run_recurrence2(const kpmx_mat* h2, const Tile& t, int64_t steps, double* sums) {
+    kpmx_ctx* ctx = h2->ctx;
+    size_two_step(h2, t, steps);
+    const int64_t n = h2->rows;
+    const size_t blk = (size_t)n * t.vw * scalar_bytes(h2->scalar);
+    void* W[4] = {ctx->work0.get(blk), ctx->work1.get(blk), ctx->work3.get(blk), ctx->work4.get(blk)};
+    auto r = [&](int64_t j) -> void* { return j == 0 ? t.data : W[j % 4]; };
+    const int64_t P = ctx->two_step[h2->scalar];
+    const int grid = step2_grid(h2->halo_ntiles, ctx->two_step_wg_per_cu[h2->scalar] * ctx->num_cu);
+    const int pw = step2_partial_width(h2->scalar, t.vw);
+    const int width = 4 * t.vw;
+    const int pair_cols = h2->scalar == KPMX_F64 ? t.vw : 0;
+    const size_t per_step = (size_t)grid * pw * sizeof(double);
+    int64_t chunk = std::max<int64_t>(2, partials_chunk(per_step, steps));
+    chunk -= chunk % 2;
+    double* partials = static_cast<double*>(ctx->partials.get(per_step * (size_t)chunk));
+    Step2Launch S;
+    S.scalar = h2->scalar;
+    S.vw = t.vw;
+    S.max_width = h2->max_width;
+    S.indices = h2->sell_indices;
+    S.vals = h2->sell_vals;
+    S.slice_ptr = h2->slice_ptr;
+    S.slice_width = h2->slice_width;
+    S.h1_ptr = h2->d_h1_ptr;
+    S.h1_rows = h2->d_h1_rows;
+    S.n = n;
+    S.ntiles = h2->halo_ntiles;
+    S.tile_rows = P;
+    S.num_wg = grid;
+    int64_t k = 0, chunk_start = 0;
+    for (; k + 1 < steps; k += 2) {
+        S.xprev = k == 0 ? nullptr : r(k - 1);
+        S.x = r(k);
+        S.out1 = r(k + 1);
+        S.out2 = r(k + 2);
+        S.hscaleA = k == 0 ? 0.5 : 1.0;
+        S.partialsA = partials + (size_t)(k - chunk_start) * grid * pw;
+        S.partialsB = partials + (size_t)(k + 1 - chunk_start) * grid * pw;
+        if (launch_step2(S, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported two-step configuration"};
+        KPMX_HIP(hipGetLastError());
+        if (k + 2 - chunk_start == chunk || k + 2 >= steps - 1) {
+            launch_reduce_partials(partials, grid, pw, k + 2 - chunk_start, sums + (size_t)chunk_start * width,
+                                   ctx->stream, pair_cols);
+            KPMX_HIP(hipGetLastError());
+            chunk_start = k + 2;
+        }
+    }
+    if (k < steps) {  // an odd step count ends with one single step
+        StepLaunch L = base_launch(h2, t.vw, true);
+        L.mode = MODE_DIAG;
+        L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
+        L.num_wg = step_grid(L);
+        L.x = r(k);
+        L.yin = k == 0 ? nullptr : r(k - 1);
+        L.yout = r(k + 1);
+        L.hscale = k == 0 ? 0.5 : 1.0;
+        L.sub_y = k == 0 ? 0 : 1;
+        L.partials = partials;
+        launch(L, ctx->stream);
+        const bool pairs = step_uses_pairs(L);
+        launch_reduce_partials(partials, L.num_wg, pairs ? 2 * t.vw : width, 1, sums + (size_t)k * width, ctx->stream,
+                               pairs ? t.vw : 0);
+        KPMX_HIP(hipGetLastError());
+    }
+}
+
 void check_pair(const kpmx_mat* h2, const kpmx_block* r0) {
     KPMX_REQUIRE(h2 && r0, "null matrix or starter block");
     KPMX_REQUIRE(h2->ctx == r0->ctx, "matrix and starter block belong to different contexts");
@@ -519,8 +684,11 @@ void diag_enqueue(const kpmx_mat* h2, const kpmx_block* r0, int64_t steps) {
     const auto rts = run_tiles(r0, r0->sums_cap);
     for (size_t i = 0; i < rts.size(); ++i) {
         const Tile sub = stage_run_tile(h2, r0, rts[i]);
-        run_recurrence(h2, sub, MODE_DIAG, steps, nullptr, static_cast<double*>(r0->sums[i]),
-                       [](StepLaunch&, int64_t) {}, [](StepLaunch const&, int64_t) {});
+        if (use_two_step(h2, sub))
+            run_recurrence2(h2, sub, steps, static_cast<double*>(r0->sums[i]));
+        else
+            run_recurrence(h2, sub, MODE_DIAG, steps, nullptr, static_cast<double*>(r0->sums[i]),
+                           [](StepLaunch&, int64_t) {}, [](StepLaunch const&, int64_t) {});
     }
 }
 
@@ -571,6 +739,14 @@ void diag_run(const kpmx_mat* h2, const kpmx_block* r0c, int64_t M) {
         const size_t per_step_bytes = (size_t)L.num_wg * 4 * t.vw * sizeof(double);
         part = std::max(part, per_step_bytes * partials_chunk(per_step_bytes, steps));
         key.push_back((uint64_t)L.tune);
+        if (use_two_step(h2, t)) {
+            size_two_step(h2, t, steps);
+            key.push_back((uint64_t)ctx->two_step[h2->scalar]);
+            key.push_back((uint64_t)ctx->two_step_wg_per_cu[h2->scalar]);
+            key.push_back((uint64_t)(uintptr_t)ctx->work3.get(work));
+            key.push_back((uint64_t)(uintptr_t)ctx->work4.get(work));
+            part = std::max(part, ctx->partials.bytes);
+        }
     }
     key.push_back((uint64_t)(uintptr_t)ctx->work0.get(work));
     key.push_back((uint64_t)(uintptr_t)ctx->work1.get(work));
@@ -1018,6 +1194,26 @@ int kpmx_ctx_get_tile_width(kpmx_ctx* ctx, int scalar, int* width) {
         KPMX_REQUIRE(ctx && width, "null argument");
         KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
         *width = ctx->tile_cap[scalar];
+    });
+}
+
+int kpmx_ctx_set_two_step(kpmx_ctx* ctx, int scalar, int tile_rows, int workgroups_per_cu) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        KPMX_REQUIRE(tile_rows == 0 || (tile_rows >= 64 && tile_rows % 64 == 0 && tile_rows <= (1 << 20)),
+                     "two-step tile rows must be 0 or a multiple of 64 up to 1048576");
+        KPMX_REQUIRE(workgroups_per_cu >= 0 && workgroups_per_cu <= 64, "workgroups per CU must lie in [0, 64]");
+        ctx->two_step[scalar] = tile_rows;
+        ctx->two_step_wg_per_cu[scalar] = workgroups_per_cu;
+    });
+}
+
+int kpmx_ctx_get_two_step(kpmx_ctx* ctx, int scalar, int* tile_rows) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx && tile_rows, "null argument");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        *tile_rows = ctx->two_step[scalar];
     });
 }
 

@@ -97,6 +97,35 @@ struct BatchedLaunch {
     double hscale = 1.0;
     int sub_y = 1;
 };
+// Two Chebyshev steps per launch over tiles of `tile_rows` rows (k_step2_sell): phase A writes
+// r_{k+1} = s H2 x - xprev to out1 on every tile's rows and halo rows (h1_rows[h1_ptr[j] ..]),
+// phase B writes r_{k+2} = H2 out1 - x to out2 on the tile's rows; the diagonal sums of the two
+// steps go to partialsA / partialsB (step2_grid slots each, step2_partial_width doubles per slot,
+// fp64 in the column-pair layout).
+struct Step2Launch {
+    int scalar = 1, vw = 16, max_width = 0;
+    const int32_t* indices = nullptr;
+    const void* vals = nullptr;
+    const int64_t* slice_ptr = nullptr;
+    const int32_t* slice_width = nullptr;
+    const int64_t* h1_ptr = nullptr;
+    const int32_t* h1_rows = nullptr;
+    const void* xprev = nullptr;
+    const void* x = nullptr;
+    void* out1 = nullptr;
+    void* out2 = nullptr;
+    double* partialsA = nullptr;
+    double* partialsB = nullptr;
+    int64_t n = 0, ntiles = 0, tile_rows = 0;
+    int num_wg = 0;  // persistent grid (a multiple of 8); 0: one workgroup per tile
+    double hscaleA = 1.0;
+};
+// rows of whole 128-byte lines (the duplicate halo stores must not share a line with other rows)
+bool step2_supported(int scalar, int vw);
+int step2_grid(int64_t ntiles, int num_wg);
+int step2_partial_width(int scalar, int vw);
+int launch_step2(const Step2Launch& S, hipStream_t stream);
+
 // The batched step kernel runs one walker per wave.
 constexpr int BATCH_WALKERS_PER_WG = 4;
 int launch_batched_step(const BatchedLaunch& B, hipStream_t stream);

@@ -696,6 +696,177 @@ __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int
     wave_diag_store<T, VW>(a_re, a_im, b_re, b_im, a.partials, slot_index[slot], lane);
 }
 
+// Two Chebyshev steps in one launch: the temporal blocking of the reference's interleaved sweep
+// (cppcore/include/kpm/calc_moments.hpp:61-89) for SELL-64 matrices, without any ordering
+// between workgroups.  Tile j is the rows [j P, j P + P); workgroups take tiles along an
+// XCD-aware walk (the workgroups of one XCD sweep one contiguous band of tiles), and for each:
+//   phase A: r_{k+1} = s H2 r_k - r_{k-1} on the tile's rows and on its halo, the rows outside
+//            the tile that its rows reference (h1_rows[h1_ptr[j] .. h1_ptr[j+1])), all written
+//            to out1;
+//   phase B: r_{k+2} = H2 r_{k+1} - r_k on the tile's rows, gathering r_{k+1} from out1 (written
+//            by this workgroup in phase A, so from its XCD's L2), written to out2.
+// A halo row is also computed by the tile that owns it, from the same entries in the same order,
+// so every copy stored to out1 is bit-identical and the duplicate stores are harmless; rows are
+// whole 128-byte lines (the launcher requires it), so no line mixes a row this workgroup stored
+// with one it did not.  Vector traffic per two steps: r_k on tile + halo + halo of the halo,
+// r_{k-1} on tile + halo, r_{k+1} stored on tile + halo, r_{k+2} stored on the tile, against
+// six row blocks for two single steps.  The diagonal sums of both steps over the tile's rows go
+// to partialsA and partialsB (one slot per workgroup).
+template <class T>
+struct Step2Args {
+    using real = typename traits<T>::real;
+    using value = typename traits<T>::value;
+    const int32_t* indices;
+    const value* vals;
+    const int64_t* slice_ptr;
+    const int32_t* slice_width;
+    const int64_t* h1_ptr;  // ntiles + 1
+    const int32_t* h1_rows;
+    const T* xprev;  // r_{k-1}; null on the first step
+    const T* x;      // r_k
+    T* out1;         // r_{k+1}
+    T* out2;         // r_{k+2}
+    double* partialsA;
+    double* partialsB;
+    int64_t n, ntiles, tile_rows;
+    real hscaleA;  // 0.5 on the first step, else 1
+};
+
+template <class T, int VW, int KMAX>
+__global__ __launch_bounds__(BLOCK) void k_step2_sell(Step2Args<T> a) {
+    constexpr int RPW = WAVE / VW;  // rows per row group
+    constexpr int U = VW;           // row groups per 64-row slice
+    constexpr int UCM = sizeof(T) <= 4 ? 8 : sizeof(T) <= 8 ? 4 : 2;
+    constexpr int UC = U < UCM ? U : UCM;  // row groups per pass
+    static_assert(U % UC == 0, "bad two-step configuration");
+    using V = typename traits<T>::value;
+    const int lane = threadIdx.x % WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int rsub = lane / VW;
+    const int col = lane % VW;
+    // XCD-aware persistent walk: region r (the workgroups with blockIdx % 8 == r, on XCD r) owns
+    // one contiguous eighth of the tiles, dealt round-robin to its workgroups
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int64_t per_region = (a.ntiles + 7) / 8;
+    const int64_t region0 = (int64_t)(b % 8) * per_region;
+    double aa_re = 0, aa_im = 0, ab_re = 0, ab_im = 0;  // phase A sums
+    double ba_re = 0, ba_im = 0, bb_re = 0, bb_im = 0;  // phase B sums
+    for (int64_t tl = b / 8; tl < per_region && region0 + tl < a.ntiles; tl += nb / 8) {  // uniform
+        const int64_t tile = region0 + tl;
+        const int64_t t0 = tile * a.tile_rows;  // a multiple of 64: the tile is whole slices
+        const int64_t pt = a.n - t0 < a.tile_rows ? a.n - t0 : a.tile_rows;
+        const int64_t s0 = t0 / WAVE, s1 = (t0 + pt + WAVE - 1) / WAVE;
+        // The tile's slices, wave w taking slices s0 + w, s0 + w + 4, ...: lane l holds the first
+        // KMAX entries of row l of a slice (the next slice's loaded while this one's gathers are
+        // in flight) and hands them to the gathering lanes by shuffles, as in k_step_batched.
+        auto slices = [&](bool phase_a) {
+            const T* __restrict__ src = phase_a ? a.x : a.out1;
+            const T* __restrict__ ysub = phase_a ? a.xprev : a.x;
+            T* __restrict__ dst = phase_a ? a.out1 : a.out2;
+            const typename traits<T>::real hs = phase_a ? a.hscaleA : typename traits<T>::real(1);
+            int32_t p_idx[KMAX];
+            V p_val[KMAX];
+            int64_t next_base = 0;
+            int next_width = 0;
+            auto prefetch = [&](int64_t slc) {
+                next_base = a.slice_ptr[slc];
+                next_width = a.slice_width[slc];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) {
+                    if (k < next_width) {
+                        p_idx[k] = a.indices[next_base + (int64_t)k * WAVE + lane];
+                        p_val[k] = a.vals[next_base + (int64_t)k * WAVE + lane];
+                    } else {
+                        p_idx[k] = (int32_t)(slc * WAVE + lane < a.n ? slc * WAVE + lane : a.n - 1);
+                        p_val[k] = zero(V{});
+                    }
+                }
+            };
+            int64_t slc = s0 + wave;
+            if (slc < s1) prefetch(slc);
+            for (; slc < s1; slc += WAVES_PER_BLOCK) {
+                const int64_t base = next_base;
+                const int width = next_width;
+                int32_t c_idx[KMAX];
+                V c_val[KMAX];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) {
+                    c_idx[k] = p_idx[k];
+                    c_val[k] = p_val[k];
+                }
+#pragma unroll
+                for (int u0 = 0; u0 < U; u0 += UC) {
+                    int rin[UC];
+                    int64_t e[UC];
+                    bool ok[UC];
+                    T yv[UC], xo[UC], xg[UC][KMAX];
+#pragma unroll
+                    for (int u = 0; u < UC; ++u) {
+                        rin[u] = (u0 + u) * RPW + rsub;
+                        const int64_t row = slc * WAVE + rin[u];
+                        ok[u] = row < t0 + pt;
+                        e[u] = (ok[u] ? row : t0) * VW + col;
+                        yv[u] = ysub ? (phase_a ? nt_load(&ysub[e[u]]) : ysub[e[u]]) : zero(T{});
+                        xo[u] = src[e[u]];
+                    }
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+                        for (int u = 0; u < UC; ++u) xg[u][k] = src[(int64_t)__shfl(c_idx[k], rin[u], WAVE) * VW + col];
+                    if (u0 == 0 && slc + WAVES_PER_BLOCK < s1) prefetch(slc + WAVES_PER_BLOCK);
+#pragma unroll
+                    for (int u = 0; u < UC; ++u) {
+                        T acc = zero(T{});
+#pragma unroll
+                        for (int k = 0; k < KMAX; ++k) acc = fmac(shfl_any(c_val[k], rin[u]), xg[u][k], acc);
+                        for (int k = KMAX; k < width; ++k) {
+                            const int64_t off = base + (int64_t)k * WAVE + rin[u];
+                            acc = fmac(a.vals[off], src[(int64_t)a.indices[off] * VW + col], acc);
+                        }
+                        T ynew = scal(acc, hs);
+                        if (ysub) ynew = sub(ynew, yv[u]);
+                        if (ok[u]) {
+                            // r_{k+1} is gathered again in phase B: a plain store keeps it in L2
+                            if (phase_a) dst[e[u]] = ynew;
+                            else nt_store(ynew, &dst[e[u]]);
+                            if (phase_a) add_diag_sums(ynew, xo[u], aa_re, aa_im, ab_re, ab_im);
+                            else add_diag_sums(ynew, xo[u], ba_re, ba_im, bb_re, bb_im);
+                        }
+                    }
+                }
+            }
+        };
+        slices(true);
+        // phase A on the halo rows, row by row (each lane loads its row's entries)
+        {
+            const int64_t h0 = a.h1_ptr[tile], nh = a.h1_ptr[tile + 1] - h0;
+            for (int64_t g = wave; g * RPW < nh; g += WAVES_PER_BLOCK) {
+                const int64_t i = g * RPW + rsub;
+                const bool ok = i < nh;
+                const int64_t row = ok ? (int64_t)a.h1_rows[h0 + i] : t0;
+                const int64_t slc = row >> 6;
+                const int64_t base = a.slice_ptr[slc] + (row & 63);
+                const int width = a.slice_width[slc];
+                T acc = zero(T{});
+                for (int k = 0; k < width; ++k) {
+                    const int64_t off = base + (int64_t)k * WAVE;
+                    acc = fmac(a.vals[off], a.x[(int64_t)a.indices[off] * VW + col], acc);
+                }
+                T ynew = scal(acc, a.hscaleA);
+                if (a.xprev) ynew = sub(ynew, a.xprev[row * VW + col]);
+                if (ok) a.out1[row * VW + col] = ynew;
+            }
+        }
+        // workgroup-scope release and acquire: every r_{k+1} this workgroup stored is visible to
+        // all of its waves (one CU) before any of them gathers one; no device-scope fence, which
+        // would write the XCD's L2 back to memory
+        __syncthreads();
+        slices(false);
+    }
+    block_reduce_store<VW>(aa_re, aa_im, ab_re, ab_im, a.partialsA);
+    block_reduce_store<VW>(ba_re, ba_im, bb_re, bb_im, a.partialsB);
+}
+
 // sums[s][j] = sum over the slots [slot_first[s], slot_first[s+1]) of partials[slot][j]: one
 // workgroup of RS_THREADS per system; thread t sums column j = t % width of the slots g, g + G,
 // g + 2G, ... (g = t / width, G = RS_THREADS / width groups, loads unrolled four deep), then the
@@ -1636,6 +1807,71 @@ static int batched_dispatch(const BatchedLaunch& B, hipStream_t stream, int* occ
 int launch_batched_step(const BatchedLaunch& B, hipStream_t stream) {
     if (B.chunk > 0 && (B.num_wg < 8 || B.num_wg % 8 != 0)) return -1;  // the XCD walk needs 8 | grid
     return batched_dispatch(B, stream, nullptr);
+}
+
+template <class T, int VW>
+static void step2_t(const Step2Launch& S, hipStream_t stream) {
+    Step2Args<T> a{};
+    a.indices = S.indices;
+    a.vals = static_cast<const typename traits<T>::value*>(S.vals);
+    a.slice_ptr = S.slice_ptr;
+    a.slice_width = S.slice_width;
+    a.h1_ptr = S.h1_ptr;
+    a.h1_rows = S.h1_rows;
+    a.xprev = static_cast<const T*>(S.xprev);
+    a.x = static_cast<const T*>(S.x);
+    a.out1 = static_cast<T*>(S.out1);
+    a.out2 = static_cast<T*>(S.out2);
+    a.partialsA = S.partialsA;
+    a.partialsB = S.partialsB;
+    a.n = S.n;
+    a.ntiles = S.ntiles;
+    a.tile_rows = S.tile_rows;
+    a.hscaleA = static_cast<typename traits<T>::real>(S.hscaleA);
+    const dim3 grid((unsigned)step2_grid(S.ntiles, S.num_wg)), block(BLOCK);
+    if (S.max_width <= 4)
+        hipLaunchKernelGGL((k_step2_sell<T, VW, 4>), grid, block, 0, stream, a);
+    else
+        hipLaunchKernelGGL((k_step2_sell<T, VW, 8>), grid, block, 0, stream, a);
+}
+
+bool step2_supported(int scalar, int vw) {
+    static const int bytes[4] = {4, 8, 8, 16};
+    return scalar >= 0 && scalar < 4 && vw >= 4 && vw <= 64 && (vw & (vw - 1)) == 0 && (vw * bytes[scalar]) % 128 == 0;
+}
+
+int step2_grid(int64_t ntiles, int num_wg) {
+    const int64_t all = (ntiles + 7) / 8 * 8;  // one workgroup per tile
+    if (num_wg <= 0 || num_wg >= all) return (int)all;
+    return num_wg < 8 ? 8 : num_wg / 8 * 8;
+}
+
+int step2_partial_width(int scalar, int vw) { return scalar == 1 ? 2 * vw : 4 * vw; }
+
+int launch_step2(const Step2Launch& S, hipStream_t stream) {
+    if (!step2_supported(S.scalar, S.vw) || S.ntiles < 1 || S.tile_rows < 1) return -1;
+    switch (S.scalar) {
+        case 0:
+            if (S.vw == 32) step2_t<float, 32>(S, stream);
+            else step2_t<float, 64>(S, stream);
+            return 0;
+        case 1:  // column pairs
+            if (S.vw == 16) step2_t<dpair, 8>(S, stream);
+            else if (S.vw == 32) step2_t<dpair, 16>(S, stream);
+            else step2_t<dpair, 32>(S, stream);
+            return 0;
+        case 2:
+            if (S.vw == 16) step2_t<cf32, 16>(S, stream);
+            else if (S.vw == 32) step2_t<cf32, 32>(S, stream);
+            else step2_t<cf32, 64>(S, stream);
+            return 0;
+        default:
+            if (S.vw == 8) step2_t<cf64, 8>(S, stream);
+            else if (S.vw == 16) step2_t<cf64, 16>(S, stream);
+            else if (S.vw == 32) step2_t<cf64, 32>(S, stream);
+            else step2_t<cf64, 64>(S, stream);
+            return 0;
+    }
 }
 
 int batched_blocks_per_cu(const BatchedLaunch& B) {
