@@ -237,8 +237,11 @@ int kpmx_tape_destroy(kpmx_tape* tape);
  *             indices; system s owns rows [row_offsets[s], row_offsets[s+1]) and may only couple
  *             to them).
  *   starters: num_vectors (1..64) starters per system, host, vector-major num_vectors x N.
- *   run:      all num_moments / 2 steps; use_graph != 0 captures the whole step sequence in one
- *             HIP graph on the first run for this num_moments and replays it afterwards.
+ *   run:      all num_moments / 2 steps.  use_graph (the run mode): 0 one launch per step (and
+ *             per-system reduction), 1 the whole step sequence captured in one HIP graph on the
+ *             first run for this num_moments and replayed afterwards, 2 one persistent kernel
+ *             launch for all steps (grid barriers between steps; the partition's workgroups must
+ *             fit the resident grid, else KPMX_EINVAL; returns after the run has finished).
  *   fetch:    mu: host, num_systems x num_vectors x num_moments (mu_0 halved). */
 int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int64_t* row_offsets, int64_t nnz,
                       const int32_t* indptr, const int32_t* indices, const void* values, kpmx_batch** out);

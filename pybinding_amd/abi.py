@@ -291,7 +291,10 @@ class Batch(_Handle):
         _check(lib().kpmx_batch_starters(self.h, ctypes.c_int64(self.nvec), _p(r0)))
 
     def run(self, num_moments, use_graph=True):
-        _check(lib().kpmx_batch_run(self.h, ctypes.c_int64(num_moments), ctypes.c_int(1 if use_graph else 0)))
+        """All num_moments / 2 steps: use_graph False (one launch per step), True (one HIP graph)
+        or "persistent" (one persistent kernel launch with grid barriers between steps)."""
+        mode = 2 if use_graph == "persistent" else (1 if use_graph else 0)
+        _check(lib().kpmx_batch_run(self.h, ctypes.c_int64(num_moments), ctypes.c_int(mode)))
 
     def autotune(self):
         """Pick the fastest slice partition (kpmx_batch_autotune); returns (workgroups, walk_chunk),

@@ -94,7 +94,9 @@ class KPMBatch:
         self.batch.set_partition(num_workgroups, walk_chunk)
 
     def moments(self, num_moments, num_random=1, use_graph=True):
-        """Undamped diagonal moments, shape (num_systems, num_random, num_moments)."""
+        """Undamped diagonal moments, shape (num_systems, num_random, num_moments).  use_graph:
+        True (the step sequence replayed from one HIP graph), False (one launch per step) or
+        "persistent" (one kernel launch for every step, grid barriers between them)."""
         if self.batch.nvec != num_random:
             self.set_random_starters(num_random)
         self.batch.run(num_moments, use_graph)

@@ -258,6 +258,8 @@ struct kpmx_batch {
     void* w0 = nullptr;
     void* w1 = nullptr;
     double* partials = nullptr;
+    double* partials2 = nullptr;     // the persistent run's two partial buffers
+    unsigned* barrier = nullptr;     // the persistent run's grid barrier
     double* sums = nullptr;
     int64_t sums_steps = 0;
     hipGraph_t graph = nullptr;
@@ -273,7 +275,8 @@ struct kpmx_batch {
     ~kpmx_batch() {
         drop_graph();
         for (void* p : {(void*)slice_ptr, (void*)slice_width, (void*)slice_sys, (void*)sidx, svals, (void*)tile_begin,
-                        (void*)slot_begin, (void*)slot_index, (void*)slot_first, r0, w0, w1, (void*)partials, (void*)sums})
+                        (void*)slot_begin, (void*)slot_index, (void*)slot_first, r0, w0, w1, (void*)partials, (void*)partials2,
+                        (void*)barrier, (void*)sums})
             dev_free(p);
     }
 };
@@ -909,7 +912,7 @@ void batch_partition(kpmx_batch* b, int num_wg, int chunk) {
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
     b->drop_graph();
     for (void** p : {(void**)&b->tile_begin, (void**)&b->slot_begin, (void**)&b->slot_index, (void**)&b->slot_first,
-                     (void**)&b->partials}) {
+                     (void**)&b->partials, (void**)&b->partials2}) {
         dev_free(*p);
         *p = nullptr;
     }
@@ -1945,8 +1948,8 @@ int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0) {
 }
 
 namespace {
-BatchedLaunch batch_launch(const kpmx_batch* b) {
-    BatchedLaunch B;
+void batch_launch(const kpmx_batch* b, BatchedLaunch& B) {
+    B = BatchedLaunch{};
     B.scalar = b->scalar;
     B.vw = b->vw;
     B.num_wg = b->num_wg;
@@ -1963,13 +1966,13 @@ BatchedLaunch batch_launch(const kpmx_batch* b) {
     B.slot_index = b->slot_index;
     B.partials = b->partials;
     B.n = b->n_pad;
-    return B;
 }
 
 void batch_enqueue(kpmx_batch* b, int64_t steps) {
     kpmx_ctx* ctx = b->ctx;
     const int width = 4 * b->vw;
-    BatchedLaunch B = batch_launch(b);
+    BatchedLaunch B;
+    batch_launch(b, B);
     for (int64_t k = 0; k < steps; ++k) {
         if (k == 0) {
             B.x = b->r0;
@@ -1992,11 +1995,48 @@ void batch_enqueue(kpmx_batch* b, int64_t steps) {
 }
 }  // namespace
 
+namespace {
+// All steps in one persistent launch (launch_batched_persistent), then a check of its barrier.
+void batch_run_persistent(kpmx_batch* b, int64_t steps) {
+    kpmx_ctx* ctx = b->ctx;
+    BatchedLaunch B;
+    batch_launch(b, B);
+    const int per_cu = batched_persistent_blocks_per_cu(B);
+    if ((int64_t)b->num_wg > (int64_t)per_cu * ctx->num_cu)
+        throw Error{KPMX_EINVAL, "persistent batched run: the partition's " + std::to_string(b->num_wg) +
+                                     " workgroups exceed the resident grid of " + std::to_string(per_cu * ctx->num_cu) +
+                                     "; choose fewer workgroups (kpmx_batch_set_partition)"};
+    const int width = 4 * b->vw;
+    if (!b->partials2)
+        b->partials2 = static_cast<double*>(dev_alloc((size_t)2 * std::max<int64_t>(b->num_slots, 1) * 256 * sizeof(double)));
+    if (!b->barrier) b->barrier = static_cast<unsigned*>(dev_alloc(4 * sizeof(unsigned)));
+    B.r0 = b->r0;
+    B.w0 = b->w0;
+    B.w1 = b->w1;
+    B.partials = b->partials2;
+    B.sums = b->sums;
+    B.slot_first = b->slot_first;
+    B.num_slots = b->num_slots;
+    B.num_sys = b->num_sys;
+    B.steps = steps;
+    B.width = width;
+    B.barrier = b->barrier;
+    if (launch_batched_persistent(B, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported batched configuration"};
+    KPMX_HIP(hipGetLastError());
+    unsigned flags[4] = {0, 0, 0, 0};
+    KPMX_HIP(hipMemcpyAsync(flags, b->barrier, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    if (flags[2] != 0)
+        throw Error{KPMX_EINVAL, "persistent batched run: the grid was not co-resident (barrier wait expired)"};
+}
+}  // namespace
+
 int kpmx_batch_run(kpmx_batch* b, int64_t M, int use_graph) {
     return guarded([&] {
         KPMX_REQUIRE(b, "null batch");
         KPMX_REQUIRE(b->r0, "no starters uploaded");
         KPMX_REQUIRE(M >= 2 && M % 2 == 0, "num_moments must be even and >= 2");
+        KPMX_REQUIRE(use_graph >= 0 && use_graph <= 2, "run mode must be 0, 1 or 2");
         kpmx_ctx* ctx = b->ctx;
         KPMX_HIP(hipSetDevice(ctx->device));
         const int64_t steps = M / 2;
@@ -2010,6 +2050,10 @@ int kpmx_batch_run(kpmx_batch* b, int64_t M, int use_graph) {
             b->drop_graph();
         }
         (void)width;
+        if (use_graph == 2) {
+            batch_run_persistent(b, steps);
+            return;
+        }
         if (!use_graph) {
             batch_enqueue(b, steps);
             KPMX_HIP(hipGetLastError());
@@ -2070,7 +2114,9 @@ int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups, int* walk_chunk) {
             if (!cand.empty() && wg == cand.back().first) break;  // no more slices to spread
             cand.push_back({wg, 0});
         }
-        const int resident = ctx->num_cu * batched_blocks_per_cu(batch_launch(b));
+        BatchedLaunch probe;
+        batch_launch(b, probe);
+        const int resident = ctx->num_cu * batched_blocks_per_cu(probe);
         for (int chunk : {1, 2, 4})
             for (int wg : {resident, resident / 2})
                 if (wg >= 8) cand.push_back({wg, chunk});

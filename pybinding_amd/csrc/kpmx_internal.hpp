@@ -96,6 +96,15 @@ struct BatchedLaunch {
     int64_t n = 0;
     double hscale = 1.0;
     int sub_y = 1;
+    // the persistent form (launch_batched_persistent): the whole recurrence in one launch
+    const void* r0 = nullptr;
+    void* w0 = nullptr;
+    void* w1 = nullptr;
+    double* sums = nullptr;             // steps x num_sys x width
+    const int64_t* slot_first = nullptr;
+    int64_t num_slots = 0, num_sys = 0, steps = 0;
+    int width = 0;                      // 4 vw
+    unsigned* barrier = nullptr;        // 4 words of device memory
 };
 // Two Chebyshev steps per launch over tiles of `tile_rows` rows (k_step2_sell): phase A writes
 // r_{k+1} = s H2 x - xprev to out1 on every tile's rows and halo rows (h1_rows[h1_ptr[j] ..]),
@@ -131,6 +140,11 @@ constexpr int BATCH_WALKERS_PER_WG = 4;
 int launch_batched_step(const BatchedLaunch& B, hipStream_t stream);
 // Workgroups of the batched step kernel that B selects that fit on one CU at once.
 int batched_blocks_per_cu(const BatchedLaunch& B);
+// All steps of the batched recurrence in one persistent launch (partials: 2 x num_slots x width
+// doubles); the grid (B.num_wg) must be resident -- at most num_cu x batched_persistent_blocks_per_cu;
+// afterwards barrier[2] != 0 reports a grid that was not.
+int launch_batched_persistent(const BatchedLaunch& B, hipStream_t stream);
+int batched_persistent_blocks_per_cu(const BatchedLaunch& B);
 void launch_reduce_segments(const double* partials, const int64_t* slot_first, int64_t num_sys, int width, double* sums,
                             hipStream_t stream);
 

@@ -586,13 +586,12 @@ __device__ __forceinline__ void wave_diag_store(double a_re, double a_im, double
 // slot_begin[w] + j, whose sums go to partial slot_index[stretch]); the partials of one system
 // are consecutive, in walker order, and k_reduce_segments sums them in order.
 template <class T, int VW, int KMAX, bool OFF32>
-__global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
-                                                        const int32_t* __restrict__ slice_width,
-                                                        const int32_t* __restrict__ slice_sys,
-                                                        const int64_t* __restrict__ tile_begin,
-                                                        const int64_t* __restrict__ slot_begin,
-                                                        const int64_t* __restrict__ slot_index, int64_t nslices,
-                                                        int walk_chunk) {
+__device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t* __restrict__ slice_ptr,
+                                             const int32_t* __restrict__ slice_width,
+                                             const int32_t* __restrict__ slice_sys,
+                                             const int64_t* __restrict__ tile_begin,
+                                             const int64_t* __restrict__ slot_begin,
+                                             const int64_t* __restrict__ slot_index, int64_t nslices, int walk_chunk) {
     constexpr int RPW = WAVE / VW;  // rows per row group
     constexpr int U = VW;           // row groups per slice
     constexpr int UCM = sizeof(T) <= 4 ? 8 : sizeof(T) <= 8 ? 4 : 2;
@@ -608,7 +607,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int
     const T* __restrict__ x = a.x;
     const int walker = wave * (int)gridDim.x + (int)blockIdx.x;
     BatchWalk walk(nslices, WAVES_PER_BLOCK * (int)gridDim.x, walker, walk_chunk, tile_begin);
-    if (!walk.valid()) return;  // uniform over the wave; the kernel has no barrier
+    if (!walk.valid()) return;  // uniform over the wave; the walk has no barrier
     int64_t slot = slot_begin[walker];
     int cur = slice_sys[walk.s];
     double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
@@ -694,6 +693,136 @@ __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int
         }
     }
     wave_diag_store<T, VW>(a_re, a_im, b_re, b_im, a.partials, slot_index[slot], lane);
+}
+
+template <class T, int VW, int KMAX, bool OFF32>
+__global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ slice_width,
+                                                        const int32_t* __restrict__ slice_sys,
+                                                        const int64_t* __restrict__ tile_begin,
+                                                        const int64_t* __restrict__ slot_begin,
+                                                        const int64_t* __restrict__ slot_index, int64_t nslices,
+                                                        int walk_chunk) {
+    batched_walk<T, VW, KMAX, OFF32>(a, slice_ptr, slice_width, slice_sys, tile_begin, slot_begin, slot_index, nslices,
+                                     walk_chunk);
+}
+
+// Grid barrier of a persistent kernel whose workgroups are all resident: bar[0] counts arrivals,
+// bar[1] is the generation, bar[2] an error flag.  One lane per workgroup arrives (device-scope
+// release of the workgroup's stores, ordered behind a workgroup barrier) and, unless it is the
+// last, waits for the generation to move on; the wait is bounded (about a second of polling), and
+// a workgroup that gives up sets the error flag, which ends every other wait too, so a grid that
+// is not co-resident ends with an error instead of hanging.  Returns false after an error.
+constexpr unsigned GRID_BARRIER_SPINS = 1u << 24;
+__device__ __forceinline__ bool grid_barrier(unsigned* bar, unsigned nb) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        int ok = __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        if (ok) {
+            const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == nb - 1) {
+                __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&bar[1], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                unsigned spins = 0;
+                while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                    if (++spins > GRID_BARRIER_SPINS ||
+                        __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                        __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(4);
+                }
+            }
+        }
+        __threadfence();
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+__global__ void k_reset_barrier(unsigned* bar) {
+    if (threadIdx.x < 4) bar[threadIdx.x] = 0u;
+}
+
+template <class T>
+struct BatchPersistArgs {
+    StepArgs<T> base;  // matrix, n
+    const T* r0;
+    T* w0;
+    T* w1;
+    double* partials;  // two buffers of num_slots x width doubles
+    double* sums;      // steps x num_sys x width
+    const int64_t* slot_first;
+    int64_t num_slots, num_sys, steps;
+    int width;
+    unsigned* barrier;
+};
+
+// The whole batched recurrence in one launch: every workgroup runs its walkers' slices of step k
+// (k_step_batched's walk; partial sums into buffer k % 2), then sums step k - 1's partials of the
+// systems s = blockIdx.x (mod gridDim.x) in a fixed order into sums[k - 1], then meets the others
+// at a grid barrier before step k + 1 reads what step k wrote.  Buffer k % 2 is next written by
+// step k + 2, after the barrier that follows step k's reduction.  The grid must be resident (the
+// launcher checks the occupancy); the barrier's bounded wait turns a violation into an error.
+template <class T, int VW, int KMAX, bool OFF32>
+__global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchPersistArgs<T> p, const int64_t* __restrict__ slice_ptr,
+                                                            const int32_t* __restrict__ slice_width,
+                                                            const int32_t* __restrict__ slice_sys,
+                                                            const int64_t* __restrict__ tile_begin,
+                                                            const int64_t* __restrict__ slot_begin,
+                                                            const int64_t* __restrict__ slot_index, int64_t nslices,
+                                                            int walk_chunk) {
+    __shared__ double part[BLOCK];
+    const unsigned nb = gridDim.x;
+    const int width = p.width;
+    const int groups = BLOCK / width;
+    const int t = threadIdx.x;
+    const int g = t / width, j = t % width;
+    for (int64_t k = 0; k <= p.steps; ++k) {
+        if (k < p.steps) {
+            StepArgs<T> a = p.base;
+            if (k == 0) {
+                a.x = p.r0;
+                a.yin = nullptr;
+                a.yout = p.w1;
+                a.hscale = 0.5;
+                a.sub_y = 0;
+            } else {
+                const bool odd = (k % 2) == 1;
+                a.x = odd ? p.w1 : p.w0;
+                a.yout = odd ? p.w0 : p.w1;
+                a.yin = (k == 1) ? p.r0 : a.yout;
+                a.hscale = 1;
+                a.sub_y = 1;
+            }
+            a.partials = p.partials + (size_t)(k % 2) * p.num_slots * width;
+            batched_walk<T, VW, KMAX, OFF32>(a, slice_ptr, slice_width, slice_sys, tile_begin, slot_begin, slot_index,
+                                             nslices, walk_chunk);
+        }
+        if (k >= 1) {  // step k - 1's per-system sums, in the order of k_reduce_segments' slots
+            const double* __restrict__ prev = p.partials + (size_t)((k - 1) % 2) * p.num_slots * width;
+            double* __restrict__ out = p.sums + (size_t)(k - 1) * p.num_sys * width;
+            for (int64_t sy = blockIdx.x; sy < p.num_sys; sy += nb) {
+                double acc = 0;
+                if (g < groups)
+                    for (int64_t q = p.slot_first[sy] + g; q < p.slot_first[sy + 1]; q += groups) acc += prev[q * width + j];
+                part[t] = acc;
+                __syncthreads();
+                for (int half = groups / 2; half >= 1; half /= 2) {
+                    if (g < half) part[t] += part[t + half * width];
+                    __syncthreads();
+                }
+                if (t < width) out[sy * width + t] = part[t];
+                __syncthreads();
+            }
+        }
+        if (k < p.steps && !grid_barrier(p.barrier, nb)) return;  // uniform over the workgroup
+    }
 }
 
 // Two Chebyshev steps in one launch: the temporal blocking of the reference's interleaved sweep
@@ -1872,6 +2001,81 @@ int launch_step2(const Step2Launch& S, hipStream_t stream) {
             else step2_t<cf64, 64>(S, stream);
             return 0;
     }
+}
+
+template <class T, int VW>
+static int batch_persistent_t(const BatchedLaunch& B, hipStream_t stream, int* occupancy) {
+    BatchPersistArgs<T> p{};
+    p.base.indices = B.indices;
+    p.base.vals = static_cast<const typename traits<T>::value*>(B.vals);
+    p.base.n = B.n;
+    p.r0 = static_cast<const T*>(B.r0);
+    p.w0 = static_cast<T*>(B.w0);
+    p.w1 = static_cast<T*>(B.w1);
+    p.partials = B.partials;
+    p.sums = B.sums;
+    p.slot_first = B.slot_first;
+    p.num_slots = B.num_slots;
+    p.num_sys = B.num_sys;
+    p.steps = B.steps;
+    p.width = B.width;
+    p.barrier = B.barrier;
+    const bool off32 = (uint64_t)B.n * VW * sizeof(T) < ((uint64_t)1 << 32);
+    auto go = [&](auto kernel) -> int {
+        if (occupancy) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(occupancy, kernel, BLOCK, 0) != hipSuccess) *occupancy = 0;
+            return 0;
+        }
+        hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, stream, B.barrier);
+        hipLaunchKernelGGL(kernel, dim3(B.num_wg), dim3(BLOCK), 0, stream, p, B.slice_ptr, B.slice_width, B.slice_sys,
+                           B.tile_begin, B.slot_begin, B.slot_index, B.nslices, B.chunk);
+        return 0;
+    };
+    if (B.max_width <= 4) return off32 ? go(k_batch_persistent<T, VW, 4, true>) : go(k_batch_persistent<T, VW, 4, false>);
+    if (B.max_width == 5) return off32 ? go(k_batch_persistent<T, VW, 5, true>) : go(k_batch_persistent<T, VW, 5, false>);
+    return off32 ? go(k_batch_persistent<T, VW, 8, true>) : go(k_batch_persistent<T, VW, 8, false>);
+}
+
+template <class T>
+static int batch_persistent_vw(const BatchedLaunch& B, hipStream_t stream, int* occupancy) {
+    switch (B.vw) {
+        case 4: return batch_persistent_t<T, 4>(B, stream, occupancy);
+        case 8: return batch_persistent_t<T, 8>(B, stream, occupancy);
+        case 16: return batch_persistent_t<T, 16>(B, stream, occupancy);
+        case 32: return batch_persistent_t<T, 32>(B, stream, occupancy);
+        case 64: return batch_persistent_t<T, 64>(B, stream, occupancy);
+        default: return -1;
+    }
+}
+
+static int batch_persistent_dispatch(const BatchedLaunch& B, hipStream_t stream, int* occupancy) {
+    switch (B.scalar) {
+        case 0: return batch_persistent_vw<float>(B, stream, occupancy);
+        case 1:
+            switch (B.vw) {  // column pairs
+                case 4: return batch_persistent_t<dpair, 2>(B, stream, occupancy);
+                case 8: return batch_persistent_t<dpair, 4>(B, stream, occupancy);
+                case 16: return batch_persistent_t<dpair, 8>(B, stream, occupancy);
+                case 32: return batch_persistent_t<dpair, 16>(B, stream, occupancy);
+                case 64: return batch_persistent_t<dpair, 32>(B, stream, occupancy);
+                default: return -1;
+            }
+        case 2: return batch_persistent_vw<cf32>(B, stream, occupancy);
+        case 3: return batch_persistent_vw<cf64>(B, stream, occupancy);
+        default: return -1;
+    }
+}
+
+int launch_batched_persistent(const BatchedLaunch& B, hipStream_t stream) {
+    if (B.chunk > 0 && (B.num_wg < 8 || B.num_wg % 8 != 0)) return -1;
+    if (!B.barrier || !B.sums || !B.slot_first || B.steps < 1 || B.width < 16 || B.width > BLOCK) return -1;
+    return batch_persistent_dispatch(B, stream, nullptr);
+}
+
+int batched_persistent_blocks_per_cu(const BatchedLaunch& B) {
+    int per_cu = 0;
+    if (batch_persistent_dispatch(B, nullptr, &per_cu) != 0) return 0;
+    return per_cu;
 }
 
 int batched_blocks_per_cu(const BatchedLaunch& B) {

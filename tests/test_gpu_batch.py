@@ -170,3 +170,39 @@ def test_batched_models(oracle):
         h2 = oracle.scale_hamiltonian(h, a, b, np.float64)
         ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], 2, np.float64), 64)
         assert np.max(np.abs(mu[s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
+
+
+@pytest.mark.parametrize("nvec,dtype,tol", [(4, np.float64, 1e-10), (8, np.float64, 1e-10), (4, np.float32, 1e-4),
+                                            (4, np.complex128, 1e-10)])
+def test_batched_persistent_run(oracle, nvec, dtype, tol):
+    """All steps in one persistent launch (grid barriers between steps) for contiguous and
+    XCD-aware partitions: every system matches the oracle and the per-step launches to rounding,
+    and a partition wider than the resident grid is refused."""
+    hs, ranges = systems(np.float64, count=25, seed=13)
+    if dtype == np.complex128:
+        hs = [h.astype(np.complex128) for h in hs]
+    elif dtype == np.float32:
+        hs = [h.astype(np.float32) for h in hs]
+    kb = kpm_batch(hs, ranges)
+    M = 70
+    try:
+        out = {}
+        for wg, chunk in [(64, 0), (256, 2), (8, 1)]:
+            kb.set_partition(wg, chunk)
+            out[wg, chunk] = kb.moments(M, nvec, use_graph="persistent")
+            again = kb.moments(M, nvec, use_graph="persistent")
+            np.testing.assert_array_equal(again, out[wg, chunk])
+            steps = kb.moments(M, nvec, use_graph=False)
+            scale = np.abs(steps[:, :, :1])
+            assert np.max(np.abs(out[wg, chunk] - steps) / scale) <= (1e-13 if tol < 1e-6 else 1e-5)
+        kb.set_partition(2056, 1)  # more workgroups than 8 per CU: never resident at once
+        with pytest.raises(RuntimeError, match="resident grid"):
+            kb.moments(M, nvec, use_graph="persistent")
+    finally:
+        kb.close()
+    mu = out[64, 0]
+    for s, (h, (lo, hi)) in enumerate(zip(hs, ranges)):
+        a, b = oracle.scale_factors(lo, hi)
+        h2 = oracle.scale_hamiltonian(h, a, b, dtype)
+        ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], nvec, dtype), M)
+        assert np.max(np.abs(mu[s] - ref) / np.abs(ref[:, :1])) <= tol, s
