@@ -634,7 +634,11 @@ def main():
     widths = launch_widths(R, tile_width)
     steps_per_run = M // 2
     step_ms = gpu_ms / (args.steps * steps_per_run * len(widths))      # one launch
-    bytes_per_launch = sum(step_bytes(n, nnz, w, s) for w in widths) / len(widths)
+    # the variants 45..59 read the matrix's value-table form (8-bit codes of the off-diagonal
+    # entries, the diagonal dense): its bytes replace nnz (s + 4) + 4 (N + 1) in a launch's count
+    coded = mat.coded_bytes() if variant >= 45 else 0
+    matrix_bytes = coded if coded else nnz * (s + 4) + 4 * (n + 1)
+    bytes_per_launch = sum(matrix_bytes + 3 * n * w * s for w in widths) / len(widths)
     achieved = bytes_per_launch / (step_ms / 1000)
     # the same step over all R vectors as one pass (the matrix read once): the algorithmic bytes of
     # the whole step over the time of its launches
@@ -646,8 +650,8 @@ def main():
         % (gpu_ms / args.steps, len(widths), "/".join(str(w) for w in sorted(set(widths))), step_ms,
            achieved / 1e9, pass_ms, R, pass_gbps))
 
-    traffic_key = "%dx%d %s v%d launch%d %s %s" % (args.cells[0], args.cells[1], dtype.name, R, max(widths), args.format,
-                                                   args.order)
+    traffic_key = "%dx%d %s v%d launch%d %s%s %s" % (args.cells[0], args.cells[1], dtype.name, R, max(widths), args.format,
+                                                     " coded" if coded else "", args.order)
     extra = {"starter_draw_s": t_draw,
              "one_pass_equivalent": {"ms_per_step": pass_ms, "achieved_GBps": pass_gbps,
                                      "frac": pass_gbps * 1e9 / HBM_PEAK, "launch_widths": widths}}
@@ -721,7 +725,8 @@ def main():
         "config": {"workload": "periodic honeycomb %dx%d cells (N=%d, nnz=%d), Anderson W=1 eV, %d moments, "
                                "%d random vectors as one starter block (run in column tiles of %d)"
                                % (args.cells[0], args.cells[1], n, nnz, M, R, tile_width),
-                   "matrix_format": args.format, "site_order": args.order, "kernel_variant": variant,
+                   "matrix_format": args.format + (" (value-table form, %d bytes)" % coded if coded else ""),
+                   "site_order": args.order, "kernel_variant": variant,
                    "moments": M, "vectors": R, "column_tile": tile_width},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": counted_traffic(traffic_key)},

@@ -93,6 +93,11 @@ int kpmx_ctx_timer_stop(kpmx_ctx* ctx, double* ms);
  *  cppcore/include/numeric/ellmatrix.hpp:64-82). */
 int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t nnz, const int32_t* indptr,
                     const int32_t* indices, const void* values, kpmx_mat** out);
+/* Bytes of the matrix's value-table form (the form the fp64 column-pair step variants 45..59
+ * read: slice pointers and widths, 4-byte indices and 1-byte codes of the off-diagonal entries,
+ * the dense diagonal and the value table), or 0 when the matrix has none (no counterpart in the
+ * reference). */
+int kpmx_mat_coded_bytes(const kpmx_mat* m, int64_t* bytes);
 int kpmx_mat_destroy(kpmx_mat* m);
 
 /* Light cone of the next recurrences on this matrix (the reference's `optimal_size` algorithm,
@@ -120,10 +125,14 @@ int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_
  * half of it, a quarter, twice or four times it, each with the tile's matrix entries staged
  * through workgroup LDS or handed out by lane shuffles within each wave) on this matrix and
  * starter block and keep the fastest for every later recurrence of that scalar type and vector
- * width on the context.  variant (optional): the chosen variant index, tile shape + 3 * grid
+ * width on the context, timed on the first column tile the recurrence runs (see
+ * kpmx_ctx_set_tile_width).  variant (optional): the chosen variant index, tile shape + 3 * grid
  * (0 resident, 1 half, 2 quarter, 3 twice, 4 four times) + 15 * staging (0 LDS, 1 shuffles,
  * 2 column pairs: an fp64 block of 8 or more vectors read as pairs of adjacent columns, 16 bytes
- * per lane; for other blocks it runs as 1). */
+ * per lane, for other blocks run as 1; 3 column pairs on the matrix's value-table form: an SELL
+ * of the off-diagonal entries with 8-bit codes into a table of at most 256 values and the
+ * diagonal as a dense vector, built at upload when the off-diagonal entries take that few
+ * distinct values, for other matrices run as 2). */
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant);
 /* Pin the diagonal step variant (as numbered above) for one scalar type and vector width on the
  * context, e.g. a choice an earlier autotuning run reported (no counterpart in the reference). */

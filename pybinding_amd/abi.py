@@ -168,6 +168,12 @@ class Mat(_Handle):
         self._ctx = ctx
         self._own(ctx)
 
+    def coded_bytes(self):
+        """Bytes of the value-table form of the SELL matrix (0: none; kpmx_mat_coded_bytes)."""
+        b = ctypes.c_int64()
+        _check(lib().kpmx_mat_coded_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
 
 class Block(_Handle):
     _destroy_fn = "kpmx_block_destroy"

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <atomic>
 #include <cmath>
 #include <limits>
@@ -135,6 +136,10 @@ struct kpmx_ctx {
     Workspace partials, sums, work0, work1, work2, aux0, aux1, aux2, panel_l, panel_r;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     int tune[4][7] = {};  // tile-shape variant per (scalar, log2 vw), chosen by kpmx_autotune_diagonal
+    // default variant for matrices with the value-table form (-1: `tune`); cleared when a variant
+    // is pinned or tuned, which then applies to every matrix
+    int tune_coded[4][7] = {{-1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1},
+                            {-1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1}};
     // widest column tile the diagonal recurrence runs per scalar type (kpmx_ctx_set_tile_width):
     // wider starter tiles are run as several recurrences over column sub-tiles
     int tile_cap[4] = {64, 64, 64, 64};
@@ -182,7 +187,18 @@ struct kpmx_mat {
     mutable int64_t halo_tile = 0, halo_ntiles = 0, halo_rows = 0;
     mutable int64_t* d_h1_ptr = nullptr;
     mutable int32_t* d_h1_rows = nullptr;
+    // value-table form (SELL over the off-diagonal entries with 8-bit codes into `ctable`, the
+    // diagonal as a dense vector), built at upload when the off-diagonal entries take at most 256
+    // distinct values (lattices with a few hopping families); null otherwise
+    int64_t* cslice_ptr = nullptr;
+    int32_t* cslice_width = nullptr;
+    int32_t* cindices = nullptr;
+    uint8_t* ccodes = nullptr;
+    void* cdiag = nullptr;
+    void* ctable = nullptr;
+    int cmax_width = 0, ctable_size = 0;
     ~kpmx_mat() {
+        for (void* p : {(void*)cslice_ptr, (void*)cslice_width, (void*)cindices, (void*)ccodes, cdiag, ctable}) dev_free(p);
         dev_free(d_h1_ptr);
         dev_free(d_h1_rows);
         dev_free(d_sell_pos);
@@ -346,6 +362,16 @@ StepLaunch base_launch(const kpmx_mat* h2, int vw, bool use_sell) {
     }
     L.num_cu = h2->ctx->num_cu;
     L.num_wg = grid_size(h2->ctx, (h2->rows + step_rows_per_tile(L.format, vw) - 1) / step_rows_per_tile(L.format, vw));
+    if (L.format == FORMAT_SELL && h2->ccodes) {
+        L.cslice_ptr = h2->cslice_ptr;
+        L.cslice_width = h2->cslice_width;
+        L.cindices = h2->cindices;
+        L.ccodes = h2->ccodes;
+        L.cdiag = h2->cdiag;
+        L.ctable = h2->ctable;
+        L.cmax_width = h2->cmax_width;
+        L.ctable_size = h2->ctable_size;
+    }
     return L;
 }
 
@@ -361,6 +387,15 @@ int log2i(int v) {
     int l = 0;
     while ((1 << l) < v) ++l;
     return l;
+}
+
+// The step variant of a (matrix, vector width): the context's choice, or its default for
+// matrices with the value-table form.
+int tuned_variant(const kpmx_mat* h2, int vw) {
+    const kpmx_ctx* ctx = h2->ctx;
+    const int lv = log2i(vw);
+    if (h2->ccodes && ctx->tune_coded[h2->scalar][lv] >= 0) return ctx->tune_coded[h2->scalar][lv];
+    return ctx->tune[h2->scalar][lv];
 }
 
 // Steps whose per-workgroup partial sums are buffered before one reduction launch (256 MiB cap).
@@ -389,7 +424,7 @@ void run_recurrence(const kpmx_mat* h2, const Tile& t, int mode, int64_t steps, 
     StepLaunch L = base_launch(h2, t.vw, true);
     L.mode = mode;
     L.w = w;
-    L.tune = tune >= 0 ? tune : ctx->tune[h2->scalar][log2i(t.vw)];
+    L.tune = tune >= 0 ? tune : tuned_variant(h2, t.vw);
     L.num_wg = step_grid(L);  // the SELL block kernel runs a resident grid
     const int width = 4 * t.vw;  // raw sums per step
     // the fp64 column-pair kernel writes half-width partials in its own layout; the reduction
@@ -554,7 +589,7 @@ void size_two_step(const kpmx_mat* h2, const Tile& t, int64_t steps) {
     int64_t chunk = std::max<int64_t>(2, partials_chunk(per_step, steps));
     StepLaunch L = base_launch(h2, t.vw, true);
     L.mode = MODE_DIAG;
-    L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
+    L.tune = tuned_variant(h2, t.vw);
     L.num_wg = step_grid(L);
     const size_t single = (size_t)L.num_wg * 4 * t.vw * sizeof(double);
     ctx->partials.get(std::max(per_step * (size_t)chunk, single));
@@ -615,7 +650,7 @@ void run_recurrence2(const kpmx_mat* h2, const Tile& t, int64_t steps, double* s
     if (k < steps) {  // an odd step count ends with one single step
         StepLaunch L = base_launch(h2, t.vw, true);
         L.mode = MODE_DIAG;
-        L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
+        L.tune = tuned_variant(h2, t.vw);
         L.num_wg = step_grid(L);
         L.x = r(k);
         L.yin = k == 0 ? nullptr : r(k - 1);
@@ -737,7 +772,7 @@ void diag_run(const kpmx_mat* h2, const kpmx_block* r0c, int64_t M) {
         if (rt.copy) copy = std::max(copy, (size_t)h2->rows * t.vw * sb);
         StepLaunch L = base_launch(h2, t.vw, true);
         L.mode = MODE_DIAG;
-        L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
+        L.tune = tuned_variant(h2, t.vw);
         L.num_wg = step_grid(L);
         const size_t per_step_bytes = (size_t)L.num_wg * 4 * t.vw * sizeof(double);
         part = std::max(part, per_step_bytes * partials_chunk(per_step_bytes, steps));
@@ -1006,6 +1041,9 @@ int kpmx_ctx_create(int device, kpmx_ctx** out) {
             ctx->tune[KPMX_F64][4] = pairs + 1;                     // 16 vectors: tile shape 1, resident grid
             ctx->tune[KPMX_F64][5] = pairs + 1;                     // 32 vectors
             ctx->tune[KPMX_F64][6] = pairs + 1 + NUM_TILE_SHAPES;   // 64 vectors: tile shape 1, half grid
+            // on matrices with the value-table form: its column-pair variant with tile shape 2 on
+            // the resident grid (16 columns: 1.207 ms per 64-vector step against 1.270 ms)
+            ctx->tune_coded[KPMX_F64][4] = pairs + NUM_TILE_SHAPES * NUM_GRID_VARIANTS + 2;
             // fp64 recurrences run in column tiles of at most 16 vectors: on the 4.19 M-site
             // workload four 16-column recurrences take 1.25 ms per step of 64 vectors, one
             // 64-column recurrence 1.29 ms
@@ -1038,6 +1076,91 @@ int kpmx_ctx_synchronize(kpmx_ctx* ctx) {
         KPMX_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
+
+namespace {
+// The value-table form of a SELL matrix (see kpmx_mat): built when every row holds at most one
+// diagonal entry and the off-diagonal entries take at most 256 distinct values (compared bit for
+// bit; the value 0 of padded slots included).  Off-diagonal entries keep their CSR order.
+void build_coded_sell(kpmx_mat* m, const int32_t* indptr, const int32_t* indices, const char* v, size_t sb) {
+    const int64_t rows = m->rows;
+    if (rows == 0 || m->nnz == 0) return;
+    std::vector<std::string> table;
+    std::map<std::string, int> code_of;
+    const std::string zero(sb, '\0');
+    auto code = [&](const char* p) -> int {
+        const std::string key(p, sb);
+        auto it = code_of.find(key);
+        if (it != code_of.end()) return it->second;
+        if (table.size() >= 256) return -1;
+        code_of[key] = (int)table.size();
+        table.push_back(key);
+        return (int)table.size() - 1;
+    };
+    const int zcode = code(zero.data());
+    std::vector<char> diag((size_t)rows * sb, 0);
+    std::vector<int32_t> offcount(rows, 0);
+    for (int64_t r = 0; r < rows; ++r) {
+        int ndiag = 0;
+        for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+            if (indices[k] == r) {
+                if (++ndiag > 1) return;
+                std::memcpy(&diag[(size_t)r * sb], v + (size_t)k * sb, sb);
+            } else {
+                if (code(v + (size_t)k * sb) < 0) return;
+                ++offcount[r];
+            }
+        }
+    }
+    const int64_t ns = (rows + 63) / 64;
+    std::vector<int64_t> sptr(ns + 1, 0);
+    std::vector<int32_t> swidth(ns, 0);
+    for (int64_t sl = 0; sl < ns; ++sl) {
+        int32_t w = 0;
+        for (int64_t r = sl * 64; r < std::min<int64_t>(rows, (sl + 1) * 64); ++r) w = std::max(w, offcount[r]);
+        swidth[sl] = w;
+        sptr[sl + 1] = sptr[sl] + (int64_t)w * 64;
+    }
+    const int64_t total = sptr[ns];
+    std::vector<int32_t> cidx((size_t)std::max<int64_t>(total, 1));
+    std::vector<uint8_t> ccode((size_t)std::max<int64_t>(total, 1), (uint8_t)zcode);
+    for (int64_t sl = 0; sl < ns; ++sl)
+        for (int lane = 0; lane < 64; ++lane) {
+            const int64_t r = sl * 64 + lane;
+            int32_t k2 = 0;
+            if (r < rows)
+                for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+                    if (indices[k] == r) continue;
+                    const int64_t dst = sptr[sl] + (int64_t)k2 * 64 + lane;
+                    cidx[(size_t)dst] = indices[k];
+                    ccode[(size_t)dst] = (uint8_t)code(v + (size_t)k * sb);
+                    ++k2;
+                }
+            for (; k2 < swidth[sl]; ++k2) {
+                const int64_t dst = sptr[sl] + (int64_t)k2 * 64 + lane;
+                cidx[(size_t)dst] = (int32_t)std::min<int64_t>(r, rows - 1);
+                ccode[(size_t)dst] = (uint8_t)zcode;
+            }
+        }
+    std::vector<char> tab(table.size() * sb);
+    for (size_t i = 0; i < table.size(); ++i) std::memcpy(&tab[i * sb], table[i].data(), sb);
+    kpmx_ctx* ctx = m->ctx;
+    m->cslice_ptr = static_cast<int64_t*>(dev_alloc((ns + 1) * sizeof(int64_t)));
+    m->cslice_width = static_cast<int32_t*>(dev_alloc(ns * sizeof(int32_t)));
+    m->cindices = static_cast<int32_t*>(dev_alloc(cidx.size() * sizeof(int32_t)));
+    m->ccodes = static_cast<uint8_t*>(dev_alloc(ccode.size()));
+    m->cdiag = dev_alloc(diag.size());
+    m->ctable = dev_alloc(tab.size());
+    KPMX_HIP(hipMemcpyAsync(m->cslice_ptr, sptr.data(), sptr.size() * sizeof(int64_t), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(m->cslice_width, swidth.data(), swidth.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(m->cindices, cidx.data(), cidx.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(m->ccodes, ccode.data(), ccode.size(), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(m->cdiag, diag.data(), diag.size(), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(m->ctable, tab.data(), tab.size(), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    m->cmax_width = ns ? *std::max_element(swidth.begin(), swidth.end()) : 0;
+    m->ctable_size = (int)table.size();
+}
+}  // namespace
 
 int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t nnz, const int32_t* indptr,
                     const int32_t* indices, const void* values, kpmx_mat** out) {
@@ -1119,9 +1242,23 @@ int kpmx_mat_upload(kpmx_ctx* ctx, int scalar, int format, int64_t rows, int64_t
                 KPMX_HIP(hipMemcpyAsync(m->sell_vals, svals.data(), total * sb, hipMemcpyHostToDevice, ctx->stream));
             }
             KPMX_HIP(hipStreamSynchronize(ctx->stream));
+            build_coded_sell(m.get(), indptr, indices, v, sb);
         }
         KPMX_HIP(hipStreamSynchronize(ctx->stream));
         *out = m.release();
+    });
+}
+
+int kpmx_mat_coded_bytes(const kpmx_mat* m, int64_t* bytes) {
+    return guarded([&] {
+        KPMX_REQUIRE(m && bytes, "null argument");
+        *bytes = 0;
+        if (!m->ccodes) return;
+        const int64_t ns = m->nslices, sb = (int64_t)scalar_bytes(m->scalar);
+        std::vector<int64_t> cptr(2);
+        KPMX_HIP(hipMemcpy(cptr.data(), m->cslice_ptr + ns, sizeof(int64_t), hipMemcpyDeviceToHost));
+        const int64_t entries = cptr[0];
+        *bytes = (ns + 1) * 8 + ns * 4 + entries * (4 + 1) + m->rows * sb + (int64_t)m->ctable_size * sb;
     });
 }
 
@@ -1281,7 +1418,7 @@ int kpmx_moments_diagonal_noalloc(const kpmx_mat* h2, const kpmx_block* r0, int6
             any_copy = any_copy || rt.copy;
             StepLaunch L = base_launch(h2, t.vw, true);
             L.mode = MODE_DIAG;
-            L.tune = ctx->tune[h2->scalar][log2i(t.vw)];
+            L.tune = tuned_variant(h2, t.vw);
             L.num_wg = step_grid(L);
             part_step = std::max(part_step, (size_t)L.num_wg * 4 * t.vw * sizeof(double));
         }
@@ -1777,6 +1914,7 @@ int kpmx_ctx_set_step_variant(kpmx_ctx* ctx, int scalar, int vector_width, int v
                      "vector width must be a power of two up to 64");
         KPMX_REQUIRE(variant >= 0 && variant < NUM_TUNE_VARIANTS, "variant out of range");
         ctx->tune[scalar][log2i(vector_width)] = variant;
+        ctx->tune_coded[scalar][log2i(vector_width)] = -1;
     });
 }
 
@@ -1798,8 +1936,10 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
         float best_ms = 0;
         for (int v = 0; v < NUM_TUNE_VARIANTS; ++v) {
             // the column-pair variants exist for fp64 blocks of 8 or more vectors only (elsewhere
-            // they run the wave-staged kernel, already timed)
+            // they run the wave-staged kernel, already timed), the value-table ones only where the
+            // matrix has that form (elsewhere they run as the column-pair ones)
             if (v >= 2 * NUM_TILE_SHAPES * NUM_GRID_VARIANTS && !(h2->scalar == KPMX_F64 && t.vw >= 8)) continue;
+            if (v >= 3 * NUM_TILE_SHAPES * NUM_GRID_VARIANTS && !(h2->ccodes && h2->cmax_width <= 4)) continue;
             float ms_v = 0;
             for (int rep = 0; rep < 4; ++rep) {  // the first repetition warms caches and code
                 KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
@@ -1817,6 +1957,7 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
             }
         }
         ctx->tune[h2->scalar][log2i(t.vw)] = best;
+        ctx->tune_coded[h2->scalar][log2i(t.vw)] = -1;
         if (variant) *variant = best;
     });
 }

@@ -37,6 +37,16 @@ struct StepLaunch {
     int sub_y = 1;
     bool query_grid = false;  // launch_step returns the resident grid instead of launching
     int tune = 0;             // tile-shape variant of the SELL block kernel (0 default, 1 smaller, 2 larger)
+    // value-table form of the SELL matrix (off-diagonal entries as 8-bit codes into ctable, the
+    // diagonal as a dense vector); null when the matrix has none
+    const int64_t* cslice_ptr = nullptr;
+    const int32_t* cslice_width = nullptr;
+    const int32_t* cindices = nullptr;
+    const uint8_t* ccodes = nullptr;
+    const void* cdiag = nullptr;
+    const void* ctable = nullptr;
+    int cmax_width = 0;
+    int ctable_size = 0;
 };
 
 // Slice sequence of walker b of nb of the batched step (the device kernel walks it and the host
@@ -176,8 +186,9 @@ int step_grid(const StepLaunch& L);
 bool step_uses_pairs(const StepLaunch& L);
 constexpr int NUM_TILE_SHAPES = 3;     // rows per wave per tile (k_step_sellb)
 constexpr int NUM_GRID_VARIANTS = 5;   // resident grid, 1/2, 1/4, 2x, 4x
-// tile shape x grid x staging (workgroup LDS, wave shuffles, fp64 column pairs)
-constexpr int NUM_TUNE_VARIANTS = 45;
+// tile shape x grid x staging (workgroup LDS, wave shuffles, fp64 column pairs, fp64 column
+// pairs on the value-table form of the matrix)
+constexpr int NUM_TUNE_VARIANTS = 60;
 int64_t step_rows_per_tile(int format, int vw);
 // sums[s][j] = sum over workgroups of partials[s][w][j] (width doubles per workgroup); with
 // pair_cols > 0 the partials are column-pair sums of pair_cols columns and the output has the

@@ -382,9 +382,21 @@ __device__ __forceinline__ cplx<R> readlane_any(cplx<R> v, int src) {
 // and hands them to the lanes that gather with them by lane shuffles, so a tile needs no LDS
 // staging and no workgroup barrier.  Otherwise as k_step_sellb (same tile walk, software
 // pipeline of the next tile's entries, non-temporal y, wide rows from global memory).
-template <class T, int VW, int KMAX, int RW, bool OFF32>
+// The value-table form of a SELL matrix: every off-diagonal entry is an 8-bit code into `table`
+// (at most 256 distinct values, e.g. a lattice's hopping families), the diagonal a dense vector,
+// so an entry costs 5 bytes instead of 12 and the diagonal needs no index.
+template <class V>
+struct CodedArgs {
+    const uint8_t* codes = nullptr;
+    const V* diag = nullptr;
+    const V* table = nullptr;
+    int table_size = 0;
+};
+
+template <class T, int VW, int KMAX, int RW, bool OFF32, bool CODED = false>
 __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
-                                                      const int32_t* __restrict__ slice_width) {
+                                                      const int32_t* __restrict__ slice_width,
+                                                      CodedArgs<typename traits<T>::value> cd) {
     constexpr int RPW = WAVE / VW;
     constexpr int U = RW / RPW;
     constexpr int TR = RW * WAVES_PER_BLOCK;
@@ -403,6 +415,12 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
     double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
     const TileWalk tw = tile_walk(a.num_tiles);
     const int sk = lane / RW, sr = lane % RW;  // the entry this lane stages: entry sk of row sr
+    // value table of the coded form in LDS; code 256 is the zero of padded staging slots
+    __shared__ V s_tab[CODED ? 257 : 1];
+    if constexpr (CODED) {
+        for (int i = threadIdx.x; i < 257; i += BLOCK) s_tab[i] = i < cd.table_size ? cd.table[i] : zero(V{});
+        __syncthreads();
+    }
     // entry k of row u of the tile, for every lane of the row: with one row per wave iteration the
     // source lane is the same for the whole wave (a scalar read-lane); with two, each half-wave
     // picks one of two scalar read-lanes; otherwise a shuffle
@@ -430,6 +448,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
     };
     int32_t p_idx = 0;
     V p_val = zero(V{});
+    int32_t p_code = 256;
     auto prefetch = [&](int64_t tile, int64_t& pbase, int& pwidth) {
         const int64_t slc = tile / TPS;
         const int rr0 = (int)(tile % TPS) * TR;
@@ -439,11 +458,13 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
         const int rin = wave * RW + sr;
         if (lane < KMAX * RW && sk < kk) {
             p_idx = sidx[pbase + (int64_t)sk * WAVE + rr0 + rin];
-            p_val = sval[pbase + (int64_t)sk * WAVE + rr0 + rin];
+            if constexpr (CODED) p_code = cd.codes[pbase + (int64_t)sk * WAVE + rr0 + rin];
+            else p_val = sval[pbase + (int64_t)sk * WAVE + rr0 + rin];
         } else {
             const int64_t own = slc * WAVE + rr0 + rin;
             p_idx = (int32_t)(own < a.n ? own : a.n - 1);
             p_val = zero(V{});
+            p_code = 256;
         }
     };
     int64_t next_base = 0;
@@ -456,6 +477,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
         const int width = next_width;
         const int32_t c_idx = p_idx;
         const V c_val = p_val;
+        const int32_t c_code = p_code;
         int rin[U];
         int64_t row[U];
         bool ok[U];
@@ -494,14 +516,22 @@ __global__ __launch_bounds__(BLOCK) void k_step_sellw(StepArgs<T> a, const int64
         for (int u = 0; u < U; ++u) {
             acc[u] = zero(T{});
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k) acc[u] = fmac(entry_val(c_val, k, u), xg[u][k], acc[u]);
+            for (int k = 0; k < KMAX; ++k) {
+                if constexpr (CODED) acc[u] = fmac(s_tab[entry_idx(c_code, k, u)], xg[u][k], acc[u]);
+                else acc[u] = fmac(entry_val(c_val, k, u), xg[u][k], acc[u]);
+            }
         }
         for (int k = KMAX; k < width; ++k) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int64_t off = base + (int64_t)k * WAVE + r0 + rin[u];
-                acc[u] = fmac(sval[off], x[(int64_t)sidx[off] * VW + col], acc[u]);
+                const V v = CODED ? s_tab[cd.codes[off]] : sval[off];
+                acc[u] = fmac(v, x[(int64_t)sidx[off] * VW + col], acc[u]);
             }
+        }
+        if constexpr (CODED) {  // the diagonal, from the dense vector, times the row's own x
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[u] = fmac(cd.diag[ok[u] ? row[u] : a.n - 1], xo[u], acc[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1473,19 +1503,20 @@ static int launch_sellw_off(StepArgs<T> a, const StepLaunch& L, hipStream_t stre
     };
     if (L.query_grid) return grid_of();
     const int grid = L.num_wg > 0 ? L.num_wg : grid_of();
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, L.slice_ptr, L.slice_width);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, L.slice_ptr, L.slice_width,
+                       CodedArgs<typename traits<T>::value>{});
     return 0;
 }
 
 // The fp64 diagonal step on column pairs: the n x VW block read as n x VW/2 pairs of doubles, so
 // every lane moves 16 bytes and a wave instruction covers 2 (VW = 64) or more rows; the matrix
 // entries stay double.  Partial sums come out in the pair layout (see k_reduce_partials).
-template <int VW, int KMAX, int RW, bool OFF32>
+template <int VW, int KMAX, int RW, bool OFF32, bool CODED = false>
 static int launch_pairs_off(const StepArgs<double>& d, const StepLaunch& L, hipStream_t stream) {
     constexpr int VP = VW / 2;
     StepArgs<dpair> a{};
     a.indptr = d.indptr;
-    a.indices = d.indices;
+    a.indices = CODED ? L.cindices : d.indices;
     a.vals = d.vals;
     a.x = reinterpret_cast<const dpair*>(d.x);
     a.yin = reinterpret_cast<const dpair*>(d.yin);
@@ -1494,7 +1525,7 @@ static int launch_pairs_off(const StepArgs<double>& d, const StepLaunch& L, hipS
     a.n = d.n;
     a.hscale = d.hscale;
     a.sub_y = d.sub_y;
-    auto kernel = k_step_sellw<dpair, VP, KMAX, RW, OFF32>;
+    auto kernel = k_step_sellw<dpair, VP, KMAX, RW, OFF32, CODED>;
     constexpr int64_t tr = (int64_t)RW * WAVES_PER_BLOCK;
     a.num_tiles = (L.n + tr - 1) / tr;
     const int gsel = (L.tune / NUM_TILE_SHAPES) % NUM_GRID_VARIANTS;
@@ -1507,14 +1538,23 @@ static int launch_pairs_off(const StepArgs<double>& d, const StepLaunch& L, hipS
     };
     if (L.query_grid) return grid_of();
     const int grid = L.num_wg > 0 ? L.num_wg : grid_of();
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, L.slice_ptr, L.slice_width);
+    CodedArgs<double> cd;
+    if constexpr (CODED) {
+        cd.codes = L.ccodes;
+        cd.diag = static_cast<const double*>(L.cdiag);
+        cd.table = static_cast<const double*>(L.ctable);
+        cd.table_size = L.ctable_size;
+    }
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), 0, stream, a, CODED ? L.cslice_ptr : L.slice_ptr,
+                       CODED ? L.cslice_width : L.slice_width, cd);
     return 0;
 }
 
-template <int VW, int KMAX, int RW>
+template <int VW, int KMAX, int RW, bool CODED = false>
 static int launch_pairs_rw(const StepArgs<double>& d, const StepLaunch& L, hipStream_t stream) {
     const bool off32 = (uint64_t)L.n * VW * sizeof(double) < ((uint64_t)1 << 32);
-    return off32 ? launch_pairs_off<VW, KMAX, RW, true>(d, L, stream) : launch_pairs_off<VW, KMAX, RW, false>(d, L, stream);
+    return off32 ? launch_pairs_off<VW, KMAX, RW, true, CODED>(d, L, stream)
+                 : launch_pairs_off<VW, KMAX, RW, false, CODED>(d, L, stream);
 }
 
 // rows per wave per tile of the pair kernel for a tile shape (those of k_step_sellb at VW / 2)
@@ -1523,13 +1563,20 @@ constexpr int pairs_rw(int tune) {
     return sellb_rw<VW / 2>(tune);
 }
 
-template <int VW, int KMAX>
+template <int VW, int KMAX, bool CODED = false>
 static int launch_pairs_k(const StepArgs<double>& d, const StepLaunch& L, hipStream_t stream) {
     switch (L.tune % NUM_TILE_SHAPES) {
-        case 1: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(1)>(d, L, stream);
-        case 2: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(2)>(d, L, stream);
-        default: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(0)>(d, L, stream);
+        case 1: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(1), CODED>(d, L, stream);
+        case 2: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(2), CODED>(d, L, stream);
+        default: return launch_pairs_rw<VW, KMAX, pairs_rw<VW>(0), CODED>(d, L, stream);
     }
+}
+
+// Whether a launch runs the value-table (coded) form: staging 3 on an fp64 block of 8 or more
+// vectors whose matrix has that form, with its off-diagonal rows staged 4 at a time.
+static bool use_coded(const StepLaunch& L) {
+    return L.scalar == 1 && L.format == FORMAT_SELL && L.mode == MODE_DIAG && L.vw >= 8 && L.ccodes &&
+           L.cmax_width <= 4 && L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) == 3;
 }
 
 template <class T, int VW, int MODE, int KMAX, int RW, bool OFF32>
@@ -1566,9 +1613,15 @@ static int launch_sellb_rw(StepArgs<T> a, const StepLaunch& L, hipStream_t strea
 
 template <class T, int VW, int MODE, int KMAX>
 static int launch_sellb_k(StepArgs<T> a, const StepLaunch& L, hipStream_t stream) {
-    // tuned diagonal variants 30..44: fp64 blocks of 8 or more vectors as column pairs
-    if constexpr (MODE == MODE_DIAG && std::is_same<T, double>::value && pairs_fit(VW, KMAX)) {
-        if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) == 2) return launch_pairs_k<VW, KMAX>(a, L, stream);
+    // tuned diagonal variants 30..44: fp64 blocks of 8 or more vectors as column pairs; 45..59:
+    // the same on the value-table form of the matrix (as 30..44 where the matrix has none)
+    if constexpr (MODE == MODE_DIAG && std::is_same<T, double>::value && VW >= 8) {
+        if constexpr (pairs_fit(VW, 4)) {
+            if (use_coded(L)) return launch_pairs_k<VW, 4, true>(a, L, stream);
+        }
+        if constexpr (pairs_fit(VW, KMAX)) {
+            if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) >= 2) return launch_pairs_k<VW, KMAX>(a, L, stream);
+        }
     }
     // the diagonal (hot) path carries the tuned variants, the other modes use the default shape
     const int tune = (MODE == MODE_DIAG) ? L.tune % NUM_TILE_SHAPES : 0;
@@ -1665,7 +1718,8 @@ static int launch_step_t(const StepLaunch& L, hipStream_t stream) {
 // Resident grid of the SELL block kernel for this configuration (L.num_wg otherwise).
 bool step_uses_pairs(const StepLaunch& L) {
     if (!(L.scalar == 1 && L.format == FORMAT_SELL && L.mode == MODE_DIAG && L.vw >= 8)) return false;
-    if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) != 2) return false;
+    if (use_coded(L)) return true;
+    if (L.tune / (NUM_TILE_SHAPES * NUM_GRID_VARIANTS) < 2) return false;
     // the staged-entry count the launcher picks (launch_sellb): 4 for rows of at most 4 entries, 5
     // for rows of exactly 5, 8 otherwise
     const int kmax = L.max_width <= 4 ? 4 : (L.max_width == 5 ? 5 : 8);

@@ -40,9 +40,14 @@ def scaled_random(n, dtype, seed=0, density=None):
     return h2
 
 
-def graphene_h2(n1, n2, dtype, disorder=0.5):
+def graphene_h2(n1, n2, dtype, disorder=0.5, random_hoppings=False):
     h = generators.honeycomb_supercell(n1, n2, disorder=disorder, dtype=np.float64)
-    a = 0.5 * 2 * (3 * 2.8 + disorder / 2) * 1.01
+    if random_hoppings:  # every bond its own hopping (symmetric), so hundreds of distinct values
+        upper = sp.triu(h, 1).tocsr()
+        upper.data = upper.data * np.random.default_rng(n1 * n2).uniform(0.8, 1.2, upper.nnz)
+        h = (upper + upper.T + sp.diags(h.diagonal())).tocsr()
+        h.sort_indices()
+    a = 0.5 * 2 * (3 * 2.8 * 1.2 + disorder / 2) * 1.01
     return sp.csr_matrix(h * (2 / a)).astype(dtype)
 
 
@@ -92,24 +97,33 @@ def test_diagonal_block_vs_oracle(ctx, oracle, dtype, nvec, fmt):
 @pytest.mark.parametrize("dtype,nvec", [(np.float64, 64), (np.float64, 32), (np.float64, 16), (np.float64, 8),
                                          (np.complex64, 8), (np.float32, 16), (np.complex128, 4)])
 def test_every_step_variant(ctx, oracle, dtype, nvec):
-    """All 45 variants of the diagonal step kernel (tile shape x grid x LDS, wave-shuffle or
-    fp64 column-pair staging), pinned one by one, against the oracle: a graphene supercell (4
-    entries per row) and a random matrix (rows of 1 to 27 entries: the 8-entry staging plus the
-    wide-row loop)."""
+    """All 60 variants of the diagonal step kernel (tile shape x grid x LDS, wave-shuffle, fp64
+    column-pair staging, and column pairs on the value-table matrix form), pinned one by one on
+    full-width column tiles, against the oracle: a graphene supercell (4 entries per row, one
+    hopping value: the value-table form applies), a graphene supercell with random hoppings (more
+    than 256 distinct values: it does not) and a random matrix (rows of 1 to 27 entries: the
+    8-entry staging plus the wide-row loop)."""
     dtype = np.dtype(dtype)
-    for seed, h2 in ((30, graphene_h2(40, 30, dtype)), (31, scaled_random(3000, dtype, seed=31))):
-        r0 = oracle.random_starter(h2.shape[0], nvec, dtype)
-        ref = oracle.diag_moments(h2, r0, 66)
-        mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
-        try:
-            for variant in range(45):
-                ctx.set_step_variant(dtype, nvec, variant)
-                mu = abi.moments_diagonal(mat, blk, 66)
-                assert normwise(mu, ref) <= TOL[dtype], (variant, seed)
-        finally:
-            ctx.set_step_variant(dtype, nvec, 0)
-            blk.close()
-            mat.close()
+    cases = ((30, graphene_h2(40, 30, dtype)), (32, graphene_h2(36, 20, dtype, random_hoppings=True)),
+             (31, scaled_random(3000, dtype, seed=31)))
+    default_width = ctx.tile_width(dtype)
+    ctx.set_tile_width(dtype, 64)
+    try:
+        for seed, h2 in cases:
+            r0 = oracle.random_starter(h2.shape[0], nvec, dtype)
+            ref = oracle.diag_moments(h2, r0, 66)
+            mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
+            try:
+                for variant in range(60):
+                    ctx.set_step_variant(dtype, nvec, variant)
+                    mu = abi.moments_diagonal(mat, blk, 66)
+                    assert normwise(mu, ref) <= TOL[dtype], (variant, seed)
+            finally:
+                ctx.set_step_variant(dtype, nvec, 0)
+                blk.close()
+                mat.close()
+    finally:
+        ctx.set_tile_width(dtype, default_width)
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
