@@ -198,13 +198,15 @@ def cpu_baseline(cells, dtype, nvec, target_seconds=15.0):
                       % (nvec, m, t)}
 
 
-def batched_realizations(steps=2, site_order="hilbert", partition=None):
+def batched_realizations(steps=2, site_order="hilbert", partition=None, value_table=True):
     """256 square-lattice flakes (one orbital, t = -1), sizes log-uniform in [1e4, 5e5] sites
     (seed 3), Anderson W_f uniform in [0, 4] (seed 4), bounds +-(4 + W_f/2), fp64, 1024 moments,
     4 reference random vectors per flake, all flakes in one block-diagonal recurrence replayed from
     one HIP graph; each flake's sites in the Hilbert order of their positions ("hilbert") or in
     row-major order ("given").  partition: (workgroups, walk_chunk) to use instead of the
-    autotuner's choice.  Returns the GNNZ*moments/s figure of that run."""
+    autotuner's choice.  value_table: run on the value-table matrix form (1-byte codes into one
+    table of hoppings per flake, dense diagonal) rather than the standard one.  Returns the
+    GNNZ*moments/s figure of that run."""
     from pybinding_amd import generators
     from pybinding_amd.batch import kpm_batch
     t0 = time.time()
@@ -220,8 +222,11 @@ def batched_realizations(steps=2, site_order="hilbert", partition=None):
         pos.append(generators.square_positions(side, side))
     kb = kpm_batch(hs, ranges, positions=pos if site_order == "hilbert" else None)
     kb.set_random_starters(4)
+    kb.set_value_table(value_table)
     n = sum(kb.sizes)
     nnz = sum(h.nnz for h in hs)
+    nnz_diag = sum(int(np.count_nonzero(h.indices == np.repeat(np.arange(h.shape[0]), np.diff(h.indptr))))
+                   for h in hs)
     log("batched: %d flakes, %d sites, nnz %d (%.1fs setup)" % (len(hs), n, nnz, time.time() - t0))
     M = 1024
     if partition is None:
@@ -242,12 +247,18 @@ def batched_realizations(steps=2, site_order="hilbert", partition=None):
     assert np.all(np.isfinite(mu))
     kb.close()
     value = nnz * M * 4 / (ms / 1000) / 1e9
-    # algorithmic bytes per step over the whole batch (fp64, 4 vectors): nnz*12 + 4*(N+1) + 3*N*4*8
-    bytes_step = nnz * 12 + 4 * (n + 1) + 3 * n * 4 * 8
+    # algorithmic bytes per step over the whole batch (fp64, 4 vectors): the matrix (standard form:
+    # 4-byte column + 8-byte value per entry; value-table form: 4-byte column + 1-byte code per
+    # off-diagonal entry and an 8-byte diagonal per site) + 3 vector blocks of N x 4 x 8 bytes
+    if value_table:
+        bytes_step = (nnz - nnz_diag) * 5 + n * 8 + 3 * n * 4 * 8
+    else:
+        bytes_step = nnz * 12 + 4 * (n + 1) + 3 * n * 4 * 8
     gbps = bytes_step * (M // 2) / (ms / 1000) / 1e9
     log("batched: %.1f ms per run (one graph launch), %.1f GNNZ*moments/s, %.0f GB/s algorithmic" % (ms, value, gbps))
     return {"value": value, "unit": "GNNZ*moments/s", "ms_per_run": ms, "achieved_GBps": gbps, "workgroups": wg,
             "walk_chunk": chunk, "site_order": site_order,
+            "matrix_format": "value table" if value_table else "standard",
             "workload": "256 square flakes, %d sites, 1024 moments, 4 vectors each, one HIP graph" % n}
 
 

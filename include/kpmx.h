@@ -264,6 +264,18 @@ int kpmx_batch_run(kpmx_batch* b, int64_t num_moments, int use_graph);
  * Results stay within rounding of any other partition (only the grouping of each system's sums
  * changes). */
 int kpmx_batch_set_partition(kpmx_batch* b, int num_workgroups, int walk_chunk);
+/* Matrix form of the batched step (no counterpart in the reference).  The value-table form --
+ * off-diagonal entries as a 4-byte column and a 1-byte code into a table of at most 256 values
+ * per system, the diagonal as a dense vector over the rows -- is built at upload when every
+ * system's off-diagonal values fit one table and no row holds two diagonal entries, and is then
+ * the default.  enable = 0 selects the standard form (4-byte column + value per stored entry),
+ * 1 the value-table form (KPMX_EINVAL when the batch has none).  The persistent run (run mode 2)
+ * always reads the standard form.  Results agree to rounding (the diagonal term is added last). */
+int kpmx_batch_set_value_table(kpmx_batch* b, int enable);
+/* available / enabled (optional): whether the batch has / uses the value-table form;
+ * matrix_bytes (optional): bytes of the selected form that one step reads -- every stored slot
+ * (padding included) with its column and value or code, plus the diagonal and the tables. */
+int kpmx_batch_format(const kpmx_batch* b, int* available, int* enabled, int64_t* matrix_bytes);
 /* Autotuner of the batched step (no counterpart in the reference): times the step on this batch
  * and its starters over contiguous ranges for 1 to 64 workgroups per CU (four walkers, one per
  * wave, each) and over the XCD-aware walk (runs of 1, 2 or 4 slices on the resident grid and half

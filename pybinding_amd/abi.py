@@ -314,6 +314,18 @@ class Batch(_Handle):
         XCD-aware walk with runs of `walk_chunk` slices (kpmx_batch_set_partition)."""
         _check(lib().kpmx_batch_set_partition(self.h, ctypes.c_int(num_workgroups), ctypes.c_int(walk_chunk)))
 
+    def set_value_table(self, enable):
+        """Select the value-table matrix form (True) or the standard one (False) for the step
+        (kpmx_batch_set_value_table); RuntimeError when the batch has no value-table form."""
+        _check(lib().kpmx_batch_set_value_table(self.h, ctypes.c_int(1 if enable else 0)))
+
+    def format(self):
+        """(value table available, value table in use, matrix bytes one step reads) of the
+        batch (kpmx_batch_format)."""
+        avail, used, nbytes = ctypes.c_int(), ctypes.c_int(), ctypes.c_int64()
+        _check(lib().kpmx_batch_format(self.h, ctypes.byref(avail), ctypes.byref(used), ctypes.byref(nbytes)))
+        return bool(avail.value), bool(used.value), nbytes.value
+
     def fetch(self, num_moments):
         out = np.empty((len(self.sizes), self.nvec, num_moments), dtype=self.dtype)
         _check(lib().kpmx_batch_fetch(self.h, ctypes.c_int64(num_moments), _p(out)))

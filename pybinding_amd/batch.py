@@ -93,6 +93,17 @@ class KPMBatch:
         """Fix the slice partition of the batched step (see `abi.Batch.set_partition`)."""
         self.batch.set_partition(num_workgroups, walk_chunk)
 
+    def set_value_table(self, enable):
+        """Run the step on the value-table matrix form (1-byte codes into per-system tables of
+        off-diagonal values, dense diagonal; the default where every system fits it) or on the
+        standard form (see `abi.Batch.set_value_table`)."""
+        self.batch.set_value_table(enable)
+
+    @property
+    def matrix_format(self):
+        """(value table available, value table in use, matrix bytes one step reads)."""
+        return self.batch.format()
+
     def moments(self, num_moments, num_random=1, use_graph=True):
         """Undamped diagonal moments, shape (num_systems, num_random, num_moments).  use_graph:
         True (the step sequence replayed from one HIP graph), False (one launch per step) or

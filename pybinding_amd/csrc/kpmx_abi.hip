@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <atomic>
 #include <cmath>
@@ -14,6 +15,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/kpmx.h"
@@ -260,6 +262,20 @@ struct kpmx_batch {
     int32_t* slice_sys = nullptr;
     int32_t* sidx = nullptr;
     void* svals = nullptr;
+    // the value-table form (build_batch_coded): off-diagonal SELL with 1-byte codes into one table
+    // of at most 256 values per system, dense diagonal over the padded rows
+    bool has_coded = false, use_coded = false;
+    int cmax_width = 0;
+    int64_t s_entries = 0;  // stored slots of the standard form (padding included)
+    int64_t c_entries = 0;  // stored off-diagonal slots (padding included)
+    int64_t table_len = 0;
+    int64_t* cslice_ptr = nullptr;
+    int32_t* cslice_width = nullptr;
+    int32_t* cidx = nullptr;
+    uint8_t* ccodes = nullptr;
+    void* cdiag = nullptr;
+    void* ctable = nullptr;
+    int32_t* ctab_off = nullptr;
     int num_wg = 0;
     int walk_chunk = 0;  // BatchWalk of the step kernel
     int64_t num_slots = 0;
@@ -292,7 +308,8 @@ struct kpmx_batch {
         drop_graph();
         for (void* p : {(void*)slice_ptr, (void*)slice_width, (void*)slice_sys, (void*)sidx, svals, (void*)tile_begin,
                         (void*)slot_begin, (void*)slot_index, (void*)slot_first, r0, w0, w1, (void*)partials, (void*)partials2,
-                        (void*)barrier, (void*)sums})
+                        (void*)barrier, (void*)sums, (void*)cslice_ptr, (void*)cslice_width, (void*)cidx, (void*)ccodes,
+                        cdiag, ctable, (void*)ctab_off})
             dev_free(p);
     }
 };
@@ -1962,6 +1979,127 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
     });
 }
 
+namespace {
+// The value-table form of a batch (the step kernel's CODED path).  Per system, the distinct
+// off-diagonal values (compared bit for bit, code 0 is zero: padding and explicit zeros) make a
+// table of at most 256 entries; the diagonal entries go to a dense vector over the padded rows.
+// Returns without building when a system has more than 255 distinct off-diagonal values or a row
+// holds two diagonal entries.  The SELL slices are the standard form's (same slices, same
+// systems); only the off-diagonal widths change.
+void build_batch_coded(kpmx_batch* b, const int64_t* row_offsets, const int32_t* indptr, const int32_t* indices,
+                       const void* values) {
+    const size_t sb = scalar_bytes(b->scalar);
+    const int64_t S = b->num_sys, ns_tot = b->nslices;
+    const char* v = static_cast<const char*>(values);
+    struct Key {
+        uint64_t lo = 0, hi = 0;
+        bool operator==(const Key& o) const { return lo == o.lo && hi == o.hi; }
+    };
+    auto key_of = [&](int64_t k) {
+        Key key;
+        char buf[16] = {};
+        std::memcpy(buf, v + (size_t)k * sb, sb);
+        std::memcpy(&key.lo, buf, 8);
+        std::memcpy(&key.hi, buf + 8, 8);
+        return key;
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const { return std::hash<uint64_t>()(k.lo * 0x9E3779B97F4A7C15ull ^ k.hi); }
+    };
+    std::vector<int32_t> tab_off(S + 1, 0);
+    std::vector<char> table;
+    std::vector<uint8_t> code_of_entry((size_t)indptr[row_offsets[S]]);
+    std::vector<int32_t> cwidth(ns_tot, 0);
+    std::vector<int64_t> diag_entry(b->n_pad, -1);  // CSR position of every padded row's diagonal
+    for (int64_t sy = 0; sy < S; ++sy) {
+        std::unordered_map<Key, int, KeyHash> codes;
+        std::vector<Key> keys{Key{}};
+        codes.emplace(Key{}, 0);
+        Key last{};
+        int last_code = 0;
+        const int64_t shift = b->pad_off[sy] - row_offsets[sy];
+        for (int64_t r = row_offsets[sy]; r < row_offsets[sy + 1]; ++r) {
+            int32_t off = 0;
+            for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+                if (indices[k] == r) {
+                    if (diag_entry[r + shift] >= 0) return;  // two diagonal entries in one row
+                    diag_entry[r + shift] = k;
+                    continue;
+                }
+                ++off;
+                const Key key = key_of(k);
+                if (!(key == last)) {
+                    auto it = codes.find(key);
+                    if (it == codes.end()) {
+                        if (keys.size() >= 256) return;  // more values than one byte codes
+                        it = codes.emplace(key, (int)keys.size()).first;
+                        keys.push_back(key);
+                    }
+                    last = key;
+                    last_code = it->second;
+                }
+                code_of_entry[k] = (uint8_t)last_code;
+            }
+            const int64_t sl = (r + shift) / 64;
+            cwidth[sl] = std::max(cwidth[sl], off);
+        }
+        tab_off[sy + 1] = tab_off[sy] + (int32_t)keys.size();
+        for (const Key& key : keys) {
+            char buf[16];
+            std::memcpy(buf, &key.lo, 8);
+            std::memcpy(buf + 8, &key.hi, 8);
+            table.insert(table.end(), buf, buf + sb);
+        }
+    }
+    std::vector<int64_t> cptr(ns_tot + 1, 0);
+    for (int64_t sl = 0; sl < ns_tot; ++sl) cptr[sl + 1] = cptr[sl] + (int64_t)cwidth[sl] * 64;
+    const int64_t total = cptr[ns_tot];
+    std::vector<int32_t> cidx((size_t)std::max<int64_t>(total, 1));
+    std::vector<uint8_t> ccodes((size_t)std::max<int64_t>(total, 1), 0);
+    std::vector<char> diag((size_t)b->n_pad * sb, 0);
+    for (int64_t sy = 0; sy < S; ++sy) {
+        const int64_t shift = b->pad_off[sy] - row_offsets[sy];
+        for (int64_t pr = b->pad_off[sy]; pr < b->pad_off[sy + 1]; ++pr) {
+            const int64_t sl = pr / 64, lane = pr % 64, r = pr - shift;
+            int32_t k = 0;
+            if (r < row_offsets[sy + 1]) {
+                for (int32_t e = indptr[r]; e < indptr[r + 1]; ++e) {
+                    if (e == diag_entry[pr]) continue;
+                    const int64_t dst = cptr[sl] + (int64_t)k * 64 + lane;
+                    cidx[dst] = (int32_t)(indices[e] + shift);
+                    ccodes[dst] = code_of_entry[e];
+                    ++k;
+                }
+                if (diag_entry[pr] >= 0) std::memcpy(&diag[(size_t)pr * sb], v + (size_t)diag_entry[pr] * sb, sb);
+            }
+            for (; k < cwidth[sl]; ++k) cidx[cptr[sl] + (int64_t)k * 64 + lane] = (int32_t)pr;  // code 0: zero
+        }
+    }
+    auto up = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) KPMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->ctx->stream));
+    };
+    b->cslice_ptr = static_cast<int64_t*>(dev_alloc((ns_tot + 1) * sizeof(int64_t)));
+    b->cslice_width = static_cast<int32_t*>(dev_alloc(ns_tot * sizeof(int32_t)));
+    b->cidx = static_cast<int32_t*>(dev_alloc(cidx.size() * sizeof(int32_t)));
+    b->ccodes = static_cast<uint8_t*>(dev_alloc(ccodes.size()));
+    b->cdiag = dev_alloc(diag.size());
+    b->ctable = dev_alloc(table.size());
+    b->ctab_off = static_cast<int32_t*>(dev_alloc(tab_off.size() * sizeof(int32_t)));
+    up(b->cslice_ptr, cptr.data(), cptr.size() * sizeof(int64_t));
+    up(b->cslice_width, cwidth.data(), cwidth.size() * sizeof(int32_t));
+    up(b->cidx, cidx.data(), cidx.size() * sizeof(int32_t));
+    up(b->ccodes, ccodes.data(), ccodes.size());
+    up(b->cdiag, diag.data(), diag.size());
+    up(b->ctable, table.data(), table.size());
+    up(b->ctab_off, tab_off.data(), tab_off.size() * sizeof(int32_t));
+    KPMX_HIP(hipStreamSynchronize(b->ctx->stream));  // the host vectors go out of scope
+    b->cmax_width = ns_tot ? *std::max_element(cwidth.begin(), cwidth.end()) : 0;
+    b->c_entries = total;
+    b->table_len = tab_off[S];
+    b->has_coded = b->use_coded = true;
+}
+}  // namespace
+
 int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int64_t* row_offsets, int64_t nnz,
                       const int32_t* indptr, const int32_t* indices, const void* values, kpmx_batch** out) {
     return guarded([&] {
@@ -2032,6 +2170,7 @@ int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int6
             }
         }
         b->max_width = ns_tot ? *std::max_element(swidth.begin(), swidth.end()) : 0;
+        b->s_entries = total;
         b->slice_sys_host = ssys;
         auto up = [&](void* dst, const void* src, size_t bytes) {
             if (bytes) KPMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -2046,6 +2185,8 @@ int kpmx_batch_upload(kpmx_ctx* ctx, int scalar, int64_t num_systems, const int6
         up(b->slice_sys, ssys.data(), ssys.size() * sizeof(int32_t));
         up(b->sidx, sidx.data(), sidx.size() * sizeof(int32_t));
         up(b->svals, svals.data(), svals.size());
+        build_batch_coded(b.get(), row_offsets, indptr, indices, values);
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
         batch_partition(b.get(), ctx->num_cu * 4, 0);
         *out = b.release();
     });
@@ -2089,7 +2230,9 @@ int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0) {
 }
 
 namespace {
-void batch_launch(const kpmx_batch* b, BatchedLaunch& B) {
+// The step launch of a batch in its current form; `standard` asks for the standard form whatever
+// is selected (the persistent run has no value-table path).
+void batch_launch(const kpmx_batch* b, BatchedLaunch& B, bool standard = false) {
     B = BatchedLaunch{};
     B.scalar = b->scalar;
     B.vw = b->vw;
@@ -2107,6 +2250,17 @@ void batch_launch(const kpmx_batch* b, BatchedLaunch& B) {
     B.slot_index = b->slot_index;
     B.partials = b->partials;
     B.n = b->n_pad;
+    if (b->use_coded && !standard) {
+        B.max_width = b->cmax_width;
+        B.indices = b->cidx;
+        B.vals = nullptr;
+        B.slice_ptr = b->cslice_ptr;
+        B.slice_width = b->cslice_width;
+        B.codes = b->ccodes;
+        B.diag = b->cdiag;
+        B.table = b->ctable;
+        B.tab_off = b->ctab_off;
+    }
 }
 
 void batch_enqueue(kpmx_batch* b, int64_t steps) {
@@ -2141,7 +2295,7 @@ namespace {
 void batch_run_persistent(kpmx_batch* b, int64_t steps) {
     kpmx_ctx* ctx = b->ctx;
     BatchedLaunch B;
-    batch_launch(b, B);
+    batch_launch(b, B, true);
     const int per_cu = batched_persistent_blocks_per_cu(B);
     if ((int64_t)b->num_wg > (int64_t)per_cu * ctx->num_cu)
         throw Error{KPMX_EINVAL, "persistent batched run: the partition's " + std::to_string(b->num_wg) +
@@ -2227,6 +2381,34 @@ int kpmx_batch_set_partition(kpmx_batch* b, int num_workgroups, int walk_chunk) 
                      "the XCD-aware walk needs a multiple of 8 workgroups");
         KPMX_HIP(hipSetDevice(b->ctx->device));
         batch_partition(b, num_workgroups, walk_chunk);
+    });
+}
+
+int kpmx_batch_set_value_table(kpmx_batch* b, int enable) {
+    return guarded([&] {
+        KPMX_REQUIRE(b, "null batch");
+        KPMX_REQUIRE(enable == 0 || enable == 1, "enable must be 0 or 1");
+        KPMX_REQUIRE(!enable || b->has_coded,
+                     "this batch has no value-table form (a system has more than 255 distinct off-diagonal values or "
+                     "a row with two diagonal entries)");
+        if (b->use_coded != (enable == 1)) {
+            KPMX_HIP(hipSetDevice(b->ctx->device));
+            KPMX_HIP(hipStreamSynchronize(b->ctx->stream));
+            b->drop_graph();
+            b->use_coded = enable == 1;
+        }
+    });
+}
+
+int kpmx_batch_format(const kpmx_batch* b, int* available, int* enabled, int64_t* matrix_bytes) {
+    return guarded([&] {
+        KPMX_REQUIRE(b, "null batch");
+        const int64_t sb = (int64_t)scalar_bytes(b->scalar);
+        if (available) *available = b->has_coded ? 1 : 0;
+        if (enabled) *enabled = b->use_coded ? 1 : 0;
+        if (matrix_bytes)
+            *matrix_bytes = b->use_coded ? b->c_entries * 5 + b->n_pad * sb + b->table_len * sb
+                                         : b->s_entries * (4 + sb);
     });
 }
 

@@ -106,6 +106,14 @@ struct BatchedLaunch {
     int64_t n = 0;
     double hscale = 1.0;
     int sub_y = 1;
+    // the value-table form (codes != nullptr): indices / slice_ptr / slice_width / max_width
+    // describe the off-diagonal entries only, each a 1-byte code into its system's table
+    // (table + tab_off[system], at most 256 values of the value type), and the diagonal is the
+    // dense vector diag over the padded rows
+    const uint8_t* codes = nullptr;
+    const void* diag = nullptr;
+    const void* table = nullptr;
+    const int32_t* tab_off = nullptr;
     // the persistent form (launch_batched_persistent): the whole recurrence in one launch
     const void* r0 = nullptr;
     void* w0 = nullptr;

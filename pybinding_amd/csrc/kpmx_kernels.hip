@@ -615,13 +615,25 @@ __device__ __forceinline__ void wave_diag_store(double a_re, double a_im, double
 // the host numbers the stretches by replaying the walks (walker w's j-th stretch is stretch
 // slot_begin[w] + j, whose sums go to partial slot_index[stretch]); the partials of one system
 // are consecutive, in walker order, and k_reduce_segments sums them in order.
-template <class T, int VW, int KMAX, bool OFF32>
+// The value-table form of a batch: per system a table of at most 256 off-diagonal values
+// (table[tab_off[s] ...]), every off-diagonal entry a 1-byte code into its system's table, the
+// diagonal a dense vector over the padded rows.
+template <class V>
+struct CodedBatch {
+    const uint8_t* codes = nullptr;
+    const V* diag = nullptr;
+    const V* table = nullptr;
+    const int32_t* tab_off = nullptr;
+};
+
+template <class T, int VW, int KMAX, bool OFF32, bool CODED>
 __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t* __restrict__ slice_ptr,
                                              const int32_t* __restrict__ slice_width,
                                              const int32_t* __restrict__ slice_sys,
                                              const int64_t* __restrict__ tile_begin,
                                              const int64_t* __restrict__ slot_begin,
-                                             const int64_t* __restrict__ slot_index, int64_t nslices, int walk_chunk) {
+                                             const int64_t* __restrict__ slot_index, int64_t nslices, int walk_chunk,
+                                             const CodedBatch<typename traits<T>::value>& cb) {
     constexpr int RPW = WAVE / VW;  // rows per row group
     constexpr int U = VW;           // row groups per slice
     constexpr int UCM = sizeof(T) <= 4 ? 8 : sizeof(T) <= 8 ? 4 : 2;
@@ -643,17 +655,21 @@ __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t
     double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
     int32_t p_idx[KMAX];
     V p_val[KMAX];
+    V p_diag = zero(V{});
     int64_t next_base = 0;
     int next_width = 0, next_sys = 0;
     auto prefetch = [&](int64_t slc) {
         next_base = slice_ptr[slc];
         next_width = slice_width[slc];
         next_sys = slice_sys[slc];
+        const V* __restrict__ tab = CODED ? cb.table + cb.tab_off[next_sys] : nullptr;
+        if constexpr (CODED) p_diag = cb.diag[slc * WAVE + lane];
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
             if (k < next_width) {
                 p_idx[k] = sidx[next_base + (int64_t)k * WAVE + lane];
-                p_val[k] = sval[next_base + (int64_t)k * WAVE + lane];
+                if constexpr (CODED) p_val[k] = tab[cb.codes[next_base + (int64_t)k * WAVE + lane]];
+                else p_val[k] = sval[next_base + (int64_t)k * WAVE + lane];
             } else {
                 p_idx[k] = (int32_t)(slc * WAVE + lane);
                 p_val[k] = zero(V{});
@@ -683,6 +699,7 @@ __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t
         const int width = next_width;
         int32_t c_idx[KMAX];
         V c_val[KMAX];
+        const V c_diag = p_diag;
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
             c_idx[k] = p_idx[k];
@@ -713,8 +730,10 @@ __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t
                 for (int k = 0; k < KMAX; ++k) acc = fmac(shfl_any(c_val[k], rin[u]), xg[u][k], acc);
                 for (int k = KMAX; k < width; ++k) {
                     const int64_t off = base + (int64_t)k * WAVE + rin[u];
-                    acc = fmac(sval[off], x[(int64_t)sidx[off] * VW + col], acc);
+                    const V v = CODED ? cb.table[cb.tab_off[sys] + cb.codes[off]] : sval[off];
+                    acc = fmac(v, x[(int64_t)sidx[off] * VW + col], acc);
                 }
+                if constexpr (CODED) acc = fmac(shfl_any(c_diag, rin[u]), xo[u], acc);
                 T ynew = scal(acc, a.hscale);
                 if (a.sub_y) ynew = sub(ynew, yv[u]);
                 nt_store(ynew, &a.yout[e[u]]);
@@ -725,16 +744,16 @@ __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t
     wave_diag_store<T, VW>(a_re, a_im, b_re, b_im, a.partials, slot_index[slot], lane);
 }
 
-template <class T, int VW, int KMAX, bool OFF32>
+template <class T, int VW, int KMAX, bool OFF32, bool CODED>
 __global__ __launch_bounds__(BLOCK) void k_step_batched(StepArgs<T> a, const int64_t* __restrict__ slice_ptr,
                                                         const int32_t* __restrict__ slice_width,
                                                         const int32_t* __restrict__ slice_sys,
                                                         const int64_t* __restrict__ tile_begin,
                                                         const int64_t* __restrict__ slot_begin,
                                                         const int64_t* __restrict__ slot_index, int64_t nslices,
-                                                        int walk_chunk) {
-    batched_walk<T, VW, KMAX, OFF32>(a, slice_ptr, slice_width, slice_sys, tile_begin, slot_begin, slot_index, nslices,
-                                     walk_chunk);
+                                                        int walk_chunk, CodedBatch<typename traits<T>::value> cb) {
+    batched_walk<T, VW, KMAX, OFF32, CODED>(a, slice_ptr, slice_width, slice_sys, tile_begin, slot_begin, slot_index,
+                                            nslices, walk_chunk, cb);
 }
 
 // Grid barrier of a persistent kernel whose workgroups are all resident: bar[0] counts arrivals,
@@ -799,14 +818,14 @@ struct BatchPersistArgs {
 // at a grid barrier before step k + 1 reads what step k wrote.  Buffer k % 2 is next written by
 // step k + 2, after the barrier that follows step k's reduction.  The grid must be resident (the
 // launcher checks the occupancy); the barrier's bounded wait turns a violation into an error.
-template <class T, int VW, int KMAX, bool OFF32>
+template <class T, int VW, int KMAX, bool OFF32, bool CODED>
 __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchPersistArgs<T> p, const int64_t* __restrict__ slice_ptr,
                                                             const int32_t* __restrict__ slice_width,
                                                             const int32_t* __restrict__ slice_sys,
                                                             const int64_t* __restrict__ tile_begin,
                                                             const int64_t* __restrict__ slot_begin,
                                                             const int64_t* __restrict__ slot_index, int64_t nslices,
-                                                            int walk_chunk) {
+                                                            int walk_chunk, CodedBatch<typename traits<T>::value> cb) {
     __shared__ double part[BLOCK];
     const unsigned nb = gridDim.x;
     const int width = p.width;
@@ -831,8 +850,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_persistent(BatchPersistArgs<T> 
                 a.sub_y = 1;
             }
             a.partials = p.partials + (size_t)(k % 2) * p.num_slots * width;
-            batched_walk<T, VW, KMAX, OFF32>(a, slice_ptr, slice_width, slice_sys, tile_begin, slot_begin, slot_index,
-                                             nslices, walk_chunk);
+            batched_walk<T, VW, KMAX, OFF32, CODED>(a, slice_ptr, slice_width, slice_sys, tile_begin, slot_begin,
+                                                    slot_index, nslices, walk_chunk, cb);
         }
         if (k >= 1) {  // step k - 1's per-system sums, in the order of k_reduce_segments' slots
             const double* __restrict__ prev = p.partials + (size_t)((k - 1) % 2) * p.num_slots * width;
@@ -1936,6 +1955,12 @@ static void batched_t(const BatchedLaunch& B, hipStream_t stream, int* occupancy
     a.n = B.n;
     a.hscale = static_cast<typename traits<T>::real>(B.hscale);
     a.sub_y = B.sub_y;
+    using V = typename traits<T>::value;
+    CodedBatch<V> cb{};
+    cb.codes = B.codes;
+    cb.diag = static_cast<const V*>(B.diag);
+    cb.table = static_cast<const V*>(B.table);
+    cb.tab_off = B.tab_off;
     const bool off32 = (uint64_t)B.n * VW * sizeof(T) < ((uint64_t)1 << 32);
     auto go = [&](auto kernel) {
         if (occupancy) {
@@ -1943,17 +1968,25 @@ static void batched_t(const BatchedLaunch& B, hipStream_t stream, int* occupancy
             return;
         }
         hipLaunchKernelGGL(kernel, dim3(B.num_wg), dim3(BLOCK), 0, stream, a, B.slice_ptr, B.slice_width, B.slice_sys,
-                           B.tile_begin, B.slot_begin, B.slot_index, B.nslices, B.chunk);
+                           B.tile_begin, B.slot_begin, B.slot_index, B.nslices, B.chunk, cb);
     };
-    if (B.max_width <= 4) {
-        if (off32) go(k_step_batched<T, VW, 4, true>);
-        else go(k_step_batched<T, VW, 4, false>);
+    if (B.codes) {  // the value-table form: off-diagonal widths 1-4 (lattices up to 4 neighbours) or wider
+        if (B.max_width <= 4) {
+            if (off32) go(k_step_batched<T, VW, 4, true, true>);
+            else go(k_step_batched<T, VW, 4, false, true>);
+        } else {
+            if (off32) go(k_step_batched<T, VW, 8, true, true>);
+            else go(k_step_batched<T, VW, 8, false, true>);
+        }
+    } else if (B.max_width <= 4) {
+        if (off32) go(k_step_batched<T, VW, 4, true, false>);
+        else go(k_step_batched<T, VW, 4, false, false>);
     } else if (B.max_width == 5) {   // square lattices with onsite terms
-        if (off32) go(k_step_batched<T, VW, 5, true>);
-        else go(k_step_batched<T, VW, 5, false>);
+        if (off32) go(k_step_batched<T, VW, 5, true, false>);
+        else go(k_step_batched<T, VW, 5, false, false>);
     } else {
-        if (off32) go(k_step_batched<T, VW, 8, true>);
-        else go(k_step_batched<T, VW, 8, false>);
+        if (off32) go(k_step_batched<T, VW, 8, true, false>);
+        else go(k_step_batched<T, VW, 8, false, false>);
     }
 }
 
@@ -2082,12 +2115,16 @@ static int batch_persistent_t(const BatchedLaunch& B, hipStream_t stream, int* o
         }
         hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, stream, B.barrier);
         hipLaunchKernelGGL(kernel, dim3(B.num_wg), dim3(BLOCK), 0, stream, p, B.slice_ptr, B.slice_width, B.slice_sys,
-                           B.tile_begin, B.slot_begin, B.slot_index, B.nslices, B.chunk);
+                           B.tile_begin, B.slot_begin, B.slot_index, B.nslices, B.chunk,
+                           CodedBatch<typename traits<T>::value>{});
         return 0;
     };
-    if (B.max_width <= 4) return off32 ? go(k_batch_persistent<T, VW, 4, true>) : go(k_batch_persistent<T, VW, 4, false>);
-    if (B.max_width == 5) return off32 ? go(k_batch_persistent<T, VW, 5, true>) : go(k_batch_persistent<T, VW, 5, false>);
-    return off32 ? go(k_batch_persistent<T, VW, 8, true>) : go(k_batch_persistent<T, VW, 8, false>);
+    if (B.codes) return -1;  // the persistent form runs the standard matrix form only
+    if (B.max_width <= 4)
+        return off32 ? go(k_batch_persistent<T, VW, 4, true, false>) : go(k_batch_persistent<T, VW, 4, false, false>);
+    if (B.max_width == 5)
+        return off32 ? go(k_batch_persistent<T, VW, 5, true, false>) : go(k_batch_persistent<T, VW, 5, false, false>);
+    return off32 ? go(k_batch_persistent<T, VW, 8, true, false>) : go(k_batch_persistent<T, VW, 8, false, false>);
 }
 
 template <class T>

@@ -206,3 +206,67 @@ def test_batched_persistent_run(oracle, nvec, dtype, tol):
         h2 = oracle.scale_hamiltonian(h, a, b, dtype)
         ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], nvec, dtype), M)
         assert np.max(np.abs(mu[s] - ref) / np.abs(ref[:, :1])) <= tol, s
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.float32, 1e-4), (np.complex128, 1e-10)])
+def test_batched_value_table_form(oracle, dtype, tol):
+    """The value-table form (1-byte codes into one table of off-diagonal values per system, dense
+    diagonal) is built for flakes with one hopping value each (clean ones without diagonal
+    entries, disordered ones with them), is the default, reads fewer bytes than the standard form,
+    and matches it to rounding and every system the oracle, under both run modes."""
+    hs, ranges = systems(np.float64, count=12, seed=17)
+    hs[3] = generators.square_flake(30, 20, t=-1.0, disorder=0.0, dtype=np.float64)  # no onsite terms
+    ranges[3] = (-4.0, 4.0)
+    hs = [h.astype(dtype) for h in hs]
+    kb = kpm_batch(hs, ranges)
+    M = 90
+    try:
+        avail, used, coded_bytes = kb.matrix_format
+        assert avail and used
+        coded = kb.moments(M, 4)
+        coded_steps = kb.moments(M, 4, use_graph=False)
+        kb.set_value_table(False)
+        avail, used, std_bytes = kb.matrix_format
+        assert avail and not used and coded_bytes < std_bytes
+        std = kb.moments(M, 4)
+        kb.set_value_table(True)
+        again = kb.moments(M, 4)
+    finally:
+        kb.close()
+    np.testing.assert_array_equal(coded, again)
+    scale = np.abs(std[:, :, :1])
+    assert np.max(np.abs(coded - std) / scale) <= (1e-13 if tol < 1e-6 else 1e-5)
+    assert np.max(np.abs(coded_steps - coded) / scale) <= (1e-13 if tol < 1e-6 else 1e-5)
+    for s, (h, (lo, hi)) in enumerate(zip(hs, ranges)):
+        a, b = oracle.scale_factors(lo, hi)
+        h2 = oracle.scale_hamiltonian(h, a, b, dtype)
+        ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], 4, dtype), M)
+        assert np.max(np.abs(coded[s] - ref) / np.abs(ref[:, :1])) <= tol, s
+
+
+def test_batched_without_value_table(oracle):
+    """A system with more than 255 distinct off-diagonal values leaves the batch on the standard
+    form: the value-table form is not offered (selecting it raises) and the moments match the
+    oracle."""
+    hs, ranges = systems(np.float64, count=4, seed=19)
+    rng = np.random.default_rng(5)
+    h = generators.square_flake(40, 30, t=-1.0, disorder=1.0, rng=rng).tocsr()
+    off = h.indices != np.repeat(np.arange(h.shape[0]), np.diff(h.indptr))
+    h.data[off] *= 1 + 0.05 * rng.random(int(off.sum()))  # many distinct hoppings
+    hs.append(((h + h.T) * 0.5).tocsr())
+    ranges.append((-5.0, 5.0))
+    kb = kpm_batch(hs, ranges)
+    M = 60
+    try:
+        avail, used, _ = kb.matrix_format
+        assert not avail and not used
+        with pytest.raises(RuntimeError, match="no value-table form"):
+            kb.set_value_table(True)
+        mu = kb.moments(M, 2)
+    finally:
+        kb.close()
+    for s, (hh, (lo, hi)) in enumerate(zip(hs, ranges)):
+        a, b = oracle.scale_factors(lo, hi)
+        h2 = oracle.scale_hamiltonian(hh, a, b, np.float64)
+        ref = oracle.diag_moments(h2, oracle.random_starter(hh.shape[0], 2, np.float64), M)
+        assert np.max(np.abs(mu[s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
