@@ -277,9 +277,9 @@ int kpmx_batch_set_value_table(kpmx_batch* b, int enable);
  * (padding included) with its column and value or code, plus the diagonal and the tables. */
 int kpmx_batch_format(const kpmx_batch* b, int* available, int* enabled, int64_t* matrix_bytes);
 /* Autotuner of the batched step (no counterpart in the reference): times the step on this batch
- * and its starters over contiguous ranges for 1 to 64 workgroups per CU (four walkers, one per
- * wave, each) and over the XCD-aware walk (runs of 1, 2 or 4 slices on the resident grid and half
- * of it), and keeps the fastest partition.  num_workgroups, walk_chunk (optional): the chosen
+ * and its starters over contiguous ranges for 1 to 256 workgroups per CU (four walkers, one per
+ * wave, each) and over the XCD-aware walk (runs of 1, 2 or 4 slices on half, one, two and four
+ * times the resident grid), and keeps the fastest partition.  num_workgroups, walk_chunk (optional): the chosen
  * partition. */
 int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups, int* walk_chunk);
 int kpmx_batch_fetch(kpmx_batch* b, int64_t num_moments, void* mu);

@@ -2427,11 +2427,12 @@ int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups, int* walk_chunk) {
         dev_free(b->sums);
         b->sums = static_cast<double*>(dev_alloc((size_t)steps * b->num_sys * 64 * 4 * sizeof(double)));
         b->sums_steps = -1;  // the timing runs leave no result to fetch
-        // candidates: contiguous ranges over 1 ... 64 workgroups per CU (in steps of half a
-        // workgroup per CU up to 4), and the XCD-aware walk on the resident grid (and half of it)
-        // with runs of 1, 2 or 4 slices
+        // candidates: contiguous ranges over 1 ... 256 workgroups per CU (in steps of half a
+        // workgroup per CU up to 4; the grids beyond the resident one leave the balance of uneven
+        // ranges to the hardware's dispatch of workgroups as others finish), and the XCD-aware
+        // walk on half, one, two and four times the resident grid with runs of 1, 2 or 4 slices
         std::vector<std::pair<int, int>> cand;  // (workgroups, chunk)
-        for (int half_per_cu : {2, 3, 4, 5, 6, 7, 8, 12, 16, 32, 64, 128}) {
+        for (int half_per_cu : {2, 3, 4, 5, 6, 7, 8, 12, 16, 32, 64, 128, 256, 512}) {
             const int wg = (int)std::min<int64_t>((b->nslices + BATCH_WALKERS_PER_WG - 1) / BATCH_WALKERS_PER_WG,
                                                   (int64_t)ctx->num_cu * half_per_cu / 2);
             if (!cand.empty() && wg == cand.back().first) break;  // no more slices to spread
@@ -2441,8 +2442,8 @@ int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups, int* walk_chunk) {
         batch_launch(b, probe);
         const int resident = ctx->num_cu * batched_blocks_per_cu(probe);
         for (int chunk : {1, 2, 4})
-            for (int wg : {resident, resident / 2})
-                if (wg >= 8) cand.push_back({wg, chunk});
+            for (int wg : {resident / 2, resident, 2 * resident, 4 * resident})
+                if (wg >= 8 && (int64_t)wg * BATCH_WALKERS_PER_WG <= b->nslices) cand.push_back({wg, chunk});
         int best = -1;
         float best_ms = 0;
         for (size_t c = 0; c < cand.size(); ++c) {

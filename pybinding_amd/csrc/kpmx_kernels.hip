@@ -616,8 +616,11 @@ __device__ __forceinline__ void wave_diag_store(double a_re, double a_im, double
 // slot_begin[w] + j, whose sums go to partial slot_index[stretch]); the partials of one system
 // are consecutive, in walker order, and k_reduce_segments sums them in order.
 // The value-table form of a batch: per system a table of at most 256 off-diagonal values
-// (table[tab_off[s] ...]), every off-diagonal entry a 1-byte code into its system's table, the
-// diagonal a dense vector over the padded rows.
+// (table[tab_off[s] .. tab_off[s + 1]), code 0 the zero of padding), every off-diagonal entry a
+// 1-byte code into its system's table, the diagonal a dense vector over the padded rows.  Each
+// wave keeps the table of the system it sweeps in its own LDS slot (reloaded when its walk
+// reaches another system) and decodes a slice's codes there as it starts the slice, so the
+// prefetch of the next slice carries no dependent load.
 template <class V>
 struct CodedBatch {
     const uint8_t* codes = nullptr;
@@ -655,23 +658,28 @@ __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t
     double a_re = 0, a_im = 0, b_re = 0, b_im = 0;
     int32_t p_idx[KMAX];
     V p_val[KMAX];
+    uint32_t p_code[KMAX];
     V p_diag = zero(V{});
     int64_t next_base = 0;
     int next_width = 0, next_sys = 0;
+    // the wave's LDS copy of the current system's value table (value-table form only)
+    __shared__ __align__(16) unsigned char s_tab_raw[CODED ? WAVES_PER_BLOCK * 256 * sizeof(V) : 16];
+    V* const wtab = reinterpret_cast<V*>(s_tab_raw) + (CODED ? wave * 256 : 0);
+    int tab_sys = -1;
     auto prefetch = [&](int64_t slc) {
         next_base = slice_ptr[slc];
         next_width = slice_width[slc];
         next_sys = slice_sys[slc];
-        const V* __restrict__ tab = CODED ? cb.table + cb.tab_off[next_sys] : nullptr;
         if constexpr (CODED) p_diag = cb.diag[slc * WAVE + lane];
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
             if (k < next_width) {
                 p_idx[k] = sidx[next_base + (int64_t)k * WAVE + lane];
-                if constexpr (CODED) p_val[k] = tab[cb.codes[next_base + (int64_t)k * WAVE + lane]];
+                if constexpr (CODED) p_code[k] = cb.codes[next_base + (int64_t)k * WAVE + lane];
                 else p_val[k] = sval[next_base + (int64_t)k * WAVE + lane];
             } else {
                 p_idx[k] = (int32_t)(slc * WAVE + lane);
+                p_code[k] = 0;
                 p_val[k] = zero(V{});
             }
         }
@@ -695,6 +703,18 @@ __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t
             cur = sys;
             a_re = a_im = b_re = b_im = 0;
         }
+        if constexpr (CODED) {
+            if (sys != tab_sys) {  // uniform over the wave: its LDS slot takes this system's table
+                const int32_t t0 = cb.tab_off[sys], len = cb.tab_off[sys + 1] - t0;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (int j = lane; j < len; j += WAVE) wtab[j] = cb.table[t0 + j];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                tab_sys = sys;
+            }
+        }
         const int64_t base = next_base;
         const int width = next_width;
         int32_t c_idx[KMAX];
@@ -703,7 +723,8 @@ __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
             c_idx[k] = p_idx[k];
-            c_val[k] = p_val[k];
+            if constexpr (CODED) c_val[k] = wtab[p_code[k]];
+            else c_val[k] = p_val[k];
         }
 #pragma unroll
         for (int u0 = 0; u0 < U; u0 += UC) {
@@ -730,7 +751,7 @@ __device__ __forceinline__ void batched_walk(const StepArgs<T>& a, const int64_t
                 for (int k = 0; k < KMAX; ++k) acc = fmac(shfl_any(c_val[k], rin[u]), xg[u][k], acc);
                 for (int k = KMAX; k < width; ++k) {
                     const int64_t off = base + (int64_t)k * WAVE + rin[u];
-                    const V v = CODED ? cb.table[cb.tab_off[sys] + cb.codes[off]] : sval[off];
+                    const V v = CODED ? wtab[cb.codes[off]] : sval[off];
                     acc = fmac(v, x[(int64_t)sidx[off] * VW + col], acc);
                 }
                 if constexpr (CODED) acc = fmac(shfl_any(c_diag, rin[u]), xo[u], acc);

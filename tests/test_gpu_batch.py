@@ -58,7 +58,7 @@ def test_batched_rejects_coupled_systems():
 
 
 def test_batched_autotune_keeps_results(oracle):
-    """The autotuner's partitions (contiguous slice ranges over 256 ... 16384 workgroups, the
+    """The autotuner's partitions (contiguous slice ranges over 256 ... 65536 workgroups, the
     XCD-aware walk) change only how each system's sums are grouped: every system still matches the
     oracle, and the tuned run agrees with the default partition to rounding."""
     hs, ranges = systems(np.float64, count=40, seed=8)
