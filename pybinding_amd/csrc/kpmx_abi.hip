@@ -2430,7 +2430,8 @@ int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups, int* walk_chunk) {
         // candidates: contiguous ranges over 1 ... 256 workgroups per CU (in steps of half a
         // workgroup per CU up to 4; the grids beyond the resident one leave the balance of uneven
         // ranges to the hardware's dispatch of workgroups as others finish), and the XCD-aware
-        // walk on half, one, two and four times the resident grid with runs of 1, 2 or 4 slices
+        // walk on 1/2 ... 32 times the resident grid with runs of 1, 2 or 4 slices or one run per
+        // walker (contiguous ranges inside each XCD's region)
         std::vector<std::pair<int, int>> cand;  // (workgroups, chunk)
         for (int half_per_cu : {2, 3, 4, 5, 6, 7, 8, 12, 16, 32, 64, 128, 256, 512}) {
             const int wg = (int)std::min<int64_t>((b->nslices + BATCH_WALKERS_PER_WG - 1) / BATCH_WALKERS_PER_WG,
@@ -2441,9 +2442,15 @@ int kpmx_batch_autotune(kpmx_batch* b, int* num_workgroups, int* walk_chunk) {
         BatchedLaunch probe;
         batch_launch(b, probe);
         const int resident = ctx->num_cu * batched_blocks_per_cu(probe);
-        for (int chunk : {1, 2, 4})
-            for (int wg : {resident / 2, resident, 2 * resident, 4 * resident})
-                if (wg >= 8 && (int64_t)wg * BATCH_WALKERS_PER_WG <= b->nslices) cand.push_back({wg, chunk});
+        for (int64_t wg : {(int64_t)resident / 2, (int64_t)resident, 2 * (int64_t)resident, 4 * (int64_t)resident,
+                           8 * (int64_t)resident, 16 * (int64_t)resident, 32 * (int64_t)resident}) {
+            const int64_t walkers = wg * BATCH_WALKERS_PER_WG;
+            if (wg < 8 || wg % 8 != 0 || walkers > b->nslices || wg > (1 << 24)) continue;
+            const int one_run = (int)((b->nslices + walkers - 1) / walkers);
+            for (int chunk : {1, 2, 4, one_run})  // runs longer than one_run are one run too
+                if (chunk <= one_run && std::find(cand.begin(), cand.end(), std::make_pair((int)wg, chunk)) == cand.end())
+                    cand.push_back({(int)wg, chunk});
+        }
         int best = -1;
         float best_ms = 0;
         for (size_t c = 0; c < cand.size(); ++c) {
