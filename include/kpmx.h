@@ -272,6 +272,14 @@ int kpmx_batch_set_partition(kpmx_batch* b, int num_workgroups, int walk_chunk);
  * 1 the value-table form (KPMX_EINVAL when the batch has none).  The persistent run (run mode 2)
  * always reads the standard form.  Results agree to rounding (the diagonal term is added last). */
 int kpmx_batch_set_value_table(kpmx_batch* b, int enable);
+/* Column tiles of the batch's vectors (no counterpart in the reference): the next
+ * kpmx_batch_starters splits its num_vectors columns into tiles of at most `width` vectors
+ * (0: the library's default for the scalar type), each run as its own recurrence over the whole
+ * batch (the matrix is read once per tile, the walker's registers hold one tile's columns).
+ * kpmx_batch_tiles reports the current split: num_tiles tiles of tile_vectors vectors (the last
+ * may hold fewer). */
+int kpmx_batch_set_tile_width(kpmx_batch* b, int width);
+int kpmx_batch_tiles(const kpmx_batch* b, int64_t* num_tiles, int64_t* tile_vectors);
 /* available / enabled (optional): whether the batch has / uses the value-table form;
  * matrix_bytes (optional): bytes of the selected form that one step reads -- every stored slot
  * (padding included) with its column and value or code, plus the diagonal and the tables. */

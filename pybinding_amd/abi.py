@@ -314,6 +314,17 @@ class Batch(_Handle):
         XCD-aware walk with runs of `walk_chunk` slices (kpmx_batch_set_partition)."""
         _check(lib().kpmx_batch_set_partition(self.h, ctypes.c_int(num_workgroups), ctypes.c_int(walk_chunk)))
 
+    def set_tile_width(self, width):
+        """Widest column tile (vectors) for the next `starters` upload; 0 = the library default
+        (kpmx_batch_set_tile_width)."""
+        _check(lib().kpmx_batch_set_tile_width(self.h, ctypes.c_int(width)))
+
+    def tiles(self):
+        """(number of column tiles, vectors per tile) of the uploaded starters (kpmx_batch_tiles)."""
+        nt, tv = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().kpmx_batch_tiles(self.h, ctypes.byref(nt), ctypes.byref(tv)))
+        return nt.value, tv.value
+
     def set_value_table(self, enable):
         """Select the value-table matrix form (True) or the standard one (False) for the step
         (kpmx_batch_set_value_table); RuntimeError when the batch has no value-table form."""

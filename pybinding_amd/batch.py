@@ -93,6 +93,13 @@ class KPMBatch:
         """Fix the slice partition of the batched step (see `abi.Batch.set_partition`)."""
         self.batch.set_partition(num_workgroups, walk_chunk)
 
+    def set_tile_width(self, width):
+        """Run the vectors as column tiles of at most `width` vectors (0: the library default),
+        each tile its own recurrence over the whole batch; re-uploads the current starters."""
+        self.batch.set_tile_width(width)
+        if self.batch.nvec:
+            self.set_random_starters(self.batch.nvec)
+
     def set_value_table(self, enable):
         """Run the step on the value-table matrix form (1-byte codes into per-system tables of
         off-diagonal values, dense diagonal; the default where every system fits it) or on the

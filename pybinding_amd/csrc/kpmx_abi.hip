@@ -249,6 +249,12 @@ struct kpmx_block {
 
 // Batched independent systems (block-diagonal H2, one block per system, each starting on a
 // 64-row slice boundary) with their starter vectors and device workspaces.
+// Widest column tile of a batch's vectors per scalar type (f32, f64, c64, c128).  One block of
+// all vectors by default: on 256 fp64 flakes (31 M sites) 16, 32 and 64 vectors took 306, 638 and
+// 1282 ms per 256 moments in one block against 321, 642 and 1285 ms as tiles of 8 (tiles of 4:
+// 357, 715, 1430 ms); narrower tiles save work memory (two tile-wide work blocks).
+constexpr int BATCH_TILE_DEFAULT[4] = {64, 64, 64, 64};
+
 struct kpmx_batch {
     kpmx_ctx* ctx = nullptr;
     int scalar = 1;
@@ -283,9 +289,12 @@ struct kpmx_batch {
     int64_t* slot_begin = nullptr;
     int64_t* slot_index = nullptr;
     int64_t* slot_first = nullptr;
-    // starters and workspaces
+    // starters and workspaces: the vectors run as ntiles column tiles of tile_cols vectors (the
+    // last may be narrower), each a row-major n_pad x vw block of r0 and its own recurrence
     int64_t nvec = 0;
     int vw = 4;
+    int tile_cap = 0;  // widest tile (0: the default of the scalar type)
+    int64_t ntiles = 1, tile_cols = 0;
     void* r0 = nullptr;
     void* w0 = nullptr;
     void* w1 = nullptr;
@@ -2199,19 +2208,27 @@ int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0) {
         kpmx_ctx* ctx = b->ctx;
         KPMX_HIP(hipSetDevice(ctx->device));
         const size_t sb = scalar_bytes(b->scalar);
-        const int vw = std::max(4, pick_vw(num_vectors));
-        if (b->vw != vw || !b->r0) {
+        const int64_t cap = b->tile_cap > 0 ? b->tile_cap : BATCH_TILE_DEFAULT[b->scalar];
+        const int64_t tile_cols = std::min<int64_t>(num_vectors, cap);
+        const int64_t ntiles = (num_vectors + tile_cols - 1) / tile_cols;
+        const int vw = std::max(4, pick_vw(tile_cols));
+        if (b->vw != vw || b->ntiles != ntiles || !b->r0) {
             KPMX_HIP(hipStreamSynchronize(ctx->stream));
             dev_free(b->r0);
             dev_free(b->w0);
             dev_free(b->w1);
+            dev_free(b->sums);
             b->r0 = b->w0 = b->w1 = nullptr;
+            b->sums = nullptr;
+            b->sums_steps = 0;
             b->drop_graph();
-            b->r0 = dev_alloc((size_t)b->n_pad * vw * sb);
+            b->r0 = dev_alloc((size_t)ntiles * b->n_pad * vw * sb);
             b->w0 = dev_alloc((size_t)b->n_pad * vw * sb);
             b->w1 = dev_alloc((size_t)b->n_pad * vw * sb);
         }
         b->vw = vw;
+        b->ntiles = ntiles;
+        b->tile_cols = tile_cols;
         b->nvec = num_vectors;
         // padded vector-major copy on the host, then one gather into the row-major block
         const int64_t n = b->row_off.back();
@@ -2223,7 +2240,10 @@ int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0) {
                             (size_t)(b->row_off[sy + 1] - b->row_off[sy]) * sb);
         void* staging = ctx->aux0.get(padded.size());
         KPMX_HIP(hipMemcpyAsync(staging, padded.data(), padded.size(), hipMemcpyHostToDevice, ctx->stream));
-        launch_gather_block(b->scalar, staging, b->n_pad, 0, num_vectors, vw, b->r0, ctx->stream);
+        for (int64_t t = 0; t < ntiles; ++t)
+            launch_gather_block(b->scalar, staging, b->n_pad, t * tile_cols,
+                                std::min<int64_t>(tile_cols, num_vectors - t * tile_cols), vw,
+                                static_cast<char*>(b->r0) + (size_t)t * b->n_pad * vw * sb, ctx->stream);
         KPMX_HIP(hipGetLastError());
         KPMX_HIP(hipStreamSynchronize(ctx->stream));
     });
@@ -2263,14 +2283,22 @@ void batch_launch(const kpmx_batch* b, BatchedLaunch& B, bool standard = false) 
     }
 }
 
-void batch_enqueue(kpmx_batch* b, int64_t steps) {
+// Bytes of the per-system sums of a run of `steps` steps: every column tile's steps x systems x 4 vw.
+size_t batch_sums_bytes(const kpmx_batch* b, int64_t steps) {
+    return (size_t)b->ntiles * steps * b->num_sys * 4 * b->vw * sizeof(double);
+}
+
+// The recurrence of column tile t: sums of step k at sums + ((t * steps + k) * num_sys) * 4 vw.
+void batch_enqueue(kpmx_batch* b, int64_t steps, int64_t t = 0) {
     kpmx_ctx* ctx = b->ctx;
     const int width = 4 * b->vw;
+    const void* r0 = static_cast<const char*>(b->r0) + (size_t)t * b->n_pad * b->vw * scalar_bytes(b->scalar);
+    double* sums = b->sums + (size_t)t * steps * b->num_sys * width;
     BatchedLaunch B;
     batch_launch(b, B);
     for (int64_t k = 0; k < steps; ++k) {
         if (k == 0) {
-            B.x = b->r0;
+            B.x = r0;
             B.yin = nullptr;
             B.yout = b->w1;
             B.hscale = 0.5;
@@ -2279,12 +2307,12 @@ void batch_enqueue(kpmx_batch* b, int64_t steps) {
             const bool odd = (k % 2) == 1;
             B.x = odd ? b->w1 : b->w0;
             B.yout = odd ? b->w0 : b->w1;
-            B.yin = (k == 1) ? b->r0 : B.yout;
+            B.yin = (k == 1) ? r0 : B.yout;
             B.hscale = 1.0;
             B.sub_y = 1;
         }
         if (launch_batched_step(B, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported batched configuration"};
-        launch_reduce_segments(b->partials, b->slot_first, b->num_sys, width, b->sums + (size_t)k * b->num_sys * width,
+        launch_reduce_segments(b->partials, b->slot_first, b->num_sys, width, sums + (size_t)k * b->num_sys * width,
                                ctx->stream);
     }
 }
@@ -2305,24 +2333,26 @@ void batch_run_persistent(kpmx_batch* b, int64_t steps) {
     if (!b->partials2)
         b->partials2 = static_cast<double*>(dev_alloc((size_t)2 * std::max<int64_t>(b->num_slots, 1) * 256 * sizeof(double)));
     if (!b->barrier) b->barrier = static_cast<unsigned*>(dev_alloc(4 * sizeof(unsigned)));
-    B.r0 = b->r0;
     B.w0 = b->w0;
     B.w1 = b->w1;
     B.partials = b->partials2;
-    B.sums = b->sums;
     B.slot_first = b->slot_first;
     B.num_slots = b->num_slots;
     B.num_sys = b->num_sys;
     B.steps = steps;
     B.width = width;
     B.barrier = b->barrier;
-    if (launch_batched_persistent(B, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported batched configuration"};
-    KPMX_HIP(hipGetLastError());
-    unsigned flags[4] = {0, 0, 0, 0};
-    KPMX_HIP(hipMemcpyAsync(flags, b->barrier, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
-    KPMX_HIP(hipStreamSynchronize(ctx->stream));
-    if (flags[2] != 0)
-        throw Error{KPMX_EINVAL, "persistent batched run: the grid was not co-resident (barrier wait expired)"};
+    for (int64_t t = 0; t < b->ntiles; ++t) {  // one launch per column tile
+        B.r0 = static_cast<const char*>(b->r0) + (size_t)t * b->n_pad * b->vw * scalar_bytes(b->scalar);
+        B.sums = b->sums + (size_t)t * steps * b->num_sys * width;
+        if (launch_batched_persistent(B, ctx->stream) != 0) throw Error{KPMX_EINVAL, "unsupported batched configuration"};
+        KPMX_HIP(hipGetLastError());
+        unsigned flags[4] = {0, 0, 0, 0};
+        KPMX_HIP(hipMemcpyAsync(flags, b->barrier, sizeof(flags), hipMemcpyDeviceToHost, ctx->stream));
+        KPMX_HIP(hipStreamSynchronize(ctx->stream));
+        if (flags[2] != 0)
+            throw Error{KPMX_EINVAL, "persistent batched run: the grid was not co-resident (barrier wait expired)"};
+    }
 }
 }  // namespace
 
@@ -2335,22 +2365,20 @@ int kpmx_batch_run(kpmx_batch* b, int64_t M, int use_graph) {
         kpmx_ctx* ctx = b->ctx;
         KPMX_HIP(hipSetDevice(ctx->device));
         const int64_t steps = M / 2;
-        const int width = 4 * b->vw;
         if (!b->partials) b->partials = static_cast<double*>(dev_alloc((size_t)b->num_slots * 64 * 4 * sizeof(double)));
         if (b->sums_steps != steps || !b->sums) {
             KPMX_HIP(hipStreamSynchronize(ctx->stream));
             dev_free(b->sums);
-            b->sums = static_cast<double*>(dev_alloc((size_t)steps * b->num_sys * 64 * 4 * sizeof(double)));
+            b->sums = static_cast<double*>(dev_alloc(batch_sums_bytes(b, steps)));
             b->sums_steps = steps;
             b->drop_graph();
         }
-        (void)width;
         if (use_graph == 2) {
             batch_run_persistent(b, steps);
             return;
         }
         if (!use_graph) {
-            batch_enqueue(b, steps);
+            for (int64_t t = 0; t < b->ntiles; ++t) batch_enqueue(b, steps, t);
             KPMX_HIP(hipGetLastError());
             return;
         }
@@ -2358,7 +2386,7 @@ int kpmx_batch_run(kpmx_batch* b, int64_t M, int use_graph) {
             b->drop_graph();
             KPMX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
             try {
-                batch_enqueue(b, steps);
+                for (int64_t t = 0; t < b->ntiles; ++t) batch_enqueue(b, steps, t);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(ctx->stream, &g);
@@ -2381,6 +2409,23 @@ int kpmx_batch_set_partition(kpmx_batch* b, int num_workgroups, int walk_chunk) 
                      "the XCD-aware walk needs a multiple of 8 workgroups");
         KPMX_HIP(hipSetDevice(b->ctx->device));
         batch_partition(b, num_workgroups, walk_chunk);
+    });
+}
+
+int kpmx_batch_set_tile_width(kpmx_batch* b, int width) {
+    return guarded([&] {
+        KPMX_REQUIRE(b, "null batch");
+        KPMX_REQUIRE(width >= 0 && width <= 64, "tile width must be 0 (default) or 1 to 64 vectors");
+        b->tile_cap = width;
+    });
+}
+
+int kpmx_batch_tiles(const kpmx_batch* b, int64_t* num_tiles, int64_t* tile_vectors) {
+    return guarded([&] {
+        KPMX_REQUIRE(b, "null batch");
+        KPMX_REQUIRE(b->r0, "no starters uploaded");
+        if (num_tiles) *num_tiles = b->ntiles;
+        if (tile_vectors) *tile_vectors = b->tile_cols;
     });
 }
 
@@ -2487,21 +2532,25 @@ int kpmx_batch_fetch(kpmx_batch* b, int64_t M, void* mu) {
         KPMX_HIP(hipSetDevice(ctx->device));
         const int width = 4 * b->vw;
         const int64_t steps = M / 2;
-        std::vector<double> s((size_t)steps * b->num_sys * width);
+        std::vector<double> s(batch_sums_bytes(b, steps) / sizeof(double));
         KPMX_HIP(hipMemcpyAsync(s.data(), b->sums, s.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         KPMX_HIP(hipStreamSynchronize(ctx->stream));
         const bool cplx = is_complex(b->scalar);
-        // per system: the diagonal collector over that system's sums; mu is S x V x M
+        // per system and column tile: the diagonal collector over that system's sums; mu is S x V x M
         std::vector<double> per((size_t)steps * width);
         for (int64_t sy = 0; sy < b->num_sys; ++sy) {
-            for (int64_t k = 0; k < steps; ++k)
-                std::copy(&s[((size_t)k * b->num_sys + sy) * width], &s[((size_t)k * b->num_sys + sy) * width] + width,
-                          &per[(size_t)k * width]);
             const size_t off = (size_t)sy * b->nvec * M * (cplx ? 2 : 1);
-            if (b->scalar == 0 || b->scalar == 2)
-                diag_collect<float>(per, b->vw, b->nvec, M, cplx, static_cast<float*>(mu) + off, 0);
-            else
-                diag_collect<double>(per, b->vw, b->nvec, M, cplx, static_cast<double*>(mu) + off, 0);
+            for (int64_t t = 0; t < b->ntiles; ++t) {
+                const double* st = s.data() + (size_t)t * steps * b->num_sys * width;
+                for (int64_t k = 0; k < steps; ++k)
+                    std::copy(st + ((size_t)k * b->num_sys + sy) * width, st + ((size_t)k * b->num_sys + sy) * width + width,
+                              &per[(size_t)k * width]);
+                const int64_t c0 = t * b->tile_cols, cols = std::min<int64_t>(b->tile_cols, b->nvec - c0);
+                if (b->scalar == 0 || b->scalar == 2)
+                    diag_collect<float>(per, b->vw, cols, M, cplx, static_cast<float*>(mu) + off, c0);
+                else
+                    diag_collect<double>(per, b->vw, cols, M, cplx, static_cast<double*>(mu) + off, c0);
+            }
         }
     });
 }

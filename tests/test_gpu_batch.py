@@ -270,3 +270,35 @@ def test_batched_without_value_table(oracle):
         h2 = oracle.scale_hamiltonian(hh, a, b, np.float64)
         ref = oracle.diag_moments(h2, oracle.random_starter(hh.shape[0], 2, np.float64), M)
         assert np.max(np.abs(mu[s] - ref) / np.abs(ref[:, :1])) <= 1e-10, s
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-10), (np.complex128, 1e-10), (np.float32, 1e-4)])
+def test_batched_column_tiles(oracle, dtype, tol):
+    """Ten vectors per system as column tiles of 4, 4 and 2 vectors (three recurrences over the
+    batch) against one 16-wide block: the same moments to rounding under every run mode, and
+    every system's moments the oracle's."""
+    hs, ranges = systems(np.float64, count=9, seed=23)
+    hs = [h.astype(dtype) for h in hs]
+    kb = kpm_batch(hs, ranges)
+    M, R = 74, 10
+    close = 1e-13 if tol < 1e-6 else 1e-5
+    try:
+        kb.set_tile_width(64)
+        kb.set_random_starters(R)
+        assert kb.batch.tiles() == (1, 10)
+        one = kb.moments(M, R)
+        kb.set_tile_width(4)
+        assert kb.batch.tiles() == (3, 4)
+        tiled = {mode: kb.moments(M, R, use_graph=mode) for mode in (True, False)}
+        kb.set_partition(64, 0)
+        tiled["persistent"] = kb.moments(M, R, use_graph="persistent")
+    finally:
+        kb.close()
+    scale = np.abs(one[:, :, :1])
+    for mode, mu in tiled.items():
+        assert np.max(np.abs(mu - one) / scale) <= close, mode
+    for s, (h, (lo, hi)) in enumerate(zip(hs, ranges)):
+        a, b = oracle.scale_factors(lo, hi)
+        h2 = oracle.scale_hamiltonian(h, a, b, dtype)
+        ref = oracle.diag_moments(h2, oracle.random_starter(h.shape[0], R, dtype), M)
+        assert np.max(np.abs(tiled[True][s] - ref) / np.abs(ref[:, :1])) <= tol, s
