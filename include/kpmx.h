@@ -164,6 +164,15 @@ int kpmx_ctx_get_tile_width(kpmx_ctx* ctx, int scalar, int* width);
  * The moments are the same up to rounding.  get: the current tile rows. */
 int kpmx_ctx_set_two_step(kpmx_ctx* ctx, int scalar, int tile_rows, int workgroups_per_cu);
 int kpmx_ctx_get_two_step(kpmx_ctx* ctx, int scalar, int* tile_rows);
+/* The LDS form of the two-step recurrence (fp64 blocks of 8 or 16 vectors; no counterpart in the
+ * reference): tiles of 256 rows, one 64-row slice per wave; each workgroup keeps r_k+1 of its tile
+ * and of the tile's halo rows in LDS and the wave keeps r_k of its slice in registers, so a launch
+ * reads two vectors and writes two; it runs on a persistent grid of workgroups_per_cu workgroups
+ * per CU (0: as many as fit).  Reads the value-table form where the single step would.  Other
+ * scalar types and widths, and matrices whose tiles have too many halo rows for 64 KB of LDS
+ * ((256 + halo rows) x 8 bytes x vectors), keep single steps.  kpmx_ctx_set_two_step turns it off
+ * again. */
+int kpmx_ctx_set_two_step_lds(kpmx_ctx* ctx, int scalar, int workgroups_per_cu);
 
 /* Graph policy (no counterpart in the reference, whose CPU loop has no launch cost): with
  * KPMX_GRAPH_ALWAYS the diagonal recurrence of a starter block is captured into a HIP graph on its

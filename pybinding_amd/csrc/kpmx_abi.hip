@@ -149,6 +149,7 @@ struct kpmx_ctx {
     // (kpmx_ctx_set_two_step)
     int two_step[4] = {0, 0, 0, 0};
     int two_step_wg_per_cu[4] = {0, 0, 0, 0};  // persistent grid of the two-step kernel; 0: a workgroup per tile
+    int two_step_lds[4] = {0, 0, 0, 0};        // the LDS form (k_step2_lds) of the two-step recurrence
     Workspace work3, work4;  // the third and fourth vector of the two-step recurrence
     int graph_policy = KPMX_GRAPH_NEVER;
     int adjoint_variant = -1;  // adjoint step variant (kpmx_ctx_set_adjoint_variant); -1: the default
@@ -189,6 +190,11 @@ struct kpmx_mat {
     mutable int64_t halo_tile = 0, halo_ntiles = 0, halo_rows = 0;
     mutable int64_t* d_h1_ptr = nullptr;
     mutable int32_t* d_h1_rows = nullptr;
+    // the LDS two-step form's tile-local column positions (tiles of STEP2_LDS_TILE rows) of the
+    // standard SELL (d_lidx) and of the value-table SELL (d_clidx), and the most halo rows of a tile
+    mutable uint16_t* d_lidx = nullptr;
+    mutable uint16_t* d_clidx = nullptr;
+    mutable int lds_hmax = -1;
     // value-table form (SELL over the off-diagonal entries with 8-bit codes into `ctable`, the
     // diagonal as a dense vector), built at upload when the off-diagonal entries take at most 256
     // distinct values (lattices with a few hopping families); null otherwise
@@ -203,6 +209,8 @@ struct kpmx_mat {
         for (void* p : {(void*)cslice_ptr, (void*)cslice_width, (void*)cindices, (void*)ccodes, cdiag, ctable}) dev_free(p);
         dev_free(d_h1_ptr);
         dev_free(d_h1_rows);
+        dev_free(d_lidx);
+        dev_free(d_clidx);
         dev_free(d_sell_pos);
         dev_free(indptr);
         dev_free(indices);
@@ -594,23 +602,137 @@ void ensure_halo(const kpmx_mat* h2, int64_t P) {
     h2->halo_rows = (int64_t)hrows.size();
 }
 
-// Whether the diagonal recurrence of this column tile runs two steps per launch.
+// Whether the LDS two-step form can take this matrix: a value-table form the single step would
+// read (at most 4 off-diagonal entries per row) or the standard SELL.
+bool lds_coded(const kpmx_mat* h2) { return h2->ccodes && h2->cmax_width <= 4; }
+
+// Tile-local positions of every stored entry's column for the LDS two-step form (tiles of
+// STEP2_LDS_TILE rows): a column inside the tile is its offset in the tile, one outside is
+// STEP2_LDS_TILE + its index in the tile's sorted halo rows.  Built for the form the kernel reads
+// (the value-table SELL when lds_coded, else the standard SELL) from device copies, once.
+void ensure_lidx(const kpmx_mat* h2) {
+    constexpr int64_t P = STEP2_LDS_TILE;
+    ensure_halo(h2, P);
+    const bool coded = lds_coded(h2);
+    if (coded ? h2->d_clidx : h2->d_lidx) return;
+    kpmx_ctx* ctx = h2->ctx;
+    const int64_t n = h2->rows, ns = h2->nslices, ntiles = h2->halo_ntiles;
+    std::vector<int64_t> hptr(ntiles + 1), sptr(ns + 1);
+    std::vector<int32_t> hrows((size_t)std::max<int64_t>(h2->halo_rows, 1)), swidth(ns);
+    KPMX_HIP(hipMemcpyAsync(hptr.data(), h2->d_h1_ptr, hptr.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    if (h2->halo_rows)
+        KPMX_HIP(hipMemcpyAsync(hrows.data(), h2->d_h1_rows, h2->halo_rows * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    const int64_t* d_ptr = coded ? h2->cslice_ptr : h2->slice_ptr;
+    const int32_t* d_width = coded ? h2->cslice_width : h2->slice_width;
+    const int32_t* d_idx = coded ? h2->cindices : h2->sell_indices;
+    KPMX_HIP(hipMemcpyAsync(sptr.data(), d_ptr, sptr.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    KPMX_HIP(hipMemcpyAsync(swidth.data(), d_width, swidth.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    const int64_t total = sptr[ns];
+    std::vector<int32_t> idx((size_t)std::max<int64_t>(total, 1));
+    if (total)
+        KPMX_HIP(hipMemcpyAsync(idx.data(), d_idx, total * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    int hmax = 0;
+    for (int64_t j = 0; j < ntiles; ++j) hmax = std::max<int>(hmax, (int)(hptr[j + 1] - hptr[j]));
+    KPMX_REQUIRE(P + hmax < 65536, "the LDS two-step form needs fewer halo rows per tile");
+    std::vector<uint16_t> loc((size_t)std::max<int64_t>(total, 1), 0);
+    for (int64_t slc = 0; slc < ns; ++slc) {
+        const int64_t j = slc * 64 / P, t0 = j * P, t1 = std::min(n, t0 + P);
+        const int32_t* h = hrows.data() + hptr[j];
+        const int64_t nh = hptr[j + 1] - hptr[j];
+        for (int64_t e = sptr[slc]; e < sptr[slc + 1]; ++e) {
+            const int64_t c = idx[(size_t)e];
+            int64_t pos;
+            if (c >= t0 && c < t1) {
+                pos = c - t0;
+            } else {
+                const int32_t* it = std::lower_bound(h, h + nh, (int32_t)c);
+                KPMX_REQUIRE(it != h + nh && *it == c, "internal error: a column outside its tile's halo");
+                pos = P + (it - h);
+            }
+            loc[(size_t)e] = (uint16_t)pos;
+        }
+    }
+    uint16_t* d = static_cast<uint16_t*>(dev_alloc(loc.size() * sizeof(uint16_t)));
+    KPMX_HIP(hipMemcpyAsync(d, loc.data(), loc.size() * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
+    KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    (coded ? h2->d_clidx : h2->d_lidx) = d;
+    h2->lds_hmax = hmax;
+}
+
+// Whether the diagonal recurrence of this column tile runs two steps per launch; the LDS form
+// also needs the tile's rows and its largest halo to fit one workgroup's LDS (else single steps).
 bool use_two_step(const kpmx_mat* h2, const Tile& t) {
-    return h2->ctx->two_step[h2->scalar] > 0 && h2->format == KPMX_FMT_SELL && h2->step_rows.empty() &&
-           step2_supported(h2->scalar, t.vw) && h2->rows >= 2 * 64;
+    const kpmx_ctx* ctx = h2->ctx;
+    if (ctx->two_step[h2->scalar] <= 0 || h2->format != KPMX_FMT_SELL || !h2->step_rows.empty() || h2->rows < 2 * 64)
+        return false;
+    if (!ctx->two_step_lds[h2->scalar]) return step2_supported(h2->scalar, t.vw);
+    if (!step2_lds_supported(h2->scalar, t.vw)) return false;
+    ensure_lidx(h2);
+    return step2_lds_bytes(t.vw, h2->lds_hmax) <= STEP2_LDS_MAX_BYTES;
+}
+
+// The launch description of the two-step recurrence of this column tile (matrix, halo, grid);
+// vectors and partials are the caller's.
+Step2Launch two_step_plan(const kpmx_mat* h2, const Tile& t) {
+    kpmx_ctx* ctx = h2->ctx;
+    const bool lds = ctx->two_step_lds[h2->scalar] != 0;
+    const int64_t P = lds ? STEP2_LDS_TILE : ctx->two_step[h2->scalar];
+    if (lds) ensure_lidx(h2);
+    else ensure_halo(h2, P);
+    Step2Launch S;
+    S.scalar = h2->scalar;
+    S.vw = t.vw;
+    S.max_width = h2->max_width;
+    S.indices = h2->sell_indices;
+    S.vals = h2->sell_vals;
+    S.slice_ptr = h2->slice_ptr;
+    S.slice_width = h2->slice_width;
+    S.h1_ptr = h2->d_h1_ptr;
+    S.h1_rows = h2->d_h1_rows;
+    S.n = h2->rows;
+    S.ntiles = h2->halo_ntiles;
+    S.tile_rows = P;
+    if (!lds) {
+        S.num_wg = step2_grid(h2->halo_ntiles, ctx->two_step_wg_per_cu[h2->scalar] * ctx->num_cu);
+        return S;
+    }
+    S.hmax = h2->lds_hmax;
+    if (lds_coded(h2)) {
+        S.max_width = h2->cmax_width;
+        S.indices = h2->cindices;
+        S.vals = nullptr;
+        S.slice_ptr = h2->cslice_ptr;
+        S.slice_width = h2->cslice_width;
+        S.codes = h2->ccodes;
+        S.diag = h2->cdiag;
+        S.table = h2->ctable;
+        S.table_size = h2->ctable_size;
+        S.lidx = h2->d_clidx;
+    } else {
+        S.lidx = h2->d_lidx;
+    }
+    const int per_cu = step2_lds_blocks_per_cu(S);
+    KPMX_REQUIRE(per_cu > 0, "the LDS two-step form does not fit this matrix's halo (" + std::to_string(S.hmax) +
+                                 " rows per tile) in LDS");
+    const int want = ctx->two_step_wg_per_cu[h2->scalar] > 0 ? ctx->two_step_wg_per_cu[h2->scalar] : per_cu;
+    const int64_t all = (h2->halo_ntiles + 7) / 8 * 8;
+    S.num_wg = (int)std::max<int64_t>(8, std::min<int64_t>(all, (int64_t)want * ctx->num_cu / 8 * 8));
+    return S;
 }
 
 // Device memory the two-step recurrence of this tile uses (sized before a graph capture).
 void size_two_step(const kpmx_mat* h2, const Tile& t, int64_t steps) {
     kpmx_ctx* ctx = h2->ctx;
-    const int64_t P = ctx->two_step[h2->scalar];
-    ensure_halo(h2, P);
+    const Step2Launch plan = two_step_plan(h2, t);
     const size_t blk = (size_t)h2->rows * t.vw * scalar_bytes(h2->scalar);
     ctx->work0.get(blk);
     ctx->work1.get(blk);
     ctx->work3.get(blk);
     ctx->work4.get(blk);
-    const int grid = step2_grid(h2->halo_ntiles, ctx->two_step_wg_per_cu[h2->scalar] * ctx->num_cu);
+    const int grid = plan.num_wg;
     const size_t per_step = (size_t)grid * step2_partial_width(h2->scalar, t.vw) * sizeof(double);
     int64_t chunk = std::max<int64_t>(2, partials_chunk(per_step, steps));
     StepLaunch L = base_launch(h2, t.vw, true);
@@ -632,8 +754,8 @@ void run_recurrence2(const kpmx_mat* h2, const Tile& t, int64_t steps, double* s
     const size_t blk = (size_t)n * t.vw * scalar_bytes(h2->scalar);
     void* W[4] = {ctx->work0.get(blk), ctx->work1.get(blk), ctx->work3.get(blk), ctx->work4.get(blk)};
     auto r = [&](int64_t j) -> void* { return j == 0 ? t.data : W[j % 4]; };
-    const int64_t P = ctx->two_step[h2->scalar];
-    const int grid = step2_grid(h2->halo_ntiles, ctx->two_step_wg_per_cu[h2->scalar] * ctx->num_cu);
+    Step2Launch S = two_step_plan(h2, t);
+    const int grid = S.num_wg;
     const int pw = step2_partial_width(h2->scalar, t.vw);
     const int width = 4 * t.vw;
     const int pair_cols = h2->scalar == KPMX_F64 ? t.vw : 0;
@@ -641,20 +763,6 @@ void run_recurrence2(const kpmx_mat* h2, const Tile& t, int64_t steps, double* s
     int64_t chunk = std::max<int64_t>(2, partials_chunk(per_step, steps));
     chunk -= chunk % 2;
     double* partials = static_cast<double*>(ctx->partials.get(per_step * (size_t)chunk));
-    Step2Launch S;
-    S.scalar = h2->scalar;
-    S.vw = t.vw;
-    S.max_width = h2->max_width;
-    S.indices = h2->sell_indices;
-    S.vals = h2->sell_vals;
-    S.slice_ptr = h2->slice_ptr;
-    S.slice_width = h2->slice_width;
-    S.h1_ptr = h2->d_h1_ptr;
-    S.h1_rows = h2->d_h1_rows;
-    S.n = n;
-    S.ntiles = h2->halo_ntiles;
-    S.tile_rows = P;
-    S.num_wg = grid;
     int64_t k = 0, chunk_start = 0;
     for (; k + 1 < steps; k += 2) {
         S.xprev = k == 0 ? nullptr : r(k - 1);
@@ -807,6 +915,7 @@ void diag_run(const kpmx_mat* h2, const kpmx_block* r0c, int64_t M) {
             size_two_step(h2, t, steps);
             key.push_back((uint64_t)ctx->two_step[h2->scalar]);
             key.push_back((uint64_t)ctx->two_step_wg_per_cu[h2->scalar]);
+            key.push_back((uint64_t)ctx->two_step_lds[h2->scalar]);
             key.push_back((uint64_t)(uintptr_t)ctx->work3.get(work));
             key.push_back((uint64_t)(uintptr_t)ctx->work4.get(work));
             part = std::max(part, ctx->partials.bytes);
@@ -1372,6 +1481,18 @@ int kpmx_ctx_set_two_step(kpmx_ctx* ctx, int scalar, int tile_rows, int workgrou
         KPMX_REQUIRE(workgroups_per_cu >= 0 && workgroups_per_cu <= 64, "workgroups per CU must lie in [0, 64]");
         ctx->two_step[scalar] = tile_rows;
         ctx->two_step_wg_per_cu[scalar] = workgroups_per_cu;
+        ctx->two_step_lds[scalar] = 0;
+    });
+}
+
+int kpmx_ctx_set_two_step_lds(kpmx_ctx* ctx, int scalar, int workgroups_per_cu) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_REQUIRE(scalar == KPMX_F64, "the LDS two-step form is fp64 only");
+        KPMX_REQUIRE(workgroups_per_cu >= 0 && workgroups_per_cu <= 64, "workgroups per CU must lie in [0, 64]");
+        ctx->two_step[scalar] = STEP2_LDS_TILE;
+        ctx->two_step_wg_per_cu[scalar] = workgroups_per_cu;
+        ctx->two_step_lds[scalar] = 1;
     });
 }
 

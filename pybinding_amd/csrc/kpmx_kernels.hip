@@ -1066,6 +1066,200 @@ __global__ __launch_bounds__(BLOCK) void k_step2_sell(Step2Args<T> a) {
     block_reduce_store<VW>(ba_re, ba_im, bb_re, bb_im, a.partialsB);
 }
 
+// Two Chebyshev steps per launch with the intermediate vector in LDS (fp64 column pairs).
+// Tiles of STEP2_LDS_TILE rows, walked like k_step2_sell; wave w owns slice w of the tile.
+// Phase A: the wave computes r_{k+1} = s H2 r_k - r_{k-1} on its slice (stored to out1 for the
+// next launch and to its LDS rows) and keeps r_k of its rows in registers; the waves then compute
+// r_{k+1} on the tile's halo rows (row by row, from the global r_k) into the LDS rows after the
+// tile's.  After a workgroup barrier, phase B computes r_{k+2} = H2 r_{k+1} - r_k on the slice,
+// gathering r_{k+1} from LDS by the tile-local column positions `lidx` (host-built) and taking
+// r_k from registers.  A second barrier ends the tile before the next one overwrites the LDS.
+struct Step2LdsArgs {
+    const int32_t* indices;  // global columns of the matrix form read
+    const uint16_t* lidx;    // the same columns as positions in the tile's LDS rows
+    const double* vals;      // standard form
+    const uint8_t* codes;    // value-table form: codes, dense diagonal, table
+    const double* diag;
+    const double* table;
+    int table_size;
+    const int64_t* slice_ptr;
+    const int32_t* slice_width;
+    const int64_t* h1_ptr;
+    const int32_t* h1_rows;
+    const dpair* xprev;  // r_{k-1}; null on the first step
+    const dpair* x;      // r_k
+    dpair* out1;         // r_{k+1}
+    dpair* out2;         // r_{k+2}
+    double* partialsA;
+    double* partialsB;
+    int64_t n, ntiles;
+    double hscaleA;
+};
+
+template <int VW, int KMAX, bool CODED>
+__global__ __launch_bounds__(BLOCK) void k_step2_lds(Step2LdsArgs a) {
+    using T = dpair;
+    using V = double;
+    constexpr int RPW = WAVE / VW;  // rows per row group
+    constexpr int U = VW;           // row groups per slice
+    constexpr int UC = 2;           // row groups per pass
+    static_assert(U % UC == 0 && WAVES_PER_BLOCK * WAVE == STEP2_LDS_TILE, "bad LDS two-step configuration");
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    T* const s_r = reinterpret_cast<T*>(s_dyn);  // (STEP2_LDS_TILE + hmax) rows x VW
+    __shared__ V s_tab[CODED ? 257 : 1];
+    if constexpr (CODED) {
+        for (int i = threadIdx.x; i < 257; i += BLOCK) s_tab[i] = i < a.table_size ? a.table[i] : 0.0;
+        __syncthreads();
+    }
+    const int lane = threadIdx.x % WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int rsub = lane / VW;
+    const int col = lane % VW;
+    auto value = [&](int64_t off) -> V {
+        if constexpr (CODED) return s_tab[a.codes[off]];
+        else return a.vals[off];
+    };
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int64_t per_region = (a.ntiles + 7) / 8;
+    const int64_t region0 = (int64_t)(b % 8) * per_region;
+    double aa_re = 0, aa_im = 0, ab_re = 0, ab_im = 0;  // phase A sums
+    double ba_re = 0, ba_im = 0, bb_re = 0, bb_im = 0;  // phase B sums
+    for (int64_t tl = b / 8; tl < per_region && region0 + tl < a.ntiles; tl += nb / 8) {  // uniform
+        const int64_t tile = region0 + tl;
+        const int64_t t0 = tile * STEP2_LDS_TILE;
+        const int64_t tend = a.n < t0 + STEP2_LDS_TILE ? a.n : t0 + STEP2_LDS_TILE;
+        const int64_t slc = t0 / WAVE + wave;
+        const bool has_slice = slc * WAVE < tend;  // uniform over the wave
+        int32_t c_idx[KMAX];
+        V c_val[KMAX];
+        V c_diag = 0;
+        int64_t base = 0;
+        int width = 0;
+        T keep[U];
+        if (has_slice) {
+            base = a.slice_ptr[slc];
+            width = a.slice_width[slc];
+            const int64_t own = slc * WAVE + lane < a.n ? slc * WAVE + lane : a.n - 1;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                if (k < width) {
+                    c_idx[k] = a.indices[base + (int64_t)k * WAVE + lane];
+                    c_val[k] = value(base + (int64_t)k * WAVE + lane);
+                } else {
+                    c_idx[k] = (int32_t)own;
+                    c_val[k] = 0;
+                }
+            }
+            if constexpr (CODED) c_diag = a.diag[own];
+#pragma unroll
+            for (int u0 = 0; u0 < U; u0 += UC) {
+                int rin[UC];
+                int64_t e[UC];
+                bool ok[UC];
+                T yv[UC], xo[UC], xg[UC][KMAX];
+#pragma unroll
+                for (int u = 0; u < UC; ++u) {
+                    rin[u] = (u0 + u) * RPW + rsub;
+                    const int64_t row = slc * WAVE + rin[u];
+                    ok[u] = row < tend;
+                    e[u] = (ok[u] ? row : t0) * VW + col;
+                    yv[u] = a.xprev ? nt_load(&a.xprev[e[u]]) : zero(T{});
+                    xo[u] = a.x[e[u]];
+                    keep[u0 + u] = xo[u];
+                }
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+                    for (int u = 0; u < UC; ++u) xg[u][k] = a.x[(int64_t)__shfl(c_idx[k], rin[u], WAVE) * VW + col];
+#pragma unroll
+                for (int u = 0; u < UC; ++u) {
+                    T acc = zero(T{});
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k) acc = fmac(shfl_any(c_val[k], rin[u]), xg[u][k], acc);
+                    for (int k = KMAX; k < width; ++k) {
+                        const int64_t off = base + (int64_t)k * WAVE + rin[u];
+                        acc = fmac(value(off), a.x[(int64_t)a.indices[off] * VW + col], acc);
+                    }
+                    if constexpr (CODED) acc = fmac(shfl_any(c_diag, rin[u]), xo[u], acc);
+                    T ynew = scal(acc, a.hscaleA);
+                    if (a.xprev) ynew = sub(ynew, yv[u]);
+                    if (ok[u]) {
+                        nt_store(ynew, &a.out1[e[u]]);
+                        s_r[(wave * WAVE + rin[u]) * VW + col] = ynew;
+                        add_diag_sums(ynew, xo[u], aa_re, aa_im, ab_re, ab_im);
+                    }
+                }
+            }
+        }
+        // phase A on the halo rows, row by row (each lane loads its row's entries)
+        {
+            const int64_t h0 = a.h1_ptr[tile], nh = a.h1_ptr[tile + 1] - h0;
+            for (int64_t g = wave; g * RPW < nh; g += WAVES_PER_BLOCK) {
+                const int64_t i = g * RPW + rsub;
+                const bool ok = i < nh;
+                const int64_t row = ok ? (int64_t)a.h1_rows[h0 + i] : t0;
+                const int64_t hs = row >> 6;
+                const int64_t hb = a.slice_ptr[hs] + (row & 63);
+                const int hw = a.slice_width[hs];
+                T acc = zero(T{});
+                for (int k = 0; k < hw; ++k) {
+                    const int64_t off = hb + (int64_t)k * WAVE;
+                    acc = fmac(value(off), a.x[(int64_t)a.indices[off] * VW + col], acc);
+                }
+                if constexpr (CODED) acc = fmac(a.diag[row], a.x[row * VW + col], acc);
+                T ynew = scal(acc, a.hscaleA);
+                if (a.xprev) ynew = sub(ynew, a.xprev[row * VW + col]);
+                if (ok) s_r[(STEP2_LDS_TILE + i) * VW + col] = ynew;
+            }
+        }
+        __syncthreads();
+        if (has_slice) {
+            int32_t c_loc[KMAX];
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k)
+                c_loc[k] = k < width ? (int32_t)a.lidx[base + (int64_t)k * WAVE + lane] : wave * WAVE + lane;
+#pragma unroll
+            for (int u0 = 0; u0 < U; u0 += UC) {
+                int rin[UC];
+                int64_t e[UC];
+                bool ok[UC];
+                T own[UC], xg[UC][KMAX];
+#pragma unroll
+                for (int u = 0; u < UC; ++u) {
+                    rin[u] = (u0 + u) * RPW + rsub;
+                    const int64_t row = slc * WAVE + rin[u];
+                    ok[u] = row < tend;
+                    e[u] = (ok[u] ? row : t0) * VW + col;
+                    own[u] = s_r[(wave * WAVE + rin[u]) * VW + col];
+                }
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+#pragma unroll
+                    for (int u = 0; u < UC; ++u) xg[u][k] = s_r[__shfl(c_loc[k], rin[u], WAVE) * VW + col];
+#pragma unroll
+                for (int u = 0; u < UC; ++u) {
+                    T acc = zero(T{});
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k) acc = fmac(shfl_any(c_val[k], rin[u]), xg[u][k], acc);
+                    for (int k = KMAX; k < width; ++k) {
+                        const int64_t off = base + (int64_t)k * WAVE + rin[u];
+                        acc = fmac(value(off), s_r[(int)a.lidx[off] * VW + col], acc);
+                    }
+                    if constexpr (CODED) acc = fmac(shfl_any(c_diag, rin[u]), own[u], acc);
+                    const T ynew = sub(acc, keep[u0 + u]);
+                    if (ok[u]) {
+                        nt_store(ynew, &a.out2[e[u]]);
+                        add_diag_sums(ynew, own[u], ba_re, ba_im, bb_re, bb_im);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    block_reduce_store<VW>(aa_re, aa_im, ab_re, ab_im, a.partialsA);
+    block_reduce_store<VW>(ba_re, ba_im, bb_re, bb_im, a.partialsB);
+}
+
 // sums[s][j] = sum over the slots [slot_first[s], slot_first[s+1]) of partials[slot][j]: one
 // workgroup of RS_THREADS per system; thread t sums column j = t % width of the slots g, g + G,
 // g + 2G, ... (g = t / width, G = RS_THREADS / width groups, loads unrolled four deep), then the
@@ -2085,7 +2279,13 @@ int step2_grid(int64_t ntiles, int num_wg) {
 
 int step2_partial_width(int scalar, int vw) { return scalar == 1 ? 2 * vw : 4 * vw; }
 
+static int step2_lds_dispatch(const Step2Launch& S, hipStream_t stream, int* occupancy);
+
 int launch_step2(const Step2Launch& S, hipStream_t stream) {
+    if (S.lidx) {  // the LDS form: a persistent grid that is a multiple of 8, tiles of STEP2_LDS_TILE rows
+        if (S.ntiles < 1 || S.tile_rows != STEP2_LDS_TILE || S.num_wg < 8 || S.num_wg % 8 != 0) return -1;
+        return step2_lds_dispatch(S, stream, nullptr);
+    }
     if (!step2_supported(S.scalar, S.vw) || S.ntiles < 1 || S.tile_rows < 1) return -1;
     switch (S.scalar) {
         case 0:
@@ -2109,6 +2309,59 @@ int launch_step2(const Step2Launch& S, hipStream_t stream) {
             else step2_t<cf64, 64>(S, stream);
             return 0;
     }
+}
+
+bool step2_lds_supported(int scalar, int vw) { return scalar == 1 && (vw == 8 || vw == 16); }
+
+size_t step2_lds_bytes(int vw, int hmax) { return (size_t)(STEP2_LDS_TILE + hmax) * vw * sizeof(double); }
+
+template <int VW>
+static int step2_lds_t(const Step2Launch& S, hipStream_t stream, int* occupancy) {
+    Step2LdsArgs a{};
+    a.indices = S.indices;
+    a.lidx = S.lidx;
+    a.vals = static_cast<const double*>(S.vals);
+    a.codes = S.codes;
+    a.diag = static_cast<const double*>(S.diag);
+    a.table = static_cast<const double*>(S.table);
+    a.table_size = S.table_size;
+    a.slice_ptr = S.slice_ptr;
+    a.slice_width = S.slice_width;
+    a.h1_ptr = S.h1_ptr;
+    a.h1_rows = S.h1_rows;
+    a.xprev = static_cast<const dpair*>(S.xprev);
+    a.x = static_cast<const dpair*>(S.x);
+    a.out1 = static_cast<dpair*>(S.out1);
+    a.out2 = static_cast<dpair*>(S.out2);
+    a.partialsA = S.partialsA;
+    a.partialsB = S.partialsB;
+    a.n = S.n;
+    a.ntiles = S.ntiles;
+    a.hscaleA = S.hscaleA;
+    const size_t lds = step2_lds_bytes(2 * VW, S.hmax);
+    auto go = [&](auto kernel) -> int {
+        if (occupancy) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(occupancy, kernel, BLOCK, lds) != hipSuccess) *occupancy = 0;
+            return 0;
+        }
+        hipLaunchKernelGGL(kernel, dim3(S.num_wg), dim3(BLOCK), lds, stream, a);
+        return 0;
+    };
+    const bool coded = S.codes != nullptr;
+    if (S.max_width <= 4) return coded ? go(k_step2_lds<VW, 4, true>) : go(k_step2_lds<VW, 4, false>);
+    return coded ? go(k_step2_lds<VW, 8, true>) : go(k_step2_lds<VW, 8, false>);
+}
+
+static int step2_lds_dispatch(const Step2Launch& S, hipStream_t stream, int* occupancy) {
+    if (!step2_lds_supported(S.scalar, S.vw) || S.hmax < 0 || step2_lds_bytes(S.vw, S.hmax) > STEP2_LDS_MAX_BYTES)
+        return -1;
+    return S.vw == 8 ? step2_lds_t<4>(S, stream, occupancy) : step2_lds_t<8>(S, stream, occupancy);
+}
+
+int step2_lds_blocks_per_cu(const Step2Launch& S) {
+    int per_cu = 0;
+    if (step2_lds_dispatch(S, nullptr, &per_cu) != 0) return 0;
+    return per_cu;
 }
 
 template <class T, int VW>

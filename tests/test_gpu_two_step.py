@@ -102,3 +102,92 @@ def test_two_step_off_for_unsupported_blocks(oracle):
             ctx.set_two_step(np.float64, 128, 65)
     finally:
         ctx.close()
+
+
+def random_hoppings(h2, seed=3):
+    """The same pattern with every off-diagonal value perturbed (no value-table form)."""
+    h = h2.tocsr(copy=True)
+    rng = np.random.default_rng(seed)
+    off = h.indices != np.repeat(np.arange(h.shape[0]), np.diff(h.indptr))
+    h.data[off] *= 1 + 0.05 * rng.random(int(off.sum()))
+    h = ((h + h.T) * 0.5).tocsr()
+    h.sort_indices()
+    return h
+
+
+@pytest.mark.parametrize("shape", [(40, 28), (21, 13)])
+@pytest.mark.parametrize("coded", [True, False])
+@pytest.mark.parametrize("nvec", [16, 8, 12, 20])
+@pytest.mark.parametrize("M", [130, 132])
+def test_two_step_lds_matches_oracle(oracle, shape, coded, nvec, M):
+    """The LDS form (kpmx_ctx_set_two_step_lds): 256-row tiles with the intermediate vector in LDS,
+    on the value-table and the standard matrix form, tiles that end inside the matrix's last slice
+    or leave waves without a slice, 8 and 16 columns (20 vectors: a 16-column tile in the LDS form
+    and a 4-column one on single steps), odd and even step counts, the resident grid and one
+    workgroup per CU: within rounding of single steps and of the oracle."""
+    h2 = matrix(np.float64, *shape)
+    if not coded:
+        h2 = random_hoppings(h2)
+    r0 = oracle.random_starter(h2.shape[0], nvec, np.float64)
+    ref = oracle.diag_moments(h2, r0, M)
+    ctx = abi.Ctx(0)
+    try:
+        mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
+        assert (mat.coded_bytes() > 0) == coded
+        single = abi.moments_diagonal(mat, blk, M)
+        out = {}
+        for wg in (0, 1):
+            ctx.set_two_step_lds(wg)
+            assert ctx.two_step(np.float64) == 256
+            out[wg] = abi.moments_diagonal(mat, blk, M)
+        ctx.set_two_step(np.float64, 0)
+        np.testing.assert_array_equal(abi.moments_diagonal(mat, blk, M), single)
+        blk.close()
+        mat.close()
+    finally:
+        ctx.close()
+    scale = np.abs(ref[:, :1])
+    assert np.max(np.abs(single - ref) / scale) <= 1e-10
+    for wg, mu in out.items():
+        assert np.max(np.abs(mu - ref) / scale) <= 1e-10, wg
+        assert np.max(np.abs(mu - single) / scale) <= 1e-13, wg
+
+
+def test_two_step_lds_bfs_order_and_graph_replay(oracle):
+    """Far halos (row-major order of a periodic supercell: about 300 halo rows per 256-row tile)
+    in the LDS form with 8 columns, replayed from a HIP graph several times; with 16 columns the
+    halo does not fit 64 KB of LDS and the recurrence keeps single steps (the single-step result
+    bit for bit); other scalar types keep single steps."""
+    h2 = matrix(np.float64, 48, 32, seed=9, hilbert=False)
+    r0 = oracle.random_starter(h2.shape[0], 8, np.float64)
+    M = 98
+    ref = oracle.diag_moments(h2, r0, M)
+    r16 = oracle.random_starter(h2.shape[0], 16, np.float64)
+    ctx = abi.Ctx(0)
+    try:
+        mat, blk, b16 = ctx.mat(h2, fmt=1), ctx.block(r0), ctx.block(r16)
+        single16 = abi.moments_diagonal(mat, b16, M)
+        ctx.set_two_step_lds()
+        ctx.set_graph_policy(2)
+        runs = [abi.moments_diagonal(mat, blk, M) for _ in range(3)]
+        np.testing.assert_array_equal(abi.moments_diagonal(mat, b16, M), single16)
+        b16.close()
+        blk.close()
+        mat.close()
+        h32 = h2.astype(np.float32)
+        r32 = oracle.random_starter(h2.shape[0], 32, np.float32)
+        m32, b32 = ctx.mat(h32, fmt=1), ctx.block(r32)
+        mu32 = abi.moments_diagonal(m32, b32, M)
+        b32.close()
+        m32.close()
+        import ctypes
+        from pybinding_amd import abi as pabi
+        with pytest.raises(RuntimeError, match="fp64 only"):
+            pabi._check(pabi.lib().kpmx_ctx_set_two_step_lds(ctx.h, ctypes.c_int(0), ctypes.c_int(0)))
+    finally:
+        ctx.close()
+    for mu in runs:
+        np.testing.assert_array_equal(mu, runs[0])
+    assert np.max(np.abs(runs[0] - ref) / np.abs(ref[:, :1])) <= 1e-10
+    ref32 = oracle.diag_moments(h32, r32, M)
+    assert np.max(np.abs(mu32 - ref32) / np.abs(ref32[:, :1])) <= 1e-4
