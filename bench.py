@@ -14,8 +14,10 @@ bracketed by stream synchronisation.  value = GNNZ*moments/s = nnz * M * R / t /
 
 roofline: the fused step kernel's algorithmic bytes per launch
     B_step = nnz*(s + 4) + 4*(N + 1) + 3*N*B*s
-divided by its average launch duration (HIP events on the engine's stream around the timed
-region / number of step launches), against 8 TB/s.
+(the matrix term in the form the kernel reads: the value-table form's bytes where it reads that)
+times the steps one launch does (two in the two-step form, which itself moves fewer bytes: the
+counted traffic) divided by its average launch duration (HIP events on the engine's stream around
+the timed region / number of launches), against 8 TB/s.
 
 cpu_baseline: the oracle's restatement of the reference's default CPU compute (ELL, 4-wide fp64
 vector batches, threads over batches, interleaved BFS slices) on a bounded sample of the same
@@ -574,6 +576,10 @@ def main():
     ap.add_argument("--no-ldos", action="store_true", help="skip the LDOS light-cone run")
     ap.add_argument("--no-model", action="store_true",
                     help="skip the complex128 run of the workload built by the model builder")
+    ap.add_argument("--two-step", default="auto",
+                    help="'auto' (the autotuner decides), 'off' (single steps), or 'W:C' to pin the LDS two-step "
+                         "form with column tiles of W vectors on C workgroups per CU (e.g. 8:4); used to profile "
+                         "one configuration")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 2 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -617,12 +623,20 @@ def main():
     t_draw = time.time() - t_draw
 
     ctx = abi.Ctx(0)
-    tile_width = ctx.tile_width(dtype)
     mat = ctx.mat(h2, fmt=1 if args.format == "SELL" else 0)
     blk = ctx.block(r0)
     ctx.synchronize()
+    if args.two_step != "auto":
+        ctx.set_autotune_two_step(False)
     variant = abi.autotune_diagonal(mat, blk)
-    log("autotuned step kernel variant: %d (column tiles of %d vectors)" % (variant, tile_width))
+    if args.two_step not in ("auto", "off"):
+        w, c = (int(x) for x in args.two_step.split(":"))
+        ctx.set_tile_width(dtype, w)
+        ctx.set_two_step_lds(c)
+    tile_width = ctx.tile_width(dtype)
+    two_step, two_step_wg = ctx.two_step_lds(dtype)
+    log("autotuned step kernel variant: %d (column tiles of %d vectors)%s"
+        % (variant, tile_width, ", LDS two-step form on %d workgroups per CU" % two_step_wg if two_step else ""))
 
     for _ in range(args.warmup):
         abi.moments_diagonal_run(mat, blk, M)
@@ -641,28 +655,34 @@ def main():
     ms_per_step = wall * 1000 / args.steps
     value = nnz * M * R / (wall / args.steps) / 1e9
 
-    # fused step launches per moment computation: one per two moments and column tile
+    # fused step launches per moment computation: one per two moments and column tile (one per
+    # four moments in the two-step form, each reading r_k-1 and r_k and writing r_k+1 and r_k+2)
     widths = launch_widths(R, tile_width)
     steps_per_run = M // 2
-    step_ms = gpu_ms / (args.steps * steps_per_run * len(widths))      # one launch
-    # the variants 45..59 read the matrix's value-table form (8-bit codes of the off-diagonal
-    # entries, the diagonal dense): its bytes replace nnz (s + 4) + 4 (N + 1) in a launch's count
-    coded = mat.coded_bytes() if variant >= 45 else 0
+    steps_per_launch = 2 if two_step else 1
+    step_ms = gpu_ms / (args.steps * steps_per_run // steps_per_launch * len(widths))      # one launch
+    # the variants 45..59 and the two-step form read the matrix's value-table form (8-bit codes of
+    # the off-diagonal entries, the diagonal dense) when it has one: its bytes replace
+    # nnz (s + 4) + 4 (N + 1) in a launch's count
+    coded = mat.coded_bytes() if (variant >= 45 or two_step) else 0
     matrix_bytes = coded if coded else nnz * (s + 4) + 4 * (n + 1)
-    bytes_per_launch = sum(matrix_bytes + 3 * n * w * s for w in widths) / len(widths)
+    # algorithmic bytes of the steps one launch does (a two-step launch moves fewer: it reads and
+    # writes 4 vector blocks instead of 6, which the counted traffic shows)
+    bytes_per_launch = sum(steps_per_launch * (matrix_bytes + 3 * n * w * s) for w in widths) / len(widths)
     achieved = bytes_per_launch / (step_ms / 1000)
     # the same step over all R vectors as one pass (the matrix read once): the algorithmic bytes of
     # the whole step over the time of its launches
     pass_ms = gpu_ms / (args.steps * steps_per_run)
     vw_all = min(64, 1 << (max(R, 1) - 1).bit_length())
     pass_gbps = step_bytes(n, nnz, vw_all, s) * ((R + 63) // 64) / (pass_ms / 1000) / 1e9
-    log("GPU: %.1f ms per moment computation, %d launches of %s vectors per step, %.4f ms per launch, "
+    log("GPU: %.1f ms per moment computation, %d launches of %s vectors per %d step(s), %.4f ms per launch, "
         "%.1f GB/s algorithmic per launch; %.4f ms per step over all %d vectors = %.1f GB/s of one-pass bytes"
-        % (gpu_ms / args.steps, len(widths), "/".join(str(w) for w in sorted(set(widths))), step_ms,
+        % (gpu_ms / args.steps, len(widths), "/".join(str(w) for w in sorted(set(widths))), steps_per_launch, step_ms,
            achieved / 1e9, pass_ms, R, pass_gbps))
 
-    traffic_key = "%dx%d %s v%d launch%d %s%s %s" % (args.cells[0], args.cells[1], dtype.name, R, max(widths), args.format,
-                                                     " coded" if coded else "", args.order)
+    traffic_key = "%dx%d %s v%d launch%d %s%s %s%s" % (args.cells[0], args.cells[1], dtype.name, R, max(widths),
+                                                       args.format, " coded" if coded else "", args.order,
+                                                       " two-step" if two_step else "")
     extra = {"starter_draw_s": t_draw,
              "one_pass_equivalent": {"ms_per_step": pass_ms, "achieved_GBps": pass_gbps,
                                      "frac": pass_gbps * 1e9 / HBM_PEAK, "launch_widths": widths}}
@@ -738,7 +758,9 @@ def main():
                                % (args.cells[0], args.cells[1], n, nnz, M, R, tile_width),
                    "matrix_format": args.format + (" (value-table form, %d bytes)" % coded if coded else ""),
                    "site_order": args.order, "kernel_variant": variant,
-                   "moments": M, "vectors": R, "column_tile": tile_width},
+                   "moments": M, "vectors": R, "column_tile": tile_width,
+                   "steps_per_launch": steps_per_launch,
+                   "two_step": {"form": "LDS", "workgroups_per_cu": two_step_wg} if two_step else None},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": counted_traffic(traffic_key)},
         "cpu_baseline": cpu,

@@ -132,7 +132,9 @@ int kpmx_moments_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_
  * per lane, for other blocks run as 1; 3 column pairs on the matrix's value-table form: an SELL
  * of the off-diagonal entries with 8-bit codes into a table of at most 256 values and the
  * diagonal as a dense vector, built at upload when the off-diagonal entries take that few
- * distinct values, for other matrices run as 2). */
+ * distinct values, for other matrices run as 2).  On fp64 SELL matrices it then also times the
+ * LDS two-step form (see kpmx_ctx_set_autotune_two_step) and keeps it, with its column-tile
+ * width, when it is faster per vector. */
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant);
 /* Pin the diagonal step variant (as numbered above) for one scalar type and vector width on the
  * context, e.g. a choice an earlier autotuning run reported (no counterpart in the reference). */
@@ -173,6 +175,16 @@ int kpmx_ctx_get_two_step(kpmx_ctx* ctx, int scalar, int* tile_rows);
  * ((256 + halo rows) x 8 bytes x vectors), keep single steps.  kpmx_ctx_set_two_step turns it off
  * again. */
 int kpmx_ctx_set_two_step_lds(kpmx_ctx* ctx, int scalar, int workgroups_per_cu);
+/* enabled: whether the LDS two-step form is on for the scalar type; workgroups_per_cu: its grid
+ * (0: as many as fit). */
+int kpmx_ctx_get_two_step_lds(kpmx_ctx* ctx, int scalar, int* enabled, int* workgroups_per_cu);
+/* Whether kpmx_autotune_diagonal may also choose the LDS two-step form (default 1): for fp64
+ * SELL matrices it then times column tiles of 8 vectors with 4 workgroups per CU and of 16 vectors
+ * with 3 against the single steps per vector and keeps the two-step form, with its tile width,
+ * when it is at least 2 % faster.  Settings chosen this way are reset by the next autotuning;
+ * settings made with kpmx_ctx_set_tile_width / kpmx_ctx_set_two_step / kpmx_ctx_set_two_step_lds
+ * are the caller's and are only changed when the autotuner picks the two-step form. */
+int kpmx_ctx_set_autotune_two_step(kpmx_ctx* ctx, int enable);
 
 /* Graph policy (no counterpart in the reference, whose CPU loop has no launch cost): with
  * KPMX_GRAPH_ALWAYS the diagonal recurrence of a starter block is captured into a HIP graph on its

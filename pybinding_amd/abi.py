@@ -139,6 +139,17 @@ class Ctx(_Handle):
         _check(lib().kpmx_ctx_set_two_step_lds(self.h, ctypes.c_int(SCALAR[np.dtype(np.float64)]),
                                                ctypes.c_int(workgroups_per_cu)))
 
+    def two_step_lds(self, dtype=np.float64):
+        """(enabled, workgroups per CU) of the LDS two-step form (kpmx_ctx_get_two_step_lds)."""
+        e, w = ctypes.c_int(), ctypes.c_int()
+        _check(lib().kpmx_ctx_get_two_step_lds(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.byref(e),
+                                               ctypes.byref(w)))
+        return bool(e.value), w.value
+
+    def set_autotune_two_step(self, enable):
+        """Let the diagonal autotuner choose the LDS two-step form (kpmx_ctx_set_autotune_two_step)."""
+        _check(lib().kpmx_ctx_set_autotune_two_step(self.h, ctypes.c_int(1 if enable else 0)))
+
     def two_step(self, dtype):
         p = ctypes.c_int()
         _check(lib().kpmx_ctx_get_two_step(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.byref(p)))

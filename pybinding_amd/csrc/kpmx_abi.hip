@@ -130,6 +130,11 @@ void* scratch_address(int device) {
 }
 }  // namespace
 
+// Widest column tile of the diagonal recurrence by default: fp64 recurrences run in tiles of 16
+// vectors (on the 4.19 M-site workload four 16-column recurrences take 1.25 ms per step of 64
+// vectors, one 64-column recurrence 1.29 ms), the other types in one tile of up to 64.
+int default_tile_cap(int scalar) { return scalar == KPMX_F64 ? 16 : 64; }
+
 struct kpmx_ctx {
     int device = 0;
     int num_cu = 256;
@@ -150,6 +155,8 @@ struct kpmx_ctx {
     int two_step[4] = {0, 0, 0, 0};
     int two_step_wg_per_cu[4] = {0, 0, 0, 0};  // persistent grid of the two-step kernel; 0: a workgroup per tile
     int two_step_lds[4] = {0, 0, 0, 0};        // the LDS form (k_step2_lds) of the two-step recurrence
+    bool autotune_two_step = true;             // kpmx_autotune_diagonal may choose the LDS two-step form
+    bool two_step_auto[4] = {false, false, false, false};  // the two-step settings are the autotuner's
     Workspace work3, work4;  // the third and fourth vector of the two-step recurrence
     int graph_policy = KPMX_GRAPH_NEVER;
     int adjoint_variant = -1;  // adjoint step variant (kpmx_ctx_set_adjoint_variant); -1: the default
@@ -162,6 +169,26 @@ struct kpmx_ctx {
         bool busy = false;
         ~TapePool() { dev_free(ptr); }
     } tape_pool;
+};
+
+// Host-built arrays of the LDS two-step form for one matrix form: the tile-local position of every
+// stored entry's column (lidx) and the tiles' packed halo slices (see k_step2_lds).
+struct LdsHalo {
+    uint16_t* lidx = nullptr;
+    int64_t* hs_first = nullptr;
+    int64_t* hs_ptr = nullptr;
+    int32_t* hs_width = nullptr;
+    int32_t* hrow = nullptr;
+    int32_t* hidx = nullptr;
+    void* hval = nullptr;
+    uint8_t* hcode = nullptr;
+    void* hdiag = nullptr;
+    void release() {
+        for (void* p : {(void*)lidx, (void*)hs_first, (void*)hs_ptr, (void*)hs_width, (void*)hrow, (void*)hidx, hval,
+                        (void*)hcode, hdiag})
+            dev_free(p);
+        *this = LdsHalo{};
+    }
 };
 
 struct kpmx_mat {
@@ -190,10 +217,9 @@ struct kpmx_mat {
     mutable int64_t halo_tile = 0, halo_ntiles = 0, halo_rows = 0;
     mutable int64_t* d_h1_ptr = nullptr;
     mutable int32_t* d_h1_rows = nullptr;
-    // the LDS two-step form's tile-local column positions (tiles of STEP2_LDS_TILE rows) of the
-    // standard SELL (d_lidx) and of the value-table SELL (d_clidx), and the most halo rows of a tile
-    mutable uint16_t* d_lidx = nullptr;
-    mutable uint16_t* d_clidx = nullptr;
+    // the LDS two-step form's arrays (tiles of STEP2_LDS_TILE rows) for the standard SELL and for
+    // the value-table SELL, and the most halo rows of a tile
+    mutable LdsHalo lds_std, lds_coded;
     mutable int lds_hmax = -1;
     // value-table form (SELL over the off-diagonal entries with 8-bit codes into `ctable`, the
     // diagonal as a dense vector), built at upload when the off-diagonal entries take at most 256
@@ -209,8 +235,8 @@ struct kpmx_mat {
         for (void* p : {(void*)cslice_ptr, (void*)cslice_width, (void*)cindices, (void*)ccodes, cdiag, ctable}) dev_free(p);
         dev_free(d_h1_ptr);
         dev_free(d_h1_rows);
-        dev_free(d_lidx);
-        dev_free(d_clidx);
+        lds_std.release();
+        lds_coded.release();
         dev_free(d_sell_pos);
         dev_free(indptr);
         dev_free(indices);
@@ -606,37 +632,54 @@ void ensure_halo(const kpmx_mat* h2, int64_t P) {
 // read (at most 4 off-diagonal entries per row) or the standard SELL.
 bool lds_coded(const kpmx_mat* h2) { return h2->ccodes && h2->cmax_width <= 4; }
 
-// Tile-local positions of every stored entry's column for the LDS two-step form (tiles of
-// STEP2_LDS_TILE rows): a column inside the tile is its offset in the tile, one outside is
-// STEP2_LDS_TILE + its index in the tile's sorted halo rows.  Built for the form the kernel reads
-// (the value-table SELL when lds_coded, else the standard SELL) from device copies, once.
+// The LDS two-step form's arrays for the matrix form the kernel reads (the value-table SELL when
+// lds_coded, else the standard SELL), built once from device copies (tiles of STEP2_LDS_TILE rows):
+//  - lidx: every stored entry's column as its tile-local position -- a column inside the tile is
+//    its offset in the tile, one outside is STEP2_LDS_TILE + its index in the tile's sorted halo;
+//  - the packed halo slices: the tile's halo rows in groups of 64, each row's stored entries (its
+//    slice's width, padding included) column-major, so the kernel reads them like a SELL slice.
 void ensure_lidx(const kpmx_mat* h2) {
     constexpr int64_t P = STEP2_LDS_TILE;
     ensure_halo(h2, P);
     const bool coded = lds_coded(h2);
-    if (coded ? h2->d_clidx : h2->d_lidx) return;
+    LdsHalo& L = coded ? h2->lds_coded : h2->lds_std;
+    if (L.lidx) return;
     kpmx_ctx* ctx = h2->ctx;
     const int64_t n = h2->rows, ns = h2->nslices, ntiles = h2->halo_ntiles;
     std::vector<int64_t> hptr(ntiles + 1), sptr(ns + 1);
     std::vector<int32_t> hrows((size_t)std::max<int64_t>(h2->halo_rows, 1)), swidth(ns);
-    KPMX_HIP(hipMemcpyAsync(hptr.data(), h2->d_h1_ptr, hptr.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    if (h2->halo_rows)
-        KPMX_HIP(hipMemcpyAsync(hrows.data(), h2->d_h1_rows, h2->halo_rows * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    const int64_t* d_ptr = coded ? h2->cslice_ptr : h2->slice_ptr;
-    const int32_t* d_width = coded ? h2->cslice_width : h2->slice_width;
-    const int32_t* d_idx = coded ? h2->cindices : h2->sell_indices;
-    KPMX_HIP(hipMemcpyAsync(sptr.data(), d_ptr, sptr.size() * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    KPMX_HIP(hipMemcpyAsync(swidth.data(), d_width, swidth.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    auto down = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) KPMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    };
+    down(hptr.data(), h2->d_h1_ptr, hptr.size() * sizeof(int64_t));
+    down(hrows.data(), h2->d_h1_rows, h2->halo_rows * sizeof(int32_t));
+    down(sptr.data(), coded ? h2->cslice_ptr : h2->slice_ptr, sptr.size() * sizeof(int64_t));
+    down(swidth.data(), coded ? h2->cslice_width : h2->slice_width, swidth.size() * sizeof(int32_t));
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t total = sptr[ns];
     std::vector<int32_t> idx((size_t)std::max<int64_t>(total, 1));
-    if (total)
-        KPMX_HIP(hipMemcpyAsync(idx.data(), d_idx, total * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> val(coded ? 1 : (size_t)std::max<int64_t>(total, 1)), diag(coded ? (size_t)n : 1);
+    std::vector<uint8_t> code(coded ? (size_t)std::max<int64_t>(total, 1) : 1);
+    std::vector<double> table(coded ? (size_t)std::max(h2->ctable_size, 1) : 1, 0.0);
+    down(idx.data(), coded ? h2->cindices : h2->sell_indices, total * sizeof(int32_t));
+    if (coded) {
+        down(code.data(), h2->ccodes, total);
+        down(diag.data(), h2->cdiag, n * sizeof(double));
+        down(table.data(), h2->ctable, h2->ctable_size * sizeof(double));
+    } else {
+        down(val.data(), h2->sell_vals, total * sizeof(double));
+    }
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
+    uint8_t zcode = 0;  // the value table's zero (padding of the packed halo)
+    if (coded) {
+        const auto z = std::find(table.begin(), table.begin() + h2->ctable_size, 0.0);
+        KPMX_REQUIRE(z != table.begin() + h2->ctable_size, "internal error: the value table has no zero");
+        zcode = (uint8_t)(z - table.begin());
+    }
     int hmax = 0;
     for (int64_t j = 0; j < ntiles; ++j) hmax = std::max<int>(hmax, (int)(hptr[j + 1] - hptr[j]));
     KPMX_REQUIRE(P + hmax < 65536, "the LDS two-step form needs fewer halo rows per tile");
+    // tile-local positions
     std::vector<uint16_t> loc((size_t)std::max<int64_t>(total, 1), 0);
     for (int64_t slc = 0; slc < ns; ++slc) {
         const int64_t j = slc * 64 / P, t0 = j * P, t1 = std::min(n, t0 + P);
@@ -655,10 +698,62 @@ void ensure_lidx(const kpmx_mat* h2) {
             loc[(size_t)e] = (uint16_t)pos;
         }
     }
-    uint16_t* d = static_cast<uint16_t*>(dev_alloc(loc.size() * sizeof(uint16_t)));
-    KPMX_HIP(hipMemcpyAsync(d, loc.data(), loc.size() * sizeof(uint16_t), hipMemcpyHostToDevice, ctx->stream));
+    // packed halo slices
+    std::vector<int64_t> hs_first(ntiles + 1, 0), hs_ptr(1, 0);
+    std::vector<int32_t> hs_width, hrow, hidx;
+    std::vector<double> hval, hdiag;
+    std::vector<uint8_t> hcode;
+    for (int64_t j = 0; j < ntiles; ++j) {
+        const int64_t t0 = j * P, nh = hptr[j + 1] - hptr[j];
+        const int32_t* h = hrows.data() + hptr[j];
+        for (int64_t q0 = 0; q0 < nh; q0 += 64) {
+            int32_t w = 0;
+            for (int64_t r = q0; r < std::min(nh, q0 + 64); ++r) w = std::max(w, swidth[(size_t)(h[r] >> 6)]);
+            const size_t eb = hidx.size();
+            hidx.resize(eb + (size_t)w * 64);
+            if (coded) hcode.resize(eb + (size_t)w * 64, zcode);
+            else hval.resize(eb + (size_t)w * 64, 0.0);
+            for (int64_t r = 0; r < 64; ++r) {
+                const bool real = q0 + r < nh;
+                const int64_t row = real ? h[q0 + r] : t0;
+                hrow.push_back((int32_t)row);
+                if (coded) hdiag.push_back(real ? diag[(size_t)row] : 0.0);
+                const int64_t slc = row >> 6, lane = row & 63;
+                for (int32_t k = 0; k < w; ++k) {
+                    const size_t dst = eb + (size_t)k * 64 + r;
+                    if (real && k < swidth[(size_t)slc]) {
+                        const size_t src = (size_t)(sptr[slc] + (int64_t)k * 64 + lane);
+                        hidx[dst] = idx[src];
+                        if (coded) hcode[dst] = code[src];
+                        else hval[dst] = val[src];
+                    } else {
+                        hidx[dst] = (int32_t)row;
+                    }
+                }
+            }
+            hs_width.push_back(w);
+            hs_ptr.push_back((int64_t)hidx.size());
+        }
+        hs_first[j + 1] = (int64_t)hs_width.size();
+    }
+    auto up = [&](const void* src, size_t bytes) -> void* {
+        void* d = dev_alloc(std::max<size_t>(bytes, 8));
+        if (bytes) KPMX_HIP(hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return d;
+    };
+    L.lidx = static_cast<uint16_t*>(up(loc.data(), loc.size() * sizeof(uint16_t)));
+    L.hs_first = static_cast<int64_t*>(up(hs_first.data(), hs_first.size() * sizeof(int64_t)));
+    L.hs_ptr = static_cast<int64_t*>(up(hs_ptr.data(), hs_ptr.size() * sizeof(int64_t)));
+    L.hs_width = static_cast<int32_t*>(up(hs_width.data(), hs_width.size() * sizeof(int32_t)));
+    L.hrow = static_cast<int32_t*>(up(hrow.data(), hrow.size() * sizeof(int32_t)));
+    L.hidx = static_cast<int32_t*>(up(hidx.data(), hidx.size() * sizeof(int32_t)));
+    if (coded) {
+        L.hcode = static_cast<uint8_t*>(up(hcode.data(), hcode.size()));
+        L.hdiag = up(hdiag.data(), hdiag.size() * sizeof(double));
+    } else {
+        L.hval = up(hval.data(), hval.size() * sizeof(double));
+    }
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
-    (coded ? h2->d_clidx : h2->d_lidx) = d;
     h2->lds_hmax = hmax;
 }
 
@@ -710,14 +805,26 @@ Step2Launch two_step_plan(const kpmx_mat* h2, const Tile& t) {
         S.diag = h2->cdiag;
         S.table = h2->ctable;
         S.table_size = h2->ctable_size;
-        S.lidx = h2->d_clidx;
-    } else {
-        S.lidx = h2->d_lidx;
     }
+    const LdsHalo& L = lds_coded(h2) ? h2->lds_coded : h2->lds_std;
+    S.lidx = L.lidx;
+    S.hs_first = L.hs_first;
+    S.hs_ptr = L.hs_ptr;
+    S.hs_width = L.hs_width;
+    S.hrow = L.hrow;
+    S.hidx = L.hidx;
+    S.hval = L.hval;
+    S.hcode = L.hcode;
+    S.hdiag = L.hdiag;
+    // workgroups per CU: the caller's, else as many as LDS allows (at most 6); the kernel's
+    // registers are then bounded so that many fit
+    const int by_lds = (int)std::min<size_t>(6, (size_t)160 * 1024 / (step2_lds_bytes(t.vw, S.hmax) + 4096));
+    const int asked = ctx->two_step_wg_per_cu[h2->scalar];
+    S.min_waves = std::max(1, std::min(6, asked > 0 ? asked : by_lds));
     const int per_cu = step2_lds_blocks_per_cu(S);
     KPMX_REQUIRE(per_cu > 0, "the LDS two-step form does not fit this matrix's halo (" + std::to_string(S.hmax) +
                                  " rows per tile) in LDS");
-    const int want = ctx->two_step_wg_per_cu[h2->scalar] > 0 ? ctx->two_step_wg_per_cu[h2->scalar] : per_cu;
+    const int want = asked > 0 ? asked : per_cu;
     const int64_t all = (h2->halo_ntiles + 7) / 8 * 8;
     S.num_wg = (int)std::max<int64_t>(8, std::min<int64_t>(all, (int64_t)want * ctx->num_cu / 8 * 8));
     return S;
@@ -1179,10 +1286,7 @@ int kpmx_ctx_create(int device, kpmx_ctx** out) {
             // on matrices with the value-table form: its column-pair variant with tile shape 2 on
             // the resident grid (16 columns: 1.207 ms per 64-vector step against 1.270 ms)
             ctx->tune_coded[KPMX_F64][4] = pairs + NUM_TILE_SHAPES * NUM_GRID_VARIANTS + 2;
-            // fp64 recurrences run in column tiles of at most 16 vectors: on the 4.19 M-site
-            // workload four 16-column recurrences take 1.25 ms per step of 64 vectors, one
-            // 64-column recurrence 1.29 ms
-            ctx->tile_cap[KPMX_F64] = 16;
+            ctx->tile_cap[KPMX_F64] = default_tile_cap(KPMX_F64);
         } catch (...) {
             delete ctx;
             throw;
@@ -1461,6 +1565,7 @@ int kpmx_ctx_set_tile_width(kpmx_ctx* ctx, int scalar, int width) {
         KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
         KPMX_REQUIRE(width >= 1 && width <= 64 && (width & (width - 1)) == 0, "tile width must be 1, 2, 4, ..., 64");
         ctx->tile_cap[scalar] = width;
+        ctx->two_step_auto[scalar] = false;
     });
 }
 
@@ -1480,8 +1585,25 @@ int kpmx_ctx_set_two_step(kpmx_ctx* ctx, int scalar, int tile_rows, int workgrou
                      "two-step tile rows must be 0 or a multiple of 64 up to 1048576");
         KPMX_REQUIRE(workgroups_per_cu >= 0 && workgroups_per_cu <= 64, "workgroups per CU must lie in [0, 64]");
         ctx->two_step[scalar] = tile_rows;
+        ctx->two_step_auto[scalar] = false;
         ctx->two_step_wg_per_cu[scalar] = workgroups_per_cu;
         ctx->two_step_lds[scalar] = 0;
+    });
+}
+
+int kpmx_ctx_get_two_step_lds(kpmx_ctx* ctx, int scalar, int* enabled, int* workgroups_per_cu) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        KPMX_REQUIRE(valid_scalar(scalar), "invalid scalar code");
+        if (enabled) *enabled = ctx->two_step_lds[scalar] && ctx->two_step[scalar] > 0;
+        if (workgroups_per_cu) *workgroups_per_cu = ctx->two_step_wg_per_cu[scalar];
+    });
+}
+
+int kpmx_ctx_set_autotune_two_step(kpmx_ctx* ctx, int enable) {
+    return guarded([&] {
+        KPMX_REQUIRE(ctx, "null context");
+        ctx->autotune_two_step = enable != 0;
     });
 }
 
@@ -1491,6 +1613,7 @@ int kpmx_ctx_set_two_step_lds(kpmx_ctx* ctx, int scalar, int workgroups_per_cu) 
         KPMX_REQUIRE(scalar == KPMX_F64, "the LDS two-step form is fp64 only");
         KPMX_REQUIRE(workgroups_per_cu >= 0 && workgroups_per_cu <= 64, "workgroups per CU must lie in [0, 64]");
         ctx->two_step[scalar] = STEP2_LDS_TILE;
+        ctx->two_step_auto[scalar] = false;
         ctx->two_step_wg_per_cu[scalar] = workgroups_per_cu;
         ctx->two_step_lds[scalar] = 1;
     });
@@ -2065,11 +2188,80 @@ int kpmx_ctx_set_step_variant(kpmx_ctx* ctx, int scalar, int vector_width, int v
     });
 }
 
+namespace {
+// The LDS two-step form against single steps (fp64 SELL matrices): column tiles of 8 vectors with
+// 4 workgroups per CU and of 16 vectors with 3, each timed on the first column tile it would run;
+// the fastest per vector is kept in the context (tile width and two-step settings) when it beats
+// the single steps' `single_per_vector` milliseconds per vector and step by 2 %, else the
+// settings are left as they were.
+void autotune_two_step(const kpmx_mat* h2, const kpmx_block* r0, double single_per_vector) {
+    kpmx_ctx* ctx = h2->ctx;
+    const int sc = h2->scalar;
+    if (sc != KPMX_F64 || h2->format != KPMX_FMT_SELL || !h2->step_rows.empty() || h2->rows < 2 * 64) return;
+    const int saved[4] = {ctx->two_step[sc], ctx->two_step_wg_per_cu[sc], ctx->two_step_lds[sc], ctx->tile_cap[sc]};
+    auto set = [&](int rows, int wg, int lds, int cap) {
+        ctx->two_step[sc] = rows;
+        ctx->two_step_wg_per_cu[sc] = wg;
+        ctx->two_step_lds[sc] = lds;
+        ctx->tile_cap[sc] = cap;
+    };
+    // about 4 GB of vector traffic per timed sample of 8 columns (16 to 256 steps, even)
+    int64_t st = std::max<int64_t>(16, std::min<int64_t>(256, (int64_t)(4e9 / (3.0 * h2->rows * 8 * 8))));
+    st -= st % 2;
+    struct Cand {
+        int cap, wg;
+        bool ok;
+        float ms;
+    };
+    std::vector<Cand> cands{{8, 4, false, 0.f}, {16, 3, false, 0.f}};
+    // one untimed run of every candidate (code loading, workspaces), then three rounds that
+    // alternate the candidates; each keeps its fastest sample
+    for (int round = 0; round < 4; ++round) {
+        for (Cand& c : cands) {
+            set(STEP2_LDS_TILE, c.wg, 1, c.cap);
+            const Tile t = stage_run_tile(h2, r0, run_tiles(r0, c.cap).front());
+            if (round == 0) c.ok = use_two_step(h2, t);
+            if (!c.ok) continue;
+            double* sums = static_cast<double*>(ctx->sums.get((size_t)st * 4 * t.vw * sizeof(double)));
+            KPMX_HIP(hipEventRecord(ctx->t0, ctx->stream));
+            run_recurrence2(h2, t, st, sums);
+            KPMX_HIP(hipEventRecord(ctx->t1, ctx->stream));
+            KPMX_HIP(hipEventSynchronize(ctx->t1));
+            float ms = 0;
+            KPMX_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+            const float per = ms / (float)t.vw;  // per vector
+            if (round == 1 || (round > 1 && per < c.ms)) c.ms = per;
+        }
+    }
+    double best = 0.98 * single_per_vector;
+    int best_cap = 0, best_wg = 0;
+    for (const Cand& c : cands) {
+        const double per_vector = c.ms / (double)st;
+        if (c.ok && per_vector < best) {
+            best = per_vector;
+            best_cap = c.cap;
+            best_wg = c.wg;
+        }
+    }
+    if (best_cap) {
+        set(STEP2_LDS_TILE, best_wg, 1, best_cap);
+        ctx->two_step_auto[sc] = true;
+    } else {
+        set(saved[0], saved[1], saved[2], saved[3]);
+    }
+}
+}  // namespace
+
 int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* variant) {
     return guarded([&] {
         check_pair(h2, r0);
         kpmx_ctx* ctx = h2->ctx;
         KPMX_HIP(hipSetDevice(ctx->device));
+        if (ctx->two_step_auto[h2->scalar]) {  // an earlier autotuner choice: start from the defaults
+            ctx->two_step[h2->scalar] = ctx->two_step_wg_per_cu[h2->scalar] = ctx->two_step_lds[h2->scalar] = 0;
+            ctx->tile_cap[h2->scalar] = default_tile_cap(h2->scalar);
+            ctx->two_step_auto[h2->scalar] = false;
+        }
         // the first column tile the diagonal recurrence runs (a sub-tile of a wider starter tile)
         const Tile t = stage_run_tile(h2, r0, run_tiles(r0, diag_cap(h2)).front());
         // About 2 GB of vector traffic per timed sample (at least 9 steps, at most 255): small
@@ -2106,6 +2298,7 @@ int kpmx_autotune_diagonal(const kpmx_mat* h2, const kpmx_block* r0, int* varian
         ctx->tune[h2->scalar][log2i(t.vw)] = best;
         ctx->tune_coded[h2->scalar][log2i(t.vw)] = -1;
         if (variant) *variant = best;
+        if (ctx->autotune_two_step) autotune_two_step(h2, r0, best_ms / ((double)steps * t.vw));
     });
 }
 

@@ -155,6 +155,16 @@ struct Step2Launch {
     // slice_width then describe the off-diagonal entries, 1-byte codes into table, diag dense).
     const uint16_t* lidx = nullptr;
     int hmax = 0;
+    int min_waves = 1;  // register budget: waves per SIMD to allow (1 ... 6; widths above 4 entries: 1)
+    // packed halo slices (64 halo rows each; see k_step2_lds)
+    const int64_t* hs_first = nullptr;
+    const int64_t* hs_ptr = nullptr;
+    const int32_t* hs_width = nullptr;
+    const int32_t* hrow = nullptr;
+    const int32_t* hidx = nullptr;
+    const void* hval = nullptr;
+    const uint8_t* hcode = nullptr;
+    const void* hdiag = nullptr;
     const uint8_t* codes = nullptr;
     const void* diag = nullptr;
     const void* table = nullptr;

@@ -1070,8 +1070,10 @@ __global__ __launch_bounds__(BLOCK) void k_step2_sell(Step2Args<T> a) {
 // Tiles of STEP2_LDS_TILE rows, walked like k_step2_sell; wave w owns slice w of the tile.
 // Phase A: the wave computes r_{k+1} = s H2 r_k - r_{k-1} on its slice (stored to out1 for the
 // next launch and to its LDS rows) and keeps r_k of its rows in registers; the waves then compute
-// r_{k+1} on the tile's halo rows (row by row, from the global r_k) into the LDS rows after the
-// tile's.  After a workgroup barrier, phase B computes r_{k+2} = H2 r_{k+1} - r_k on the slice,
+// r_{k+1} on the tile's halo rows into the LDS rows after the tile's, reading the halo rows'
+// entries from the tile's packed halo slices (64 halo rows each, entries column-major like a SELL
+// slice, host-built) and r_k from global memory.  After a workgroup barrier, phase B computes
+// r_{k+2} = H2 r_{k+1} - r_k on the slice,
 // gathering r_{k+1} from LDS by the tile-local column positions `lidx` (host-built) and taking
 // r_k from registers.  A second barrier ends the tile before the next one overwrites the LDS.
 struct Step2LdsArgs {
@@ -1085,7 +1087,18 @@ struct Step2LdsArgs {
     const int64_t* slice_ptr;
     const int32_t* slice_width;
     const int64_t* h1_ptr;
-    const int32_t* h1_rows;
+    // packed halo slices: tile j owns halo slices [hs_first[j], hs_first[j + 1]); halo slice q
+    // holds halo rows q' * 64 + r of its tile (q' = q - hs_first[j]) as global row hrow[q * 64 + r],
+    // entries (hidx, hval or hcode) at hs_ptr[q] + k * 64 + r for k < hs_width[q] and, in the
+    // value-table form, the diagonal hdiag[q * 64 + r]; padding slots hold the tile's first row
+    const int64_t* hs_first;
+    const int64_t* hs_ptr;
+    const int32_t* hs_width;
+    const int32_t* hrow;
+    const int32_t* hidx;
+    const double* hval;
+    const uint8_t* hcode;
+    const double* hdiag;
     const dpair* xprev;  // r_{k-1}; null on the first step
     const dpair* x;      // r_k
     dpair* out1;         // r_{k+1}
@@ -1096,8 +1109,10 @@ struct Step2LdsArgs {
     double hscaleA;
 };
 
-template <int VW, int KMAX, bool CODED>
-__global__ __launch_bounds__(BLOCK) void k_step2_lds(Step2LdsArgs a) {
+// MINW: waves per SIMD the register allocation must allow (one wave of each of MINW workgroups
+// per CU); 1 leaves the compiler free.
+template <int VW, int KMAX, bool CODED, int MINW>
+__global__ __launch_bounds__(BLOCK, MINW) void k_step2_lds(Step2LdsArgs a) {
     using T = dpair;
     using V = double;
     constexpr int RPW = WAVE / VW;  // rows per row group
@@ -1119,6 +1134,10 @@ __global__ __launch_bounds__(BLOCK) void k_step2_lds(Step2LdsArgs a) {
         if constexpr (CODED) return s_tab[a.codes[off]];
         else return a.vals[off];
     };
+    auto hvalue = [&](int64_t off) -> V {
+        if constexpr (CODED) return s_tab[a.hcode[off]];
+        else return a.hval[off];
+    };
     const int nb = gridDim.x, b = blockIdx.x;
     const int64_t per_region = (a.ntiles + 7) / 8;
     const int64_t region0 = (int64_t)(b % 8) * per_region;
@@ -1130,7 +1149,7 @@ __global__ __launch_bounds__(BLOCK) void k_step2_lds(Step2LdsArgs a) {
         const int64_t tend = a.n < t0 + STEP2_LDS_TILE ? a.n : t0 + STEP2_LDS_TILE;
         const int64_t slc = t0 / WAVE + wave;
         const bool has_slice = slc * WAVE < tend;  // uniform over the wave
-        int32_t c_idx[KMAX];
+        int32_t c_idx[KMAX], c_loc[KMAX];
         V c_val[KMAX];
         V c_diag = 0;
         int64_t base = 0;
@@ -1140,13 +1159,16 @@ __global__ __launch_bounds__(BLOCK) void k_step2_lds(Step2LdsArgs a) {
             base = a.slice_ptr[slc];
             width = a.slice_width[slc];
             const int64_t own = slc * WAVE + lane < a.n ? slc * WAVE + lane : a.n - 1;
+            // the entries of both phases in one round trip: columns, tile-local positions, values
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
                 if (k < width) {
                     c_idx[k] = a.indices[base + (int64_t)k * WAVE + lane];
+                    c_loc[k] = (int32_t)a.lidx[base + (int64_t)k * WAVE + lane];
                     c_val[k] = value(base + (int64_t)k * WAVE + lane);
                 } else {
                     c_idx[k] = (int32_t)own;
+                    c_loc[k] = wave * WAVE + lane;
                     c_val[k] = 0;
                 }
             }
@@ -1191,33 +1213,44 @@ __global__ __launch_bounds__(BLOCK) void k_step2_lds(Step2LdsArgs a) {
                 }
             }
         }
-        // phase A on the halo rows, row by row (each lane loads its row's entries)
+        // phase A on the halo rows: row group g of every packed halo slice of the tile goes to
+        // wave g % 4 (RPW rows of VW column pairs per row group)
         {
-            const int64_t h0 = a.h1_ptr[tile], nh = a.h1_ptr[tile + 1] - h0;
-            for (int64_t g = wave; g * RPW < nh; g += WAVES_PER_BLOCK) {
-                const int64_t i = g * RPW + rsub;
-                const bool ok = i < nh;
-                const int64_t row = ok ? (int64_t)a.h1_rows[h0 + i] : t0;
-                const int64_t hs = row >> 6;
-                const int64_t hb = a.slice_ptr[hs] + (row & 63);
-                const int hw = a.slice_width[hs];
-                T acc = zero(T{});
-                for (int k = 0; k < hw; ++k) {
-                    const int64_t off = hb + (int64_t)k * WAVE;
-                    acc = fmac(value(off), a.x[(int64_t)a.indices[off] * VW + col], acc);
+            const int64_t nh = a.h1_ptr[tile + 1] - a.h1_ptr[tile];
+            const int64_t q0 = a.hs_first[tile], q1 = a.hs_first[tile + 1];
+            for (int64_t q = q0; q < q1; ++q) {
+                const int64_t hb = a.hs_ptr[q];
+                const int hw = a.hs_width[q];
+                for (int g = wave; g < WAVE / RPW; g += WAVES_PER_BLOCK) {
+                    const int r = g * RPW + rsub;
+                    const int64_t i = (q - q0) * WAVE + r;  // halo row of the tile
+                    const int64_t row = a.hrow[q * WAVE + r];
+                    const T xr = a.x[row * VW + col];
+                    const T yr = a.xprev ? a.xprev[row * VW + col] : zero(T{});
+                    V hv[KMAX];
+                    T hx[KMAX];
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k) {
+                        const int64_t off = hb + (int64_t)k * WAVE + r;
+                        hv[k] = k < hw ? hvalue(off) : 0.0;
+                        hx[k] = a.x[(int64_t)(k < hw ? a.hidx[off] : (int32_t)row) * VW + col];
+                    }
+                    T acc = zero(T{});
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k) acc = fmac(hv[k], hx[k], acc);
+                    for (int k = KMAX; k < hw; ++k) {
+                        const int64_t off = hb + (int64_t)k * WAVE + r;
+                        acc = fmac(hvalue(off), a.x[(int64_t)a.hidx[off] * VW + col], acc);
+                    }
+                    if constexpr (CODED) acc = fmac(a.hdiag[q * WAVE + r], xr, acc);
+                    T ynew = scal(acc, a.hscaleA);
+                    if (a.xprev) ynew = sub(ynew, yr);
+                    if (i < nh) s_r[(STEP2_LDS_TILE + i) * VW + col] = ynew;
                 }
-                if constexpr (CODED) acc = fmac(a.diag[row], a.x[row * VW + col], acc);
-                T ynew = scal(acc, a.hscaleA);
-                if (a.xprev) ynew = sub(ynew, a.xprev[row * VW + col]);
-                if (ok) s_r[(STEP2_LDS_TILE + i) * VW + col] = ynew;
             }
         }
         __syncthreads();
         if (has_slice) {
-            int32_t c_loc[KMAX];
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k)
-                c_loc[k] = k < width ? (int32_t)a.lidx[base + (int64_t)k * WAVE + lane] : wave * WAVE + lane;
 #pragma unroll
             for (int u0 = 0; u0 < U; u0 += UC) {
                 int rin[UC];
@@ -2328,7 +2361,14 @@ static int step2_lds_t(const Step2Launch& S, hipStream_t stream, int* occupancy)
     a.slice_ptr = S.slice_ptr;
     a.slice_width = S.slice_width;
     a.h1_ptr = S.h1_ptr;
-    a.h1_rows = S.h1_rows;
+    a.hs_first = S.hs_first;
+    a.hs_ptr = S.hs_ptr;
+    a.hs_width = S.hs_width;
+    a.hrow = S.hrow;
+    a.hidx = S.hidx;
+    a.hval = static_cast<const double*>(S.hval);
+    a.hcode = S.hcode;
+    a.hdiag = static_cast<const double*>(S.hdiag);
     a.xprev = static_cast<const dpair*>(S.xprev);
     a.x = static_cast<const dpair*>(S.x);
     a.out1 = static_cast<dpair*>(S.out1);
@@ -2348,12 +2388,20 @@ static int step2_lds_t(const Step2Launch& S, hipStream_t stream, int* occupancy)
         return 0;
     };
     const bool coded = S.codes != nullptr;
-    if (S.max_width <= 4) return coded ? go(k_step2_lds<VW, 4, true>) : go(k_step2_lds<VW, 4, false>);
-    return coded ? go(k_step2_lds<VW, 8, true>) : go(k_step2_lds<VW, 8, false>);
+    if (S.max_width > 4) return coded ? go(k_step2_lds<VW, 8, true, 1>) : go(k_step2_lds<VW, 8, false, 1>);
+    switch (S.min_waves) {  // the register budget for that many workgroups per CU
+        case 2: return coded ? go(k_step2_lds<VW, 4, true, 2>) : go(k_step2_lds<VW, 4, false, 2>);
+        case 3: return coded ? go(k_step2_lds<VW, 4, true, 3>) : go(k_step2_lds<VW, 4, false, 3>);
+        case 4: return coded ? go(k_step2_lds<VW, 4, true, 4>) : go(k_step2_lds<VW, 4, false, 4>);
+        case 5: return coded ? go(k_step2_lds<VW, 4, true, 5>) : go(k_step2_lds<VW, 4, false, 5>);
+        case 6: return coded ? go(k_step2_lds<VW, 4, true, 6>) : go(k_step2_lds<VW, 4, false, 6>);
+        default: return coded ? go(k_step2_lds<VW, 4, true, 1>) : go(k_step2_lds<VW, 4, false, 1>);
+    }
 }
 
 static int step2_lds_dispatch(const Step2Launch& S, hipStream_t stream, int* occupancy) {
-    if (!step2_lds_supported(S.scalar, S.vw) || S.hmax < 0 || step2_lds_bytes(S.vw, S.hmax) > STEP2_LDS_MAX_BYTES)
+    if (!step2_lds_supported(S.scalar, S.vw) || S.hmax < 0 || step2_lds_bytes(S.vw, S.hmax) > STEP2_LDS_MAX_BYTES ||
+        !S.hs_first || !S.hs_ptr || !S.hs_width || !S.hrow || !S.hidx || (S.codes ? !S.hcode || !S.hdiag : !S.hval))
         return -1;
     return S.vw == 8 ? step2_lds_t<4>(S, stream, occupancy) : step2_lds_t<8>(S, stream, occupancy);
 }

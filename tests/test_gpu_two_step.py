@@ -124,7 +124,8 @@ def test_two_step_lds_matches_oracle(oracle, shape, coded, nvec, M):
     on the value-table and the standard matrix form, tiles that end inside the matrix's last slice
     or leave waves without a slice, 8 and 16 columns (20 vectors: a 16-column tile in the LDS form
     and a 4-column one on single steps), odd and even step counts, the resident grid and one
-    workgroup per CU: within rounding of single steps and of the oracle."""
+    workgroup per CU and the register budgets for 3 and 4: within rounding of single steps and of
+    the oracle."""
     h2 = matrix(np.float64, *shape)
     if not coded:
         h2 = random_hoppings(h2)
@@ -136,9 +137,9 @@ def test_two_step_lds_matches_oracle(oracle, shape, coded, nvec, M):
         assert (mat.coded_bytes() > 0) == coded
         single = abi.moments_diagonal(mat, blk, M)
         out = {}
-        for wg in (0, 1):
+        for wg in (0, 1, 3, 4):  # 1, 3, 4: register budgets for that many workgroups per CU
             ctx.set_two_step_lds(wg)
-            assert ctx.two_step(np.float64) == 256
+            assert ctx.two_step(np.float64) == 256 and ctx.two_step_lds() == (True, wg)
             out[wg] = abi.moments_diagonal(mat, blk, M)
         ctx.set_two_step(np.float64, 0)
         np.testing.assert_array_equal(abi.moments_diagonal(mat, blk, M), single)
@@ -191,3 +192,31 @@ def test_two_step_lds_bfs_order_and_graph_replay(oracle):
     assert np.max(np.abs(runs[0] - ref) / np.abs(ref[:, :1])) <= 1e-10
     ref32 = oracle.diag_moments(h32, r32, M)
     assert np.max(np.abs(mu32 - ref32) / np.abs(ref32[:, :1])) <= 1e-4
+
+
+def test_autotuner_two_step_choice(oracle):
+    """The autotuner times the LDS two-step form against single steps and keeps whichever is
+    faster; either way the moments match the oracle, a later autotuning starts again from the
+    defaults, and with the choice switched off the settings stay single steps."""
+    h2 = matrix(np.float64, 96, 64)
+    r0 = oracle.random_starter(h2.shape[0], 32, np.float64)
+    M = 66
+    ref = oracle.diag_moments(h2, r0, M)
+    ctx = abi.Ctx(0)
+    try:
+        mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
+        abi.autotune_diagonal(mat, blk)
+        chosen = ctx.two_step_lds()
+        assert ctx.tile_width(np.float64) in ((8, 16) if chosen[0] else (16,))
+        mu = abi.moments_diagonal(mat, blk, M)
+        ctx.set_autotune_two_step(False)
+        abi.autotune_diagonal(mat, blk)
+        assert ctx.two_step_lds() == (False, 0) and ctx.tile_width(np.float64) == 16
+        single = abi.moments_diagonal(mat, blk, M)
+        blk.close()
+        mat.close()
+    finally:
+        ctx.close()
+    scale = np.abs(ref[:, :1])
+    assert np.max(np.abs(mu - ref) / scale) <= 1e-10
+    assert np.max(np.abs(single - ref) / scale) <= 1e-10
