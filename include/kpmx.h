@@ -166,11 +166,11 @@ int kpmx_ctx_get_tile_width(kpmx_ctx* ctx, int scalar, int* width);
  * The moments are the same up to rounding.  get: the current tile rows. */
 int kpmx_ctx_set_two_step(kpmx_ctx* ctx, int scalar, int tile_rows, int workgroups_per_cu);
 int kpmx_ctx_get_two_step(kpmx_ctx* ctx, int scalar, int* tile_rows);
-/* The LDS form of the two-step recurrence (fp64 blocks of 8 or 16 vectors; no counterpart in the
- * reference): tiles of 256 rows, one 64-row slice per wave; each workgroup keeps r_k+1 of its tile
+/* The LDS form of the two-step recurrence (fp64 blocks of 8 or 16 vectors, complex128 blocks of 4
+ * or 8; no counterpart in the reference): tiles of 256 rows, one 64-row slice per wave; each workgroup keeps r_k+1 of its tile
  * and of the tile's halo rows in LDS and the wave keeps r_k of its slice in registers, so a launch
  * reads two vectors and writes two; it runs on a persistent grid of workgroups_per_cu workgroups
- * per CU (0: as many as fit).  Reads the value-table form where the single step would.  Other
+ * per CU (0: as many as fit, at most 4), the kernel's registers bounded so that many fit.  Reads the value-table form where the single step would.  Other
  * scalar types and widths, and matrices whose tiles have too many halo rows for 64 KB of LDS
  * ((256 + halo rows) x 8 bytes x vectors), keep single steps.  kpmx_ctx_set_two_step turns it off
  * again. */
@@ -179,8 +179,8 @@ int kpmx_ctx_set_two_step_lds(kpmx_ctx* ctx, int scalar, int workgroups_per_cu);
  * (0: as many as fit). */
 int kpmx_ctx_get_two_step_lds(kpmx_ctx* ctx, int scalar, int* enabled, int* workgroups_per_cu);
 /* Whether kpmx_autotune_diagonal may also choose the LDS two-step form (default 1): for fp64
- * SELL matrices it then times column tiles of 8 vectors with 4 workgroups per CU and of 16 vectors
- * with 3 against the single steps per vector and keeps the two-step form, with its tile width,
+ * and complex128 SELL matrices it then times column tiles of 64-byte rows (8 fp64 or 4 complex128
+ * vectors) with 4 workgroups per CU and of 128-byte rows with 3 against the single steps per vector and keeps the two-step form, with its tile width,
  * when it is at least 2 % faster.  Settings chosen this way are reset by the next autotuning;
  * settings made with kpmx_ctx_set_tile_width / kpmx_ctx_set_two_step / kpmx_ctx_set_two_step_lds
  * are the caller's and are only changed when the autotuner picks the two-step form. */

@@ -133,10 +133,11 @@ class Ctx(_Handle):
         _check(lib().kpmx_ctx_set_two_step(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]), ctypes.c_int(tile_rows),
                                            ctypes.c_int(workgroups_per_cu)))
 
-    def set_two_step_lds(self, workgroups_per_cu=0):
-        """The LDS form of the two-step fp64 recurrence (256-row tiles, the intermediate vector in
-        LDS) on a persistent grid (kpmx_ctx_set_two_step_lds); set_two_step turns it off."""
-        _check(lib().kpmx_ctx_set_two_step_lds(self.h, ctypes.c_int(SCALAR[np.dtype(np.float64)]),
+    def set_two_step_lds(self, workgroups_per_cu=0, dtype=np.float64):
+        """The LDS form of the two-step recurrence (fp64 or complex128; 256-row tiles, the
+        intermediate vector in LDS) on a persistent grid (kpmx_ctx_set_two_step_lds); set_two_step
+        turns it off."""
+        _check(lib().kpmx_ctx_set_two_step_lds(self.h, ctypes.c_int(SCALAR[np.dtype(dtype)]),
                                                ctypes.c_int(workgroups_per_cu)))
 
     def two_step_lds(self, dtype=np.float64):

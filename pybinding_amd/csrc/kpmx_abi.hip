@@ -657,24 +657,26 @@ void ensure_lidx(const kpmx_mat* h2) {
     down(swidth.data(), coded ? h2->cslice_width : h2->slice_width, swidth.size() * sizeof(int32_t));
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
     const int64_t total = sptr[ns];
+    const size_t sb = scalar_bytes(h2->scalar);  // values are copied as bytes
     std::vector<int32_t> idx((size_t)std::max<int64_t>(total, 1));
-    std::vector<double> val(coded ? 1 : (size_t)std::max<int64_t>(total, 1)), diag(coded ? (size_t)n : 1);
+    std::vector<char> val(coded ? 1 : (size_t)std::max<int64_t>(total, 1) * sb), diag(coded ? (size_t)n * sb : 1);
     std::vector<uint8_t> code(coded ? (size_t)std::max<int64_t>(total, 1) : 1);
-    std::vector<double> table(coded ? (size_t)std::max(h2->ctable_size, 1) : 1, 0.0);
+    std::vector<char> table(coded ? (size_t)std::max(h2->ctable_size, 1) * sb : 1, 0);
     down(idx.data(), coded ? h2->cindices : h2->sell_indices, total * sizeof(int32_t));
     if (coded) {
         down(code.data(), h2->ccodes, total);
-        down(diag.data(), h2->cdiag, n * sizeof(double));
-        down(table.data(), h2->ctable, h2->ctable_size * sizeof(double));
+        down(diag.data(), h2->cdiag, n * sb);
+        down(table.data(), h2->ctable, h2->ctable_size * sb);
     } else {
-        down(val.data(), h2->sell_vals, total * sizeof(double));
+        down(val.data(), h2->sell_vals, total * sb);
     }
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
-    uint8_t zcode = 0;  // the value table's zero (padding of the packed halo)
+    int zcode = -1;  // the value table's zero (padding of the packed halo)
     if (coded) {
-        const auto z = std::find(table.begin(), table.begin() + h2->ctable_size, 0.0);
-        KPMX_REQUIRE(z != table.begin() + h2->ctable_size, "internal error: the value table has no zero");
-        zcode = (uint8_t)(z - table.begin());
+        const std::vector<char> zero(sb, 0);
+        for (int c = 0; c < h2->ctable_size && zcode < 0; ++c)
+            if (std::memcmp(&table[(size_t)c * sb], zero.data(), sb) == 0) zcode = c;
+        KPMX_REQUIRE(zcode >= 0, "internal error: the value table has no zero");
     }
     int hmax = 0;
     for (int64_t j = 0; j < ntiles; ++j) hmax = std::max<int>(hmax, (int)(hptr[j + 1] - hptr[j]));
@@ -701,7 +703,7 @@ void ensure_lidx(const kpmx_mat* h2) {
     // packed halo slices
     std::vector<int64_t> hs_first(ntiles + 1, 0), hs_ptr(1, 0);
     std::vector<int32_t> hs_width, hrow, hidx;
-    std::vector<double> hval, hdiag;
+    std::vector<char> hval, hdiag;
     std::vector<uint8_t> hcode;
     for (int64_t j = 0; j < ntiles; ++j) {
         const int64_t t0 = j * P, nh = hptr[j + 1] - hptr[j];
@@ -711,13 +713,15 @@ void ensure_lidx(const kpmx_mat* h2) {
             for (int64_t r = q0; r < std::min(nh, q0 + 64); ++r) w = std::max(w, swidth[(size_t)(h[r] >> 6)]);
             const size_t eb = hidx.size();
             hidx.resize(eb + (size_t)w * 64);
-            if (coded) hcode.resize(eb + (size_t)w * 64, zcode);
-            else hval.resize(eb + (size_t)w * 64, 0.0);
+            if (coded) hcode.resize(eb + (size_t)w * 64, (uint8_t)zcode);
+            else hval.resize((eb + (size_t)w * 64) * sb, 0);
+            const size_t db = hdiag.size();
+            if (coded) hdiag.resize(db + 64 * sb, 0);
             for (int64_t r = 0; r < 64; ++r) {
                 const bool real = q0 + r < nh;
                 const int64_t row = real ? h[q0 + r] : t0;
                 hrow.push_back((int32_t)row);
-                if (coded) hdiag.push_back(real ? diag[(size_t)row] : 0.0);
+                if (coded && real) std::memcpy(&hdiag[db + (size_t)r * sb], &diag[(size_t)row * sb], sb);
                 const int64_t slc = row >> 6, lane = row & 63;
                 for (int32_t k = 0; k < w; ++k) {
                     const size_t dst = eb + (size_t)k * 64 + r;
@@ -725,7 +729,7 @@ void ensure_lidx(const kpmx_mat* h2) {
                         const size_t src = (size_t)(sptr[slc] + (int64_t)k * 64 + lane);
                         hidx[dst] = idx[src];
                         if (coded) hcode[dst] = code[src];
-                        else hval[dst] = val[src];
+                        else std::memcpy(&hval[dst * sb], &val[src * sb], sb);
                     } else {
                         hidx[dst] = (int32_t)row;
                     }
@@ -749,9 +753,9 @@ void ensure_lidx(const kpmx_mat* h2) {
     L.hidx = static_cast<int32_t*>(up(hidx.data(), hidx.size() * sizeof(int32_t)));
     if (coded) {
         L.hcode = static_cast<uint8_t*>(up(hcode.data(), hcode.size()));
-        L.hdiag = up(hdiag.data(), hdiag.size() * sizeof(double));
+        L.hdiag = up(hdiag.data(), hdiag.size());
     } else {
-        L.hval = up(hval.data(), hval.size() * sizeof(double));
+        L.hval = up(hval.data(), hval.size());
     }
     KPMX_HIP(hipStreamSynchronize(ctx->stream));
     h2->lds_hmax = hmax;
@@ -766,7 +770,7 @@ bool use_two_step(const kpmx_mat* h2, const Tile& t) {
     if (!ctx->two_step_lds[h2->scalar]) return step2_supported(h2->scalar, t.vw);
     if (!step2_lds_supported(h2->scalar, t.vw)) return false;
     ensure_lidx(h2);
-    return step2_lds_bytes(t.vw, h2->lds_hmax) <= STEP2_LDS_MAX_BYTES;
+    return step2_lds_bytes(h2->scalar, t.vw, h2->lds_hmax) <= STEP2_LDS_MAX_BYTES;
 }
 
 // The launch description of the two-step recurrence of this column tile (matrix, halo, grid);
@@ -816,11 +820,12 @@ Step2Launch two_step_plan(const kpmx_mat* h2, const Tile& t) {
     S.hval = L.hval;
     S.hcode = L.hcode;
     S.hdiag = L.hdiag;
-    // workgroups per CU: the caller's, else as many as LDS allows (at most 6); the kernel's
-    // registers are then bounded so that many fit
-    const int by_lds = (int)std::min<size_t>(6, (size_t)160 * 1024 / (step2_lds_bytes(t.vw, S.hmax) + 4096));
+    // workgroups per CU: the caller's (at most 6), else as many as LDS allows up to 4 (tighter
+    // register budgets spill: 5 and 6 measured 1.5-2.4x slower); the kernel's registers are
+    // then bounded so that many fit
+    const int by_lds = (int)std::min<size_t>(4, (size_t)160 * 1024 / (step2_lds_bytes(h2->scalar, t.vw, S.hmax) + 4096));
     const int asked = ctx->two_step_wg_per_cu[h2->scalar];
-    S.min_waves = std::max(1, std::min(6, asked > 0 ? asked : by_lds));
+    S.min_waves = std::max(1, asked > 0 ? std::min(6, asked) : by_lds);
     const int per_cu = step2_lds_blocks_per_cu(S);
     KPMX_REQUIRE(per_cu > 0, "the LDS two-step form does not fit this matrix's halo (" + std::to_string(S.hmax) +
                                  " rows per tile) in LDS");
@@ -1610,7 +1615,7 @@ int kpmx_ctx_set_autotune_two_step(kpmx_ctx* ctx, int enable) {
 int kpmx_ctx_set_two_step_lds(kpmx_ctx* ctx, int scalar, int workgroups_per_cu) {
     return guarded([&] {
         KPMX_REQUIRE(ctx, "null context");
-        KPMX_REQUIRE(scalar == KPMX_F64, "the LDS two-step form is fp64 only");
+        KPMX_REQUIRE(scalar == KPMX_F64 || scalar == KPMX_C128, "the LDS two-step form is fp64 or complex128 only");
         KPMX_REQUIRE(workgroups_per_cu >= 0 && workgroups_per_cu <= 64, "workgroups per CU must lie in [0, 64]");
         ctx->two_step[scalar] = STEP2_LDS_TILE;
         ctx->two_step_auto[scalar] = false;
@@ -2189,15 +2194,18 @@ int kpmx_ctx_set_step_variant(kpmx_ctx* ctx, int scalar, int vector_width, int v
 }
 
 namespace {
-// The LDS two-step form against single steps (fp64 SELL matrices): column tiles of 8 vectors with
-// 4 workgroups per CU and of 16 vectors with 3, each timed on the first column tile it would run;
+// The LDS two-step form against single steps (fp64 and complex128 SELL matrices): column tiles of
+// 64-byte rows (8 fp64 / 4 complex128 vectors) with 4 workgroups per CU and of 128-byte rows with
+// 3, each timed on the first column tile it would run;
 // the fastest per vector is kept in the context (tile width and two-step settings) when it beats
 // the single steps' `single_per_vector` milliseconds per vector and step by 2 %, else the
 // settings are left as they were.
 void autotune_two_step(const kpmx_mat* h2, const kpmx_block* r0, double single_per_vector) {
     kpmx_ctx* ctx = h2->ctx;
     const int sc = h2->scalar;
-    if (sc != KPMX_F64 || h2->format != KPMX_FMT_SELL || !h2->step_rows.empty() || h2->rows < 2 * 64) return;
+    if ((sc != KPMX_F64 && sc != KPMX_C128) || h2->format != KPMX_FMT_SELL || !h2->step_rows.empty() ||
+        h2->rows < 2 * 64)
+        return;
     const int saved[4] = {ctx->two_step[sc], ctx->two_step_wg_per_cu[sc], ctx->two_step_lds[sc], ctx->tile_cap[sc]};
     auto set = [&](int rows, int wg, int lds, int cap) {
         ctx->two_step[sc] = rows;
@@ -2205,15 +2213,17 @@ void autotune_two_step(const kpmx_mat* h2, const kpmx_block* r0, double single_p
         ctx->two_step_lds[sc] = lds;
         ctx->tile_cap[sc] = cap;
     };
-    // about 4 GB of vector traffic per timed sample of 8 columns (16 to 256 steps, even)
-    int64_t st = std::max<int64_t>(16, std::min<int64_t>(256, (int64_t)(4e9 / (3.0 * h2->rows * 8 * 8))));
+    // about 4 GB of vector traffic per timed sample of 64-byte rows (16 to 256 steps, even)
+    int64_t st = std::max<int64_t>(16, std::min<int64_t>(256, (int64_t)(4e9 / (3.0 * h2->rows * 64))));
     st -= st % 2;
     struct Cand {
         int cap, wg;
         bool ok;
         float ms;
     };
-    std::vector<Cand> cands{{8, 4, false, 0.f}, {16, 3, false, 0.f}};
+    // tiles of 64-byte rows on 4 workgroups per CU and of 128-byte rows on 3
+    const int narrow = sc == KPMX_C128 ? 4 : 8;
+    std::vector<Cand> cands{{narrow, 4, false, 0.f}, {2 * narrow, 3, false, 0.f}};
     // one untimed run of every candidate (code loading, workspaces), then three rounds that
     // alternate the candidates; each keeps its fastest sample
     for (int round = 0; round < 4; ++round) {

@@ -146,7 +146,7 @@ struct Step2Launch {
     int64_t n = 0, ntiles = 0, tile_rows = 0;
     int num_wg = 0;  // persistent grid (a multiple of 8); 0: one workgroup per tile
     double hscaleA = 1.0;
-    // The LDS form (k_step2_lds, lidx != nullptr; fp64 column pairs): tiles of STEP2_LDS_TILE
+    // The LDS form (k_step2_lds, lidx != nullptr; fp64 column pairs or complex128): tiles of STEP2_LDS_TILE
     // rows, one 64-row slice per wave; phase A keeps r_{k+1} of the tile's rows and of its at
     // most hmax halo rows in LDS and phase B gathers it there by lidx (every stored entry's column
     // as its position in the tile's rows followed by its halo rows); the wave keeps r_k of its
@@ -177,9 +177,10 @@ int step2_partial_width(int scalar, int vw);
 int launch_step2(const Step2Launch& S, hipStream_t stream);
 constexpr int STEP2_LDS_TILE = 256;
 constexpr size_t STEP2_LDS_MAX_BYTES = 65536;  // dynamic LDS of one workgroup of the LDS form
-// fp64 blocks of 8 or 16 vectors (4 or 8 column pairs); LDS bytes of one workgroup
+// fp64 blocks of 8 or 16 vectors (4 or 8 column pairs) and complex128 blocks of 4 or 8 vectors;
+// LDS bytes of one workgroup
 bool step2_lds_supported(int scalar, int vw);
-size_t step2_lds_bytes(int vw, int hmax);
+size_t step2_lds_bytes(int scalar, int vw, int hmax);
 // resident workgroups per CU of the LDS form (0 when it cannot run)
 int step2_lds_blocks_per_cu(const Step2Launch& S);
 

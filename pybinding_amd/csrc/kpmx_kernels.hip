@@ -1076,13 +1076,15 @@ __global__ __launch_bounds__(BLOCK) void k_step2_sell(Step2Args<T> a) {
 // r_{k+2} = H2 r_{k+1} - r_k on the slice,
 // gathering r_{k+1} from LDS by the tile-local column positions `lidx` (host-built) and taking
 // r_k from registers.  A second barrier ends the tile before the next one overwrites the LDS.
+template <class T>
 struct Step2LdsArgs {
+    using V = typename traits<T>::value;
     const int32_t* indices;  // global columns of the matrix form read
     const uint16_t* lidx;    // the same columns as positions in the tile's LDS rows
-    const double* vals;      // standard form
+    const V* vals;           // standard form
     const uint8_t* codes;    // value-table form: codes, dense diagonal, table
-    const double* diag;
-    const double* table;
+    const V* diag;
+    const V* table;
     int table_size;
     const int64_t* slice_ptr;
     const int32_t* slice_width;
@@ -1096,34 +1098,33 @@ struct Step2LdsArgs {
     const int32_t* hs_width;
     const int32_t* hrow;
     const int32_t* hidx;
-    const double* hval;
+    const V* hval;
     const uint8_t* hcode;
-    const double* hdiag;
-    const dpair* xprev;  // r_{k-1}; null on the first step
-    const dpair* x;      // r_k
-    dpair* out1;         // r_{k+1}
-    dpair* out2;         // r_{k+2}
+    const V* hdiag;
+    const T* xprev;  // r_{k-1}; null on the first step
+    const T* x;      // r_k
+    T* out1;         // r_{k+1}
+    T* out2;         // r_{k+2}
     double* partialsA;
     double* partialsB;
     int64_t n, ntiles;
     double hscaleA;
 };
 
-// MINW: waves per SIMD the register allocation must allow (one wave of each of MINW workgroups
-// per CU); 1 leaves the compiler free.
-template <int VW, int KMAX, bool CODED, int MINW>
-__global__ __launch_bounds__(BLOCK, MINW) void k_step2_lds(Step2LdsArgs a) {
-    using T = dpair;
-    using V = double;
+template <class T, int VW, int KMAX, bool CODED, int MINW>
+__global__ __launch_bounds__(BLOCK, MINW) void k_step2_lds(Step2LdsArgs<T> a) {
+    using V = typename traits<T>::value;
+    static_assert(sizeof(T) == 16, "the LDS two-step form takes 16-byte elements (fp64 pairs, complex128)");
     constexpr int RPW = WAVE / VW;  // rows per row group
     constexpr int U = VW;           // row groups per slice
     constexpr int UC = 2;           // row groups per pass
     static_assert(U % UC == 0 && WAVES_PER_BLOCK * WAVE == STEP2_LDS_TILE, "bad LDS two-step configuration");
     extern __shared__ __align__(16) unsigned char s_dyn[];
     T* const s_r = reinterpret_cast<T*>(s_dyn);  // (STEP2_LDS_TILE + hmax) rows x VW
-    __shared__ V s_tab[CODED ? 257 : 1];
+    __shared__ __align__(16) unsigned char s_tab_raw[CODED ? 257 * sizeof(V) : 16];
+    V* const s_tab = reinterpret_cast<V*>(s_tab_raw);
     if constexpr (CODED) {
-        for (int i = threadIdx.x; i < 257; i += BLOCK) s_tab[i] = i < a.table_size ? a.table[i] : 0.0;
+        for (int i = threadIdx.x; i < 257; i += BLOCK) s_tab[i] = i < a.table_size ? a.table[i] : zero(V{});
         __syncthreads();
     }
     const int lane = threadIdx.x % WAVE;
@@ -1151,7 +1152,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_step2_lds(Step2LdsArgs a) {
         const bool has_slice = slc * WAVE < tend;  // uniform over the wave
         int32_t c_idx[KMAX], c_loc[KMAX];
         V c_val[KMAX];
-        V c_diag = 0;
+        V c_diag = zero(V{});
         int64_t base = 0;
         int width = 0;
         T keep[U];
@@ -1169,7 +1170,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_step2_lds(Step2LdsArgs a) {
                 } else {
                     c_idx[k] = (int32_t)own;
                     c_loc[k] = wave * WAVE + lane;
-                    c_val[k] = 0;
+                    c_val[k] = zero(V{});
                 }
             }
             if constexpr (CODED) c_diag = a.diag[own];
@@ -1232,7 +1233,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_step2_lds(Step2LdsArgs a) {
 #pragma unroll
                     for (int k = 0; k < KMAX; ++k) {
                         const int64_t off = hb + (int64_t)k * WAVE + r;
-                        hv[k] = k < hw ? hvalue(off) : 0.0;
+                        hv[k] = k < hw ? hvalue(off) : zero(V{});
                         hx[k] = a.x[(int64_t)(k < hw ? a.hidx[off] : (int32_t)row) * VW + col];
                     }
                     T acc = zero(T{});
@@ -2344,19 +2345,24 @@ int launch_step2(const Step2Launch& S, hipStream_t stream) {
     }
 }
 
-bool step2_lds_supported(int scalar, int vw) { return scalar == 1 && (vw == 8 || vw == 16); }
+bool step2_lds_supported(int scalar, int vw) {
+    return (scalar == 1 && (vw == 8 || vw == 16)) || (scalar == 3 && (vw == 4 || vw == 8));
+}
 
-size_t step2_lds_bytes(int vw, int hmax) { return (size_t)(STEP2_LDS_TILE + hmax) * vw * sizeof(double); }
+size_t step2_lds_bytes(int scalar, int vw, int hmax) {
+    return (size_t)(STEP2_LDS_TILE + hmax) * vw * (scalar == 3 ? 16 : 8);
+}
 
-template <int VW>
+template <class T, int VW>
 static int step2_lds_t(const Step2Launch& S, hipStream_t stream, int* occupancy) {
-    Step2LdsArgs a{};
+    using V = typename traits<T>::value;
+    Step2LdsArgs<T> a{};
     a.indices = S.indices;
     a.lidx = S.lidx;
-    a.vals = static_cast<const double*>(S.vals);
+    a.vals = static_cast<const V*>(S.vals);
     a.codes = S.codes;
-    a.diag = static_cast<const double*>(S.diag);
-    a.table = static_cast<const double*>(S.table);
+    a.diag = static_cast<const V*>(S.diag);
+    a.table = static_cast<const V*>(S.table);
     a.table_size = S.table_size;
     a.slice_ptr = S.slice_ptr;
     a.slice_width = S.slice_width;
@@ -2366,19 +2372,19 @@ static int step2_lds_t(const Step2Launch& S, hipStream_t stream, int* occupancy)
     a.hs_width = S.hs_width;
     a.hrow = S.hrow;
     a.hidx = S.hidx;
-    a.hval = static_cast<const double*>(S.hval);
+    a.hval = static_cast<const V*>(S.hval);
     a.hcode = S.hcode;
-    a.hdiag = static_cast<const double*>(S.hdiag);
-    a.xprev = static_cast<const dpair*>(S.xprev);
-    a.x = static_cast<const dpair*>(S.x);
-    a.out1 = static_cast<dpair*>(S.out1);
-    a.out2 = static_cast<dpair*>(S.out2);
+    a.hdiag = static_cast<const V*>(S.hdiag);
+    a.xprev = static_cast<const T*>(S.xprev);
+    a.x = static_cast<const T*>(S.x);
+    a.out1 = static_cast<T*>(S.out1);
+    a.out2 = static_cast<T*>(S.out2);
     a.partialsA = S.partialsA;
     a.partialsB = S.partialsB;
     a.n = S.n;
     a.ntiles = S.ntiles;
     a.hscaleA = S.hscaleA;
-    const size_t lds = step2_lds_bytes(2 * VW, S.hmax);
+    const size_t lds = (size_t)(STEP2_LDS_TILE + S.hmax) * VW * sizeof(T);
     auto go = [&](auto kernel) -> int {
         if (occupancy) {
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(occupancy, kernel, BLOCK, lds) != hipSuccess) *occupancy = 0;
@@ -2388,22 +2394,24 @@ static int step2_lds_t(const Step2Launch& S, hipStream_t stream, int* occupancy)
         return 0;
     };
     const bool coded = S.codes != nullptr;
-    if (S.max_width > 4) return coded ? go(k_step2_lds<VW, 8, true, 1>) : go(k_step2_lds<VW, 8, false, 1>);
+    if (S.max_width > 4) return coded ? go(k_step2_lds<T, VW, 8, true, 1>) : go(k_step2_lds<T, VW, 8, false, 1>);
     switch (S.min_waves) {  // the register budget for that many workgroups per CU
-        case 2: return coded ? go(k_step2_lds<VW, 4, true, 2>) : go(k_step2_lds<VW, 4, false, 2>);
-        case 3: return coded ? go(k_step2_lds<VW, 4, true, 3>) : go(k_step2_lds<VW, 4, false, 3>);
-        case 4: return coded ? go(k_step2_lds<VW, 4, true, 4>) : go(k_step2_lds<VW, 4, false, 4>);
-        case 5: return coded ? go(k_step2_lds<VW, 4, true, 5>) : go(k_step2_lds<VW, 4, false, 5>);
-        case 6: return coded ? go(k_step2_lds<VW, 4, true, 6>) : go(k_step2_lds<VW, 4, false, 6>);
-        default: return coded ? go(k_step2_lds<VW, 4, true, 1>) : go(k_step2_lds<VW, 4, false, 1>);
+        case 2: return coded ? go(k_step2_lds<T, VW, 4, true, 2>) : go(k_step2_lds<T, VW, 4, false, 2>);
+        case 3: return coded ? go(k_step2_lds<T, VW, 4, true, 3>) : go(k_step2_lds<T, VW, 4, false, 3>);
+        case 4: return coded ? go(k_step2_lds<T, VW, 4, true, 4>) : go(k_step2_lds<T, VW, 4, false, 4>);
+        case 5: return coded ? go(k_step2_lds<T, VW, 4, true, 5>) : go(k_step2_lds<T, VW, 4, false, 5>);
+        case 6: return coded ? go(k_step2_lds<T, VW, 4, true, 6>) : go(k_step2_lds<T, VW, 4, false, 6>);
+        default: return coded ? go(k_step2_lds<T, VW, 4, true, 1>) : go(k_step2_lds<T, VW, 4, false, 1>);
     }
 }
 
 static int step2_lds_dispatch(const Step2Launch& S, hipStream_t stream, int* occupancy) {
-    if (!step2_lds_supported(S.scalar, S.vw) || S.hmax < 0 || step2_lds_bytes(S.vw, S.hmax) > STEP2_LDS_MAX_BYTES ||
-        !S.hs_first || !S.hs_ptr || !S.hs_width || !S.hrow || !S.hidx || (S.codes ? !S.hcode || !S.hdiag : !S.hval))
+    if (!step2_lds_supported(S.scalar, S.vw) || S.hmax < 0 ||
+        step2_lds_bytes(S.scalar, S.vw, S.hmax) > STEP2_LDS_MAX_BYTES || !S.hs_first || !S.hs_ptr || !S.hs_width ||
+        !S.hrow || !S.hidx || (S.codes ? !S.hcode || !S.hdiag : !S.hval))
         return -1;
-    return S.vw == 8 ? step2_lds_t<4>(S, stream, occupancy) : step2_lds_t<8>(S, stream, occupancy);
+    if (S.scalar == 1) return S.vw == 8 ? step2_lds_t<dpair, 4>(S, stream, occupancy) : step2_lds_t<dpair, 8>(S, stream, occupancy);
+    return S.vw == 4 ? step2_lds_t<cf64, 4>(S, stream, occupancy) : step2_lds_t<cf64, 8>(S, stream, occupancy);
 }
 
 int step2_lds_blocks_per_cu(const Step2Launch& S) {

@@ -115,33 +115,39 @@ def random_hoppings(h2, seed=3):
     return h
 
 
+LDS_CASES = [(np.float64, nvec) for nvec in (16, 8, 12, 20)] + [(np.complex128, nvec) for nvec in (4, 8, 6, 12)]
+
+
 @pytest.mark.parametrize("shape", [(40, 28), (21, 13)])
 @pytest.mark.parametrize("coded", [True, False])
-@pytest.mark.parametrize("nvec", [16, 8, 12, 20])
+@pytest.mark.parametrize("dtype,nvec", LDS_CASES)
 @pytest.mark.parametrize("M", [130, 132])
-def test_two_step_lds_matches_oracle(oracle, shape, coded, nvec, M):
+def test_two_step_lds_matches_oracle(oracle, shape, coded, dtype, nvec, M):
     """The LDS form (kpmx_ctx_set_two_step_lds): 256-row tiles with the intermediate vector in LDS,
     on the value-table and the standard matrix form, tiles that end inside the matrix's last slice
-    or leave waves without a slice, 8 and 16 columns (20 vectors: a 16-column tile in the LDS form
-    and a 4-column one on single steps), odd and even step counts, the resident grid and one
+    or leave waves without a slice, fp64 with 8 and 16 columns (20 vectors: a 16-column tile in the
+    LDS form and a 4-column one on single steps) and complex128 with 4 and 8 (12 vectors: an
+    8-column tile and a 4-column one), odd and even step counts, the resident grid and one
     workgroup per CU and the register budgets for 3 and 4: within rounding of single steps and of
     the oracle."""
-    h2 = matrix(np.float64, *shape)
+    h2 = matrix(dtype, *shape)
     if not coded:
         h2 = random_hoppings(h2)
-    r0 = oracle.random_starter(h2.shape[0], nvec, np.float64)
+    r0 = oracle.random_starter(h2.shape[0], nvec, dtype)
     ref = oracle.diag_moments(h2, r0, M)
     ctx = abi.Ctx(0)
     try:
+        if dtype == np.complex128:
+            ctx.set_tile_width(dtype, 8)
         mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
         assert (mat.coded_bytes() > 0) == coded
         single = abi.moments_diagonal(mat, blk, M)
         out = {}
         for wg in (0, 1, 3, 4):  # 1, 3, 4: register budgets for that many workgroups per CU
-            ctx.set_two_step_lds(wg)
-            assert ctx.two_step(np.float64) == 256 and ctx.two_step_lds() == (True, wg)
+            ctx.set_two_step_lds(wg, dtype)
+            assert ctx.two_step(dtype) == 256 and ctx.two_step_lds(dtype) == (True, wg)
             out[wg] = abi.moments_diagonal(mat, blk, M)
-        ctx.set_two_step(np.float64, 0)
+        ctx.set_two_step(dtype, 0)
         np.testing.assert_array_equal(abi.moments_diagonal(mat, blk, M), single)
         blk.close()
         mat.close()
@@ -183,7 +189,7 @@ def test_two_step_lds_bfs_order_and_graph_replay(oracle):
         m32.close()
         import ctypes
         from pybinding_amd import abi as pabi
-        with pytest.raises(RuntimeError, match="fp64 only"):
+        with pytest.raises(RuntimeError, match="fp64 or complex128 only"):
             pabi._check(pabi.lib().kpmx_ctx_set_two_step_lds(ctx.h, ctypes.c_int(0), ctypes.c_int(0)))
     finally:
         ctx.close()
@@ -194,24 +200,25 @@ def test_two_step_lds_bfs_order_and_graph_replay(oracle):
     assert np.max(np.abs(mu32 - ref32) / np.abs(ref32[:, :1])) <= 1e-4
 
 
-def test_autotuner_two_step_choice(oracle):
+@pytest.mark.parametrize("dtype,widths,default", [(np.float64, (8, 16), 16), (np.complex128, (4, 8), 64)])
+def test_autotuner_two_step_choice(oracle, dtype, widths, default):
     """The autotuner times the LDS two-step form against single steps and keeps whichever is
     faster; either way the moments match the oracle, a later autotuning starts again from the
     defaults, and with the choice switched off the settings stay single steps."""
-    h2 = matrix(np.float64, 96, 64)
-    r0 = oracle.random_starter(h2.shape[0], 32, np.float64)
+    h2 = matrix(dtype, 96, 64)
+    r0 = oracle.random_starter(h2.shape[0], 32, dtype)
     M = 66
     ref = oracle.diag_moments(h2, r0, M)
     ctx = abi.Ctx(0)
     try:
         mat, blk = ctx.mat(h2, fmt=1), ctx.block(r0)
         abi.autotune_diagonal(mat, blk)
-        chosen = ctx.two_step_lds()
-        assert ctx.tile_width(np.float64) in ((8, 16) if chosen[0] else (16,))
+        chosen = ctx.two_step_lds(dtype)
+        assert ctx.tile_width(dtype) in (widths if chosen[0] else (default,))
         mu = abi.moments_diagonal(mat, blk, M)
         ctx.set_autotune_two_step(False)
         abi.autotune_diagonal(mat, blk)
-        assert ctx.two_step_lds() == (False, 0) and ctx.tile_width(np.float64) == 16
+        assert ctx.two_step_lds(dtype) == (False, 0) and ctx.tile_width(dtype) == default
         single = abi.moments_diagonal(mat, blk, M)
         blk.close()
         mat.close()
