@@ -207,8 +207,11 @@ int kpmx_moments_diagonal_fetch(const kpmx_mat* h2, const kpmx_block* r0, int64_
  * 64 MiB per GPU (held by the process until it ends): 2 x n x (vector width) scalars (3 x when
  * the block is run as column sub-tiles, see kpmx_ctx_set_tile_width), plus (num_moments / 2) x
  * 4 x (vector width) doubles of sums, plus num_vectors x num_moments scalars, plus one step's
- * partial sums, must fit, otherwise KPMX_EINVAL with the sizes in the message.  No device memory is allocated or freed (h2 and r0 are uploaded beforehand).  Calls
- * that overlap on one GPU share the array and race. */
+ * partial sums, must fit, otherwise KPMX_EINVAL with the sizes in the message.  No device memory
+ * is allocated or freed (h2 and r0 are uploaded beforehand).  It always runs single steps (the
+ * two-step forms need workspaces of their own), so where kpmx_moments_diagonal runs two steps per
+ * launch the two results agree to rounding rather than bit for bit.  Calls that overlap on one
+ * GPU share the array and race. */
 int kpmx_moments_diagonal_noalloc(const kpmx_mat* h2, const kpmx_block* r0, int64_t num_moments, void* mu);
 
 /* GenericMoments: mu_n = <beta| op T_n(H~) |r0> for the first vector of r0
