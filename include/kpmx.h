@@ -293,8 +293,8 @@ int kpmx_batch_set_partition(kpmx_batch* b, int num_workgroups, int walk_chunk);
  * per system, the diagonal as a dense vector over the rows -- is built at upload when every
  * system's off-diagonal values fit one table and no row holds two diagonal entries, and is then
  * the default.  enable = 0 selects the standard form (4-byte column + value per stored entry),
- * 1 the value-table form (KPMX_EINVAL when the batch has none).  The persistent run (run mode 2)
- * always reads the standard form.  Results agree to rounding (the diagonal term is added last). */
+ * 1 the value-table form (KPMX_EINVAL when the batch has none), under every run mode.  Results
+ * agree to rounding (the diagonal term is added last). */
 int kpmx_batch_set_value_table(kpmx_batch* b, int enable);
 /* Column tiles of the batch's vectors (no counterpart in the reference): the next
  * kpmx_batch_starters splits its num_vectors columns into tiles of at most `width` vectors

@@ -2575,7 +2575,7 @@ int kpmx_batch_starters(kpmx_batch* b, int64_t num_vectors, const void* r0) {
 
 namespace {
 // The step launch of a batch in its current form; `standard` asks for the standard form whatever
-// is selected (the persistent run has no value-table path).
+// is selected.
 void batch_launch(const kpmx_batch* b, BatchedLaunch& B, bool standard = false) {
     B = BatchedLaunch{};
     B.scalar = b->scalar;
@@ -2647,7 +2647,7 @@ namespace {
 void batch_run_persistent(kpmx_batch* b, int64_t steps) {
     kpmx_ctx* ctx = b->ctx;
     BatchedLaunch B;
-    batch_launch(b, B, true);
+    batch_launch(b, B);
     const int per_cu = batched_persistent_blocks_per_cu(B);
     if ((int64_t)b->num_wg > (int64_t)per_cu * ctx->num_cu)
         throw Error{KPMX_EINVAL, "persistent batched run: the partition's " + std::to_string(b->num_wg) +

@@ -2441,6 +2441,12 @@ static int batch_persistent_t(const BatchedLaunch& B, hipStream_t stream, int* o
     p.steps = B.steps;
     p.width = B.width;
     p.barrier = B.barrier;
+    using V = typename traits<T>::value;
+    CodedBatch<V> cb{};
+    cb.codes = B.codes;
+    cb.diag = static_cast<const V*>(B.diag);
+    cb.table = static_cast<const V*>(B.table);
+    cb.tab_off = B.tab_off;
     const bool off32 = (uint64_t)B.n * VW * sizeof(T) < ((uint64_t)1 << 32);
     auto go = [&](auto kernel) -> int {
         if (occupancy) {
@@ -2449,11 +2455,14 @@ static int batch_persistent_t(const BatchedLaunch& B, hipStream_t stream, int* o
         }
         hipLaunchKernelGGL(k_reset_barrier, dim3(1), dim3(64), 0, stream, B.barrier);
         hipLaunchKernelGGL(kernel, dim3(B.num_wg), dim3(BLOCK), 0, stream, p, B.slice_ptr, B.slice_width, B.slice_sys,
-                           B.tile_begin, B.slot_begin, B.slot_index, B.nslices, B.chunk,
-                           CodedBatch<typename traits<T>::value>{});
+                           B.tile_begin, B.slot_begin, B.slot_index, B.nslices, B.chunk, cb);
         return 0;
     };
-    if (B.codes) return -1;  // the persistent form runs the standard matrix form only
+    if (B.codes) {  // the value-table form
+        if (B.max_width <= 4)
+            return off32 ? go(k_batch_persistent<T, VW, 4, true, true>) : go(k_batch_persistent<T, VW, 4, false, true>);
+        return off32 ? go(k_batch_persistent<T, VW, 8, true, true>) : go(k_batch_persistent<T, VW, 8, false, true>);
+    }
     if (B.max_width <= 4)
         return off32 ? go(k_batch_persistent<T, VW, 4, true, false>) : go(k_batch_persistent<T, VW, 4, false, false>);
     if (B.max_width == 5)
