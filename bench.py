@@ -693,6 +693,20 @@ def main():
                              "step_traffic_over_triad": step_traffic_gbps / tg}
     log("stream probe: copy %.0f GB/s, read %.0f GB/s, triad (b = a + b) %.0f GB/s; step traffic at %.2f of the triad"
         % (cg, rg, tg, step_traffic_gbps / tg))
+    if two_step:
+        # the same computation with single steps (the default column tiles, the autotuned single-step
+        # variant), timed in the same run for comparison
+        ctx.set_two_step(dtype, 0)
+        ctx.set_tile_width(dtype, 16 if dtype == np.float64 else 64)
+        abi.moments_diagonal_run(mat, blk, M)
+        ctx.synchronize()
+        ctx.timer_start()
+        abi.moments_diagonal_run(mat, blk, M)
+        ms_single = ctx.timer_stop()
+        extra["single_steps"] = {"ms_per_step": ms_single / steps_per_run, "value": nnz * M * R / (ms_single / 1000) / 1e9,
+                                 "two_step_speedup": ms_single / (gpu_ms / args.steps)}
+        log("single steps on the same box: %.4f ms per step (the two-step form %.2fx faster)"
+            % (ms_single / steps_per_run, ms_single / (gpu_ms / args.steps)))
     if not args.no_plain:
         # the plain single-vector path (one vector, SELL-64 kernel) on the same matrix, timed over
         # 512 moments and reported in the same metric
